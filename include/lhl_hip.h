@@ -1,0 +1,156 @@
+/*
+ * lhl_hip.h -- C ABI of the MI355X (gfx950) implementation of the LHL hot path:
+ * the ELMHES similarity reduction J = Z H Z^-1 (Z = D P L), the per-shift
+ * Hessenberg LU of sigma*I + tau*H, and the Z / U solves with refinement.
+ *
+ * Every entry point replaces one public name of SciML/LHL.jl
+ * (src/LHLFactorization.jl, "src:LINE" below) and is what a Julia `ccall`
+ * wrapper binds.  Conventions:
+ *   - every call returns an int status: LHL_OK (0), or an error code; the
+ *     message is in lhl_last_error(h).  LHL_ERR_ARG maps to ArgumentError,
+ *     LHL_ERR_DIM to DimensionMismatch, the rest to ErrorException.
+ *   - a zero pivot of the shift LU is not an error: it is reported in *info
+ *     (1-based index of the first zero pivot, 0 if none), as src:2450.
+ *   - matrices are column-major.  Indices returned by the getters are 0-based.
+ *   - complex vectors are interleaved (re, im) pairs, Julia's ComplexF64 /
+ *     ComplexF32 memory layout.
+ *   - host pointers are borrowed for the duration of a call, which is
+ *     synchronous with respect to them; device pointers (flag *_on_device) must
+ *     live on the handle's device.
+ *   - the handle owns its device buffers and its HIP stream.  A handle is not
+ *     thread-safe; separate handles may be used concurrently (src:141-142).
+ */
+#ifndef LHL_HIP_H
+#define LHL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lhl_handle lhl_handle;       /* LHLWorkspace{T,Tr,TG}, src:144-162 */
+typedef struct lhl_shift_state lhl_shift_state;   /* LHLShift{TG,Tr}, src:84-96 */
+
+enum {
+    LHL_OK = 0,
+    LHL_ERR_ARG = 1,    /* ArgumentError */
+    LHL_ERR_DIM = 2,    /* DimensionMismatch */
+    LHL_ERR_HIP = 3,    /* a HIP runtime error */
+    LHL_ERR_STATE = 4,  /* call out of order (e.g. shift before reduce) */
+    LHL_ERR_NOMEM = 5
+};
+
+enum { LHL_F64 = 0, LHL_F32 = 1 };
+
+/* Working precision of a batched solve. */
+enum {
+    LHL_PREC_WORKING = 0,  /* plain solves in the handle's precision */
+    LHL_PREC_REFINE64 = 1  /* solves in the handle's precision, FP64 residual refinement to tol */
+};
+
+/* Library version and the number of visible devices (no GPU work). */
+int lhl_version(void);
+
+/* LHLWorkspace{T}(n; shift) (src:164-178) with T = dtype (LHL_F64/LHL_F32)
+ * and shift = T (shift_complex = 0) or Complex{T} (1).  Allocates on the
+ * current HIP device. */
+int lhl_create(lhl_handle **h, int64_t n, int dtype, int shift_complex);
+int lhl_destroy(lhl_handle *h);
+const char *lhl_last_error(const lhl_handle *h);
+
+/* lhl_reduce!(ws, J, balance) / lhl!(ws, J; balance) (src:355-390, 3192-3195).
+ * J (n x n, leading dimension ldj) is not modified.  j_dtype is the element
+ * type of J (LHL_F64 or LHL_F32); an FP64 J on an FP32 handle is balanced in
+ * FP64, reduced in FP32, and kept resident in FP64 for lhl_refine's residual.
+ * n may differ from the handle's current size: the handle resizes
+ * (src:256-275), and held shift states follow on their next lhl_shift. */
+int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype,
+               int balance, int j_on_device);
+
+/* LHLShift{TG}(ws) (src:107-116, 180): an extra shift state on the handle's
+ * reduction; is_complex selects TG = Complex{T}.  The handle's own state
+ * (ws.shift) is used when a NULL shift state is passed below. */
+int lhl_shift_create(lhl_handle *h, lhl_shift_state **sh, int is_complex);
+int lhl_shift_destroy(lhl_shift_state *sh);
+
+/* lhl_shift!(ws, sigma, tau) / lhl_shift!(sh, ws, sigma, tau) (src:2456-2489):
+ * LU of sigma*I + tau*H.  A complex sigma or tau on a real shift state is
+ * LHL_ERR_ARG (src:2466-2473).  *info receives the first zero pivot (1-based)
+ * or 0. */
+int lhl_shift(lhl_handle *h, lhl_shift_state *sh, double sigma_re, double sigma_im,
+              double tau_re, double tau_im, int64_t *info);
+
+/* lhl_ldiv!(x, ws) / lhl_ldiv!(x, sh, ws) (src:3052-3123): x <- W^-1 x for
+ * nrhs right-hand sides (columns of x, leading dimension ldx).  x_complex says
+ * x holds interleaved complex values; a real x with a complex shift state is
+ * LHL_ERR_ARG (src:3084-3085).  x_dtype is the element type of x's real
+ * parts. */
+int lhl_ldiv(lhl_handle *h, lhl_shift_state *sh, void *x, int64_t n, int64_t nrhs,
+             int64_t ldx, int x_dtype, int x_complex, int x_on_device);
+
+/* lhl_refine!(x, A, b, ws, steps) / lhl_refine!(x, A, b, sh, ws, steps)
+ * (src:3144-3157): up to `steps` rounds of r = b - W x ; ldiv(r) ; x += r.
+ * W is the dense operator (n x n, ldw, element type w_dtype, complex if
+ * x_complex) or NULL, which means sigma*I + tau*J with the handle's resident J
+ * (FP64 when the handle was reduced from an FP64 J).  With tol > 0 the loop
+ * stops once every column has ||b - W x||_inf / ||b||_inf <= tol; *iters
+ * (may be NULL) receives the rounds done.  The residual and the update are
+ * computed in the precision of x. */
+int lhl_refine(lhl_handle *h, lhl_shift_state *sh, void *x, const void *W, int64_t ldw,
+               const void *b, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype, int x_complex,
+               int on_device, int steps, double tol, int *iters);
+
+/* Many shifts and right-hand sides in one call (no reference counterpart as a
+ * single name: it is the loop `for (sigma,tau): lhl_shift!; lhl_ldiv!;
+ * lhl_refine!` of README.md:47-51 batched).  Shift s is (sigma[2s] +
+ * i sigma[2s+1], tau[2s] + i tau[2s+1]); shifts_complex selects complex shifts
+ * (on a real reduction).  Its right-hand sides are columns
+ * rhs_offsets[s] .. rhs_offsets[s+1]-1 of B (FP64, interleaved complex when
+ * shifts_complex, ldb) and the solutions go to the same columns of X (FP64,
+ * ldx).  precision: LHL_PREC_WORKING or LHL_PREC_REFINE64 (refine with the
+ * FP64 J until tol, at most max_iter rounds).  info[s] (may be NULL) receives
+ * each shift's zero-pivot index, iters[s] (may be NULL) its refinement rounds.
+ * B and X are host or device memory (on_device). */
+int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, const double *tau,
+                            int shifts_complex, const int64_t *rhs_offsets, const double *B,
+                            int64_t ldb, double *X, int64_t ldx, int on_device, int precision,
+                            double tol, int max_iter, int64_t *info, int *iters);
+
+/* applyZ!(x, ws) / applyZinv!(x, ws) (src:2069-2126) on nrhs columns of x
+ * (element type x_dtype; complex interleaved if x_complex). */
+int lhl_applyZ(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype,
+               int x_complex, int x_on_device);
+int lhl_applyZinv(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype,
+                  int x_complex, int x_on_device);
+
+/* ---- getters (host copies, for parity checks and the wrapper's properties) ---- */
+
+/* n, element type, shift element type and whether a reduction is held
+ * (ws.reduced, src:158-160). */
+int lhl_get_info(const lhl_handle *h, int64_t *n, int *dtype, int *shift_complex, int *reduced);
+/* ws.ipiv (n-2 entries, 0-based pivot rows), ws.perm, ws.iperm (n, 0-based). */
+int lhl_get_ipiv(lhl_handle *h, int32_t *ipiv);
+int lhl_get_perm(lhl_handle *h, int32_t *perm, int32_t *iperm);
+/* ws.scale, ws.iscale as doubles (powers of two). */
+int lhl_get_scale(lhl_handle *h, double *scale, double *iscale);
+/* ws.factors (n x n, ld = n, handle's element type): H in triu(.,-1), step-c
+ * multipliers in column c below the subdiagonal (src:127-129). */
+int lhl_get_factors(lhl_handle *h, void *factors);
+/* ws.Ht: H transposed, n x n with ld = n; entries below the first
+ * superdiagonal of Ht (i > j+1 of H) are zero. */
+int lhl_get_ht(lhl_handle *h, void *ht);
+/* The shift state (sh NULL = ws.shift): U (n x n, ld = n, U[k, j] at
+ * index k + j*n, zero below the diagonal), the multipliers l (n-1), swap
+ * flags (n), rdiag (n), sigma, tau, info.  For a complex state the
+ * arrays hold interleaved complex values. */
+int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh, void *U, void *l, uint8_t *swap,
+                        void *rdiag, double *sigma, double *tau, int64_t *info);
+
+/* Synchronize the handle's stream. */
+int lhl_sync(lhl_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

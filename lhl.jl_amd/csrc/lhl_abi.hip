@@ -1,0 +1,646 @@
+// The C ABI of liblhl_hip.so (declared in include/lhl_hip.h): argument and
+// state checks with the reference's error classes, staging of host/device
+// vectors into the planar device layout the kernels use, and the refinement
+// and batching loops.
+#include "lhl_common.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+using namespace lhl;
+
+namespace {
+
+int fail(lhl_handle *h, const Status &s) {
+    if (h) h->err = s.msg;
+    return s.code;
+}
+int fail(lhl_handle *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+Status alloc_sized(lhl_handle *h, int64_t n) {
+    const size_t es = elt_size(h->dtype);
+    h->lda = pad_ld(n, h->dtype);
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    LHL_TRY(h->A.ensure(es * (size_t)h->lda * nn));
+    LHL_TRY(h->Ht.ensure(es * (size_t)h->lda * nn));
+    LHL_TRY(h->scale64.ensure(8 * nn));
+    LHL_TRY(h->iscale64.ensure(8 * nn));
+    LHL_TRY(h->scaleT.ensure(es * nn));
+    LHL_TRY(h->iscaleT.ensure(es * nn));
+    LHL_TRY(h->ipiv.ensure(4 * nn));
+    LHL_TRY(h->perm.ensure(4 * nn));
+    LHL_TRY(h->iperm.ensure(4 * nn));
+    h->n = n;
+    return {};
+}
+
+Status set_device(lhl_handle *h) {
+    LHL_HIP_TRY(hipSetDevice(h->device));
+    return {};
+}
+
+// Stage nrhs columns of x (host or device; interleaved complex when complex)
+// into a planar device buffer of element type sdt.  Returns ld and plane.
+Status stage_in(lhl_handle *h, DevBuf &buf, const void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
+                int xcplx, int on_device, int sdt, int64_t &ld, int64_t &plane) {
+    const size_t xes = elt_size(xdt) * (xcplx ? 2 : 1);
+    const int planes = xcplx ? 2 : 1;
+    ld = std::max<int64_t>(n, 1);
+    plane = ld * nrhs;
+    LHL_TRY(buf.ensure(elt_size(sdt) * (size_t)plane * planes + 64));
+    if (on_device) return to_planar(h, x, ldx, xdt, xcplx, buf.p, ld, plane, sdt, n, nrhs);
+    // host: copy the raw columns to a device staging area first
+    LHL_TRY(h->bstage.ensure(xes * (size_t)n * nrhs + 64));
+    LHL_HIP_TRY(hipMemcpy2DAsync(h->bstage.p, xes * n, x, xes * ldx, xes * n, nrhs, hipMemcpyHostToDevice, h->stream));
+    return to_planar(h, h->bstage.p, n, xdt, xcplx, buf.p, ld, plane, sdt, n, nrhs);
+}
+
+Status stage_out(lhl_handle *h, const DevBuf &buf, int64_t ld, int64_t plane, int sdt, void *x, int64_t n,
+                 int64_t nrhs, int64_t ldx, int xdt, int xcplx, int on_device) {
+    const size_t xes = elt_size(xdt) * (xcplx ? 2 : 1);
+    if (on_device) return from_planar(h, buf.p, ld, plane, sdt, x, ldx, xdt, xcplx, n, nrhs);
+    LHL_TRY(h->bstage.ensure(xes * (size_t)n * nrhs + 64));
+    LHL_TRY(from_planar(h, buf.p, ld, plane, sdt, h->bstage.p, n, xdt, xcplx, n, nrhs));
+    LHL_HIP_TRY(hipMemcpy2DAsync(x, xes * ldx, h->bstage.p, xes * n, xes * n, nrhs, hipMemcpyDeviceToHost, h->stream));
+    LHL_HIP_TRY(hipStreamSynchronize(h->stream));
+    return {};
+}
+
+int shift_planes_lstride(const lhl_shift_state *sh, int64_t n, int64_t &us, int64_t &ls, int64_t &ss) {
+    const int64_t uplane = ((upack_len(n) + 63) / 64) * 64;
+    const int64_t lplane = ((n + 63) / 64) * 64;
+    us = uplane * sh->planes;
+    ls = lplane * sh->planes;
+    ss = lplane;
+    return 0;
+}
+
+// the working type of the staged vectors: FP64 unless both the factors and x are FP32
+int stage_dtype(const lhl_handle *h, int xdt) { return (h->dtype == LHL_F32 && xdt == LHL_F32) ? LHL_F32 : LHL_F64; }
+
+Status do_solve_staged(lhl_handle *h, lhl_shift_state *sh, const void *xin, void *xout, int64_t ld, int64_t plane,
+                       int64_t nrhs, int sdt, int x_complex, int accumulate) {
+    SolveArgs a;
+    int64_t us, ls, ss;
+    shift_planes_lstride(sh, h->n, us, ls, ss);
+    a.U = sh->U.p;
+    a.l = sh->l.p;
+    a.rdiag = sh->rdiag.p;
+    a.swp = sh->swp.as<uint8_t>();
+    a.ustride = us;
+    a.lstride = ls;
+    a.sstride = ss;
+    a.shift_complex = sh->is_complex;
+    a.nshift = 1;
+    a.col_begin = nullptr;
+    a.ncol = nrhs;
+    a.x_complex = x_complex;
+    a.xdt = sdt;
+    a.xin = xin;
+    a.xout = xout;
+    a.ldx = ld;
+    a.xplane = plane;
+    a.accumulate = accumulate;
+    return solve_device(h, a);
+}
+
+lhl_shift_state *resolve_shift(lhl_handle *h, lhl_shift_state *sh) { return sh ? sh : h->own; }
+
+}  // namespace
+
+extern "C" {
+
+int lhl_version(void) { return 1; }
+
+const char *lhl_last_error(const lhl_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int lhl_create(lhl_handle **out, int64_t n, int dtype, int shift_complex) {
+    if (!out) return LHL_ERR_ARG;
+    *out = nullptr;
+    if (n < 0) return LHL_ERR_DIM;
+    if (dtype != LHL_F64 && dtype != LHL_F32) return LHL_ERR_ARG;
+    lhl_handle *h = new (std::nothrow) lhl_handle();
+    if (!h) return LHL_ERR_NOMEM;
+    h->dtype = dtype;
+    h->shift_complex = shift_complex ? 1 : 0;
+    if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return LHL_ERR_HIP;
+    }
+    Status st = alloc_sized(h, n);
+    if (!st.ok()) { int c = st.code; lhl_destroy(h); return c; }
+    h->own = new lhl_shift_state();
+    h->own->h = h;
+    h->own->is_complex = h->shift_complex;
+    h->own->planes = h->shift_complex ? 2 : 1;
+    h->own->dtype = dtype;
+    *out = h;
+    return LHL_OK;
+}
+
+int lhl_destroy(lhl_handle *h) {
+    if (!h) return LHL_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->ybuf, &h->rbuf, &h->nbuf,
+                      &h->xstage, &h->bstage})
+        b->release();
+    auto drop = [](lhl_shift_state *s) {
+        for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo}) b->release();
+    };
+    if (h->own) { drop(h->own); delete h->own; }
+    for (lhl_shift_state *s : h->held) s->h = nullptr;   // held states outlive nothing: detach
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LHL_OK;
+}
+
+int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int balance, int j_on_device) {
+    if (!h) return LHL_ERR_ARG;
+    if (n < 0) return fail(h, LHL_ERR_DIM, "lhl_reduce: negative size");
+    if (j_dtype != LHL_F64 && j_dtype != LHL_F32) return fail(h, LHL_ERR_ARG, "lhl_reduce: J must be Float64 or Float32");
+    if (n > 0 && (!J || ldj < n)) return fail(h, LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n");
+    if (n > 8192 + 8192) return fail(h, LHL_ERR_ARG, "lhl_reduce: n above 16384 is not supported");
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    if (n != h->n) {
+        st = alloc_sized(h, n);
+        if (!st.ok()) return fail(h, st);
+    }
+    h->reduced = 0;
+    h->own->loaded = 0;
+    // the resident J: FP64 unless J is FP32 on an FP32 handle
+    const int rdt = (j_dtype == LHL_F32 && h->dtype == LHL_F32) ? LHL_F32 : LHL_F64;
+    h->j_dtype = rdt;
+    h->ldj = pad_ld(n, rdt);
+    st = h->J.ensure(elt_size(rdt) * (size_t)h->ldj * (size_t)std::max<int64_t>(n, 1));
+    if (!st.ok()) return fail(h, st);
+    if (n > 0) {
+        if (j_dtype == rdt) {
+            const size_t es = elt_size(rdt);
+            hipError_t e = hipMemcpy2DAsync(h->J.p, es * h->ldj, J, es * ldj, es * n, n,
+                                            j_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream);
+            if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: copy of J: ") + hipGetErrorString(e));
+        } else {
+            int64_t ld, plane;
+            st = stage_in(h, h->xstage, J, n, n, ldj, j_dtype, 0, j_on_device, rdt, ld, plane);
+            if (st.ok()) {
+                hipError_t e = hipMemcpy2DAsync(h->J.p, 8 * h->ldj, h->xstage.p, 8 * ld, 8 * n, n,
+                                                hipMemcpyDeviceToDevice, h->stream);
+                if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+            }
+            if (!st.ok()) return fail(h, st);
+        }
+    }
+    st = reduce_device(h, balance);
+    if (!st.ok()) return fail(h, st);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: ") + hipGetErrorString(e));
+    h->reduced = 1;
+    return LHL_OK;
+}
+
+int lhl_shift_create(lhl_handle *h, lhl_shift_state **out, int is_complex) {
+    if (!h || !out) return LHL_ERR_ARG;
+    lhl_shift_state *s = new (std::nothrow) lhl_shift_state();
+    if (!s) return fail(h, LHL_ERR_NOMEM, "lhl_shift_create: out of memory");
+    s->h = h;
+    s->is_complex = is_complex ? 1 : 0;
+    s->planes = s->is_complex ? 2 : 1;
+    s->dtype = h->dtype;
+    s->n = h->n;
+    h->held.push_back(s);
+    *out = s;
+    return LHL_OK;
+}
+
+int lhl_shift_destroy(lhl_shift_state *s) {
+    if (!s) return LHL_OK;
+    for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo}) b->release();
+    if (s->h) {
+        auto &v = s->h->held;
+        v.erase(std::remove(v.begin(), v.end(), s), v.end());
+    }
+    delete s;
+    return LHL_OK;
+}
+
+int lhl_shift(lhl_handle *h, lhl_shift_state *sh_, double sr, double si, double tr, double ti, int64_t *info) {
+    if (!h) return LHL_ERR_ARG;
+    lhl_shift_state *sh = resolve_shift(h, sh_);
+    if (sh->h != h) return fail(h, LHL_ERR_ARG, "lhl_shift: the shift state belongs to another workspace");
+    if (!sh->is_complex && (si != 0.0 || ti != 0.0))
+        return fail(h, LHL_ERR_ARG, "complex shift on a real shift state; build the workspace with a complex shift type "
+                                    "or use a complex LHLShift");
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "lhl_shift: no reduction held (call lhl_reduce first)");
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const double sig[2] = {sr, si}, tau[2] = {tr, ti};
+    st = shift_device(h, sh, 1, sig, tau, info);
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
+int lhl_ldiv(lhl_handle *h, lhl_shift_state *sh_, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
+             int x_complex, int on_device) {
+    if (!h) return LHL_ERR_ARG;
+    lhl_shift_state *sh = resolve_shift(h, sh_);
+    if (n != h->n)
+        return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
+                                        "x" + std::to_string(h->n));
+    if (sh->h != h) return fail(h, LHL_ERR_ARG, "lhl_ldiv: the shift state belongs to another workspace");
+    if (sh->n != h->n)
+        return fail(h, LHL_ERR_DIM, "the LHLShift is " + std::to_string(sh->n) + "x" + std::to_string(sh->n) +
+                                        ", the workspace " + std::to_string(h->n) + "x" + std::to_string(h->n));
+    if (sh->is_complex && !x_complex)
+        return fail(h, LHL_ERR_ARG, "a real x cannot hold the solution of a complex shift; use a complex vector");
+    if (xdt != LHL_F64 && xdt != LHL_F32) return fail(h, LHL_ERR_ARG, "lhl_ldiv: x must be Float64 or Float32");
+    if (nrhs < 0 || (n > 0 && nrhs > 0 && (!x || ldx < n))) return fail(h, LHL_ERR_DIM, "lhl_ldiv: bad leading dimension");
+    if (!h->reduced || !sh->loaded) return fail(h, LHL_ERR_STATE, "lhl_ldiv: no shift loaded (call lhl_shift first)");
+    if (n == 0 || nrhs == 0) return LHL_OK;
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const int sdt = stage_dtype(h, xdt);
+    int64_t ld, plane;
+    st = stage_in(h, h->xstage, x, n, nrhs, ldx, xdt, x_complex, on_device, sdt, ld, plane);
+    if (st.ok()) st = do_solve_staged(h, sh, h->xstage.p, h->xstage.p, ld, plane, nrhs, sdt, x_complex, 0);
+    if (st.ok()) st = stage_out(h, h->xstage, ld, plane, sdt, x, n, nrhs, ldx, xdt, x_complex, on_device);
+    if (st.ok()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, std::string("lhl_ldiv: ") + hipGetErrorString(e)};
+    }
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
+int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int64_t ldw, const void *b, int64_t n,
+               int64_t nrhs, int64_t ldx, int xdt, int x_complex, int on_device, int steps, double tol, int *iters) {
+    if (!h) return LHL_ERR_ARG;
+    if (iters) *iters = 0;
+    lhl_shift_state *sh = resolve_shift(h, sh_);
+    if (n != h->n) return fail(h, LHL_ERR_DIM, "lhl_refine: x has the wrong length");
+    if (sh->n != h->n) return fail(h, LHL_ERR_DIM, "lhl_refine: the LHLShift does not match the workspace size");
+    if (sh->is_complex && !x_complex)
+        return fail(h, LHL_ERR_ARG, "a real x cannot hold the solution of a complex shift; use a complex vector");
+    if (!h->reduced || !sh->loaded) return fail(h, LHL_ERR_STATE, "lhl_refine: no shift loaded");
+    if (W && ldw < n) return fail(h, LHL_ERR_DIM, "lhl_refine: W's leading dimension is smaller than n");
+    if (steps <= 0 || n == 0 || nrhs == 0) return LHL_OK;
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const int sdt = xdt;   // the residual and the update run in x's precision
+    int64_t ld, plane, ldb_, planeb;
+    st = stage_in(h, h->xstage, x, n, nrhs, ldx, xdt, x_complex, on_device, sdt, ld, plane);
+    if (!st.ok()) return fail(h, st);
+    DevBuf bst, rst, wst;
+    st = stage_in(h, bst, b, n, nrhs, ldx, xdt, x_complex, on_device, sdt, ldb_, planeb);
+    if (!st.ok()) return fail(h, st);
+    const int planes = x_complex ? 2 : 1;
+    st = rst.ensure(elt_size(sdt) * (size_t)plane * planes + 64);
+    if (!st.ok()) return fail(h, st);
+    ResidArgs ra;
+    DevBuf colshift;
+    if (W) {
+        int64_t ldw2, planew;
+        st = stage_in(h, wst, W, n, n, ldw, xdt, x_complex, on_device, sdt, ldw2, planew);
+        if (!st.ok()) return fail(h, st);
+        ra.W = wst.p;
+        ra.ldw = ldw2;
+        ra.w_complex = x_complex;
+    } else {
+        std::vector<double> cs((size_t)nrhs * 4);
+        for (int64_t c = 0; c < nrhs; c++) {
+            cs[4 * c] = sh->sigma[0]; cs[4 * c + 1] = sh->sigma[1];
+            cs[4 * c + 2] = sh->tau[0]; cs[4 * c + 3] = sh->tau[1];
+        }
+        st = colshift.ensure(8 * cs.size());
+        if (!st.ok()) return fail(h, st);
+        hipError_t e = hipMemcpy(colshift.p, cs.data(), 8 * cs.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(h, LHL_ERR_HIP, hipGetErrorString(e));
+        ra.colshift = colshift.as<double>();
+    }
+    DevBuf norms;
+    st = norms.ensure(sizeof(double) * 2 * (size_t)nrhs);
+    if (!st.ok()) return fail(h, st);
+    ra.b = bst.p;
+    ra.x = h->xstage.p;
+    ra.r = rst.p;
+    ra.ld = ld;
+    ra.plane = plane;
+    ra.ncol = nrhs;
+    ra.xdt = sdt;
+    ra.x_complex = x_complex;
+    ra.norms = norms.as<double>();
+    int done = 0;
+    std::vector<double> hn((size_t)2 * nrhs);
+    for (int it = 0; it < steps; it++) {
+        st = residual_device(h, ra);
+        if (!st.ok()) break;
+        if (tol > 0) {
+            hipError_t e = hipMemcpyAsync(hn.data(), norms.p, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+            bool ok = true;
+            for (int64_t c = 0; c < nrhs; c++)
+                if (!(hn[c] <= tol * hn[nrhs + c])) ok = false;
+            if (ok) break;
+        }
+        st = do_solve_staged(h, sh, rst.p, h->xstage.p, ld, plane, nrhs, sdt, x_complex, 1);
+        if (!st.ok()) break;
+        done++;
+    }
+    if (st.ok()) st = stage_out(h, h->xstage, ld, plane, sdt, x, n, nrhs, ldx, xdt, x_complex, on_device);
+    if (st.ok()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+    }
+    bst.release(); rst.release(); wst.release(); colshift.release(); norms.release();
+    if (iters) *iters = done;
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
+int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, const double *tau,
+                            int shifts_complex, const int64_t *rhs_offsets, const double *B, int64_t ldb, double *X,
+                            int64_t ldx, int on_device, int precision, double tol, int max_iter, int64_t *info,
+                            int *iters) {
+    if (!h) return LHL_ERR_ARG;
+    if (nshift < 0 || (nshift > 0 && (!sigma || !tau || !rhs_offsets))) return fail(h, LHL_ERR_ARG, "batched: bad shift arrays");
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "batched: no reduction held");
+    if (precision != LHL_PREC_WORKING && precision != LHL_PREC_REFINE64) return fail(h, LHL_ERR_ARG, "batched: bad precision");
+    if (precision == LHL_PREC_REFINE64 && h->j_dtype != LHL_F64)
+        return fail(h, LHL_ERR_STATE, "batched: FP64 refinement needs a workspace reduced from an FP64 J");
+    if (!shifts_complex)
+        for (int64_t s = 0; s < nshift; s++)
+            if (sigma[2 * s + 1] != 0.0 || tau[2 * s + 1] != 0.0)
+                return fail(h, LHL_ERR_ARG, "batched: complex shift with shifts_complex = 0");
+    const int64_t n = h->n;
+    const int64_t ncol = nshift > 0 ? rhs_offsets[nshift] - rhs_offsets[0] : 0;
+    if (n == 0 || nshift == 0) return LHL_OK;
+    if (ncol < 0 || ldb < n || ldx < n) return fail(h, LHL_ERR_DIM, "batched: bad right-hand-side layout");
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const int cplx = shifts_complex ? 1 : 0;
+    const int planes = cplx ? 2 : 1;
+    // stage B and X (FP64 planar)
+    int64_t ld, plane, ldb2, planeb;
+    DevBuf bst, rst, norms, colshift, colbeg;
+    const double *Bc = B + (on_device ? 0 : 0);
+    st = stage_in(h, bst, Bc + (size_t)rhs_offsets[0] * ldb * planes, n, ncol, ldb, LHL_F64, cplx, on_device, LHL_F64,
+                  ldb2, planeb);
+    if (!st.ok()) return fail(h, st);
+    st = h->xstage.ensure(8 * (size_t)planeb * planes + 64);
+    if (!st.ok()) return fail(h, st);
+    ld = ldb2;
+    plane = planeb;
+    // per-chunk pools: bound the U storage to about 8 GiB
+    const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nshift, (int64_t)((8ull << 30) / std::max<size_t>(ubytes, 1))));
+    ShiftPool pool;
+    std::vector<int> hinfo((size_t)nshift, 0);
+    if (iters) for (int64_t s = 0; s < nshift; s++) iters[s] = 0;
+    for (int64_t s0 = 0; s0 < nshift && st.ok(); s0 += chunk) {
+        const int64_t cnt = std::min(chunk, nshift - s0);
+        st = shift_pool_device(h, pool, cnt, sigma + 2 * s0, tau + 2 * s0, cplx);
+        if (!st.ok()) break;
+        std::vector<int64_t> cb((size_t)cnt + 1);
+        for (int64_t s = 0; s <= cnt; s++) cb[s] = rhs_offsets[s0 + s] - rhs_offsets[0];
+        const int64_t c0 = cb[0], c1 = cb[cnt];
+        for (auto &v : cb) v -= c0;
+        st = colbeg.ensure(sizeof(int64_t) * cb.size());
+        if (!st.ok()) break;
+        hipError_t e = hipMemcpy(colbeg.p, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        const int64_t nc = c1 - c0;
+        if (nc == 0) continue;
+        SolveArgs a;
+        a.U = pool.U.p;
+        a.l = pool.l.p;
+        a.rdiag = pool.rdiag.p;
+        a.swp = pool.swp.as<uint8_t>();
+        const int64_t lplane = ((n + 63) / 64) * 64;
+        a.ustride = pool.ustride;
+        a.lstride = lplane * planes;
+        a.sstride = lplane;
+        a.shift_complex = cplx;
+        a.nshift = cnt;
+        a.col_begin = colbeg.as<int64_t>();
+        a.ncol = nc;
+        a.x_complex = cplx;
+        a.xdt = LHL_F64;
+        a.xin = bst.as<double>() + c0 * ld;
+        a.xout = h->xstage.as<double>() + c0 * ld;
+        a.ldx = ld;
+        a.xplane = plane;
+        a.accumulate = 0;
+        st = solve_device(h, a);
+        if (!st.ok()) break;
+        e = hipMemcpy(hinfo.data() + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        if (precision == LHL_PREC_REFINE64) {
+            st = rst.ensure(8 * (size_t)plane * planes + 64);
+            if (st.ok()) st = norms.ensure(sizeof(double) * 2 * (size_t)nc);
+            std::vector<double> cs((size_t)nc * 4);
+            for (int64_t s = 0; s < cnt; s++)
+                for (int64_t c = cb[s]; c < cb[s + 1]; c++) {
+                    cs[4 * c] = sigma[2 * (s0 + s)]; cs[4 * c + 1] = sigma[2 * (s0 + s) + 1];
+                    cs[4 * c + 2] = tau[2 * (s0 + s)]; cs[4 * c + 3] = tau[2 * (s0 + s) + 1];
+                }
+            if (st.ok()) st = colshift.ensure(8 * cs.size());
+            if (!st.ok()) break;
+            e = hipMemcpy(colshift.p, cs.data(), 8 * cs.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+            ResidArgs ra;
+            ra.colshift = colshift.as<double>();
+            ra.b = bst.as<double>() + c0 * ld;
+            ra.x = h->xstage.as<double>() + c0 * ld;
+            ra.r = rst.as<double>() + c0 * ld;
+            ra.ld = ld;
+            ra.plane = plane;
+            ra.ncol = nc;
+            ra.xdt = LHL_F64;
+            ra.x_complex = cplx;
+            ra.norms = norms.as<double>();
+            std::vector<double> hn((size_t)2 * nc);
+            std::vector<char> conv((size_t)cnt, 0);
+            for (int it = 0; it < max_iter; it++) {
+                st = residual_device(h, ra);
+                if (!st.ok()) break;
+                e = hipMemcpyAsync(hn.data(), norms.p, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, h->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+                if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+                bool all = true;
+                for (int64_t s = 0; s < cnt; s++) {
+                    bool ok = true;
+                    for (int64_t c = cb[s]; c < cb[s + 1]; c++)
+                        if (!(hn[c] <= tol * hn[nc + c])) ok = false;
+                    if (!ok) { all = false; if (iters) iters[s0 + s] = it + 1; }
+                }
+                if (all) break;
+                SolveArgs r = a;
+                r.xin = ra.r;
+                r.xout = h->xstage.as<double>() + c0 * ld;
+                r.accumulate = 1;
+                st = solve_device(h, r);
+                if (!st.ok()) break;
+            }
+            if (!st.ok()) break;
+        }
+    }
+    if (st.ok()) st = stage_out(h, h->xstage, ld, plane, LHL_F64, X + (size_t)rhs_offsets[0] * ldx * planes, n, ncol, ldx,
+                                LHL_F64, cplx, on_device);
+    if (st.ok()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+    }
+    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
+                      &pool.params})
+        b->release();
+    if (info) for (int64_t s = 0; s < nshift; s++) info[s] = hinfo[s];
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
+static int applyz_common(lhl_handle *h, int inverse, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
+                         int x_complex, int on_device) {
+    if (!h) return LHL_ERR_ARG;
+    if (n != h->n)
+        return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
+                                        "x" + std::to_string(h->n));
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "applyZ: no reduction held");
+    if (n == 0 || nrhs == 0) return LHL_OK;
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const int sdt = stage_dtype(h, xdt);
+    int64_t ld, plane;
+    st = stage_in(h, h->xstage, x, n, nrhs, ldx, xdt, x_complex, on_device, sdt, ld, plane);
+    if (st.ok()) st = applyz_device(h, inverse, h->xstage.p, nrhs, ld, plane, sdt, x_complex);
+    if (st.ok()) st = stage_out(h, h->xstage, ld, plane, sdt, x, n, nrhs, ldx, xdt, x_complex, on_device);
+    if (st.ok()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+    }
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
+int lhl_applyZ(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt, int x_complex, int on_device) {
+    return applyz_common(h, 0, x, n, nrhs, ldx, xdt, x_complex, on_device);
+}
+int lhl_applyZinv(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt, int x_complex, int on_device) {
+    return applyz_common(h, 1, x, n, nrhs, ldx, xdt, x_complex, on_device);
+}
+
+int lhl_get_info(const lhl_handle *h, int64_t *n, int *dtype, int *shift_complex, int *reduced) {
+    if (!h) return LHL_ERR_ARG;
+    if (n) *n = h->n;
+    if (dtype) *dtype = h->dtype;
+    if (shift_complex) *shift_complex = h->shift_complex;
+    if (reduced) *reduced = h->reduced;
+    return LHL_OK;
+}
+
+int lhl_get_ipiv(lhl_handle *h, int32_t *ipiv) {
+    if (!h || !ipiv) return LHL_ERR_ARG;
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
+    if (h->n <= 2) return LHL_OK;
+    if (hipSetDevice(h->device) != hipSuccess ||
+        hipMemcpy(ipiv, h->ipiv.p, 4 * (size_t)(h->n - 2), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_get_ipiv: copy failed");
+    return LHL_OK;
+}
+
+int lhl_get_perm(lhl_handle *h, int32_t *perm, int32_t *iperm) {
+    if (!h) return LHL_ERR_ARG;
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
+    if (h->n == 0) return LHL_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return LHL_ERR_HIP;
+    if (perm && hipMemcpy(perm, h->perm.p, 4 * (size_t)h->n, hipMemcpyDeviceToHost) != hipSuccess) return LHL_ERR_HIP;
+    if (iperm && hipMemcpy(iperm, h->iperm.p, 4 * (size_t)h->n, hipMemcpyDeviceToHost) != hipSuccess) return LHL_ERR_HIP;
+    return LHL_OK;
+}
+
+int lhl_get_scale(lhl_handle *h, double *scale, double *iscale) {
+    if (!h) return LHL_ERR_ARG;
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
+    if (h->n == 0) return LHL_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return LHL_ERR_HIP;
+    if (scale && hipMemcpy(scale, h->scale64.p, 8 * (size_t)h->n, hipMemcpyDeviceToHost) != hipSuccess) return LHL_ERR_HIP;
+    if (iscale && hipMemcpy(iscale, h->iscale64.p, 8 * (size_t)h->n, hipMemcpyDeviceToHost) != hipSuccess) return LHL_ERR_HIP;
+    return LHL_OK;
+}
+
+int lhl_get_factors(lhl_handle *h, void *f) {
+    if (!h || !f) return LHL_ERR_ARG;
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
+    if (h->n == 0) return LHL_OK;
+    const size_t es = elt_size(h->dtype);
+    if (hipSetDevice(h->device) != hipSuccess ||
+        hipMemcpy2D(f, es * h->n, h->A.p, es * h->lda, es * h->n, h->n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_get_factors: copy failed");
+    return LHL_OK;
+}
+
+int lhl_get_ht(lhl_handle *h, void *f) {
+    if (!h || !f) return LHL_ERR_ARG;
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
+    if (h->n == 0) return LHL_OK;
+    const size_t es = elt_size(h->dtype);
+    if (hipSetDevice(h->device) != hipSuccess ||
+        hipMemcpy2D(f, es * h->n, h->Ht.p, es * h->lda, es * h->n, h->n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_get_ht: copy failed");
+    return LHL_OK;
+}
+
+int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh_, void *U, void *l, uint8_t *swap, void *rdiag,
+                        double *sigma, double *tau, int64_t *info) {
+    if (!h) return LHL_ERR_ARG;
+    lhl_shift_state *sh = resolve_shift(h, sh_);
+    if (sigma) { sigma[0] = sh->sigma[0]; sigma[1] = sh->sigma[1]; }
+    if (tau) { tau[0] = sh->tau[0]; tau[1] = sh->tau[1]; }
+    if (info) *info = sh->info;
+    if (!sh->loaded) return fail(h, LHL_ERR_STATE, "no shift loaded");
+    const int64_t n = sh->n;
+    if (n == 0) return LHL_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return LHL_ERR_HIP;
+    const size_t es = elt_size(sh->dtype);
+    const int planes = sh->planes;
+    int64_t us, ls, ss;
+    shift_planes_lstride(sh, n, us, ls, ss);
+    const int64_t up = us / planes, lp = ls / planes;
+    std::vector<unsigned char> hu(es * (size_t)us), hl(es * (size_t)ls), hr(es * (size_t)ls);
+    if (hipMemcpy(hu.data(), sh->U.p, hu.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hl.data(), sh->l.p, hl.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hr.data(), sh->rdiag.p, hr.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_get_shift_state: copy failed");
+    if (swap && hipMemcpy(swap, sh->swp.p, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_get_shift_state: copy failed");
+    auto put = [&](void *dst, int64_t idx, const unsigned char *src, int64_t sidx, int plane_off) {
+        unsigned char *d = static_cast<unsigned char *>(dst);
+        for (int pl = 0; pl < planes; pl++)
+            std::memcpy(d + es * (size_t)(idx * planes + pl), src + es * (size_t)(sidx + pl * plane_off), es);
+    };
+    if (U) {
+        std::memset(U, 0, es * (size_t)planes * n * n);
+        for (int64_t k = 0; k < n; k++)
+            for (int64_t j = k; j < n; j++) put(U, k + j * n, hu.data(), upack_off(n, k) + (j - k), (int)up);
+    }
+    if (l)
+        for (int64_t k = 0; k + 1 < n; k++) put(l, k, hl.data(), k, (int)lp);
+    if (rdiag)
+        for (int64_t k = 0; k < n; k++) put(rdiag, k, hr.data(), k, (int)lp);
+    return LHL_OK;
+}
+
+int lhl_sync(lhl_handle *h) {
+    if (!h) return LHL_ERR_ARG;
+    if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(h, LHL_ERR_HIP, "lhl_sync failed");
+    return LHL_OK;
+}
+
+}  // extern "C"

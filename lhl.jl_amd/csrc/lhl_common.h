@@ -1,0 +1,187 @@
+// Internal declarations shared by the HIP translation units of liblhl_hip.so.
+// Everything here is gfx950 (CDNA4) code: wave64, 160 KB LDS per CU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+
+#include "../../include/lhl_hip.h"
+
+namespace lhl {
+
+constexpr int kWave = 64;
+constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
+
+// Status carried out of internal helpers; the ABI layer stores the message.
+struct Status {
+    int code = LHL_OK;
+    std::string msg;
+    bool ok() const { return code == LHL_OK; }
+};
+
+#define LHL_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::lhl::Status{LHL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define LHL_TRY(expr)                     \
+    do {                                  \
+        ::lhl::Status s_ = (expr);        \
+        if (!s_.ok()) return s_;          \
+    } while (0)
+
+inline size_t elt_size(int dtype) { return dtype == LHL_F64 ? 8 : 4; }
+
+// Leading dimension of a device matrix of n rows: a multiple of 64 elements so
+// every column starts on a 256/512-byte boundary, bumped by 64 when the column
+// stride is a multiple of 16 KiB (keeps neighbouring columns off the same HBM
+// channel group).
+inline int64_t pad_ld(int64_t n, int dtype) {
+    int64_t ld = ((n + 63) / 64) * 64;
+    if (ld == 0) ld = 64;
+    if ((ld * (int64_t)elt_size(dtype)) % 16384 == 0) ld += 64;
+    return ld;
+}
+
+// Offset of row k of the packed upper-triangular U (row k holds U[k, k..n-1]).
+__host__ __device__ inline int64_t upack_off(int64_t n, int64_t k) { return k * n - (k * (k - 1)) / 2; }
+inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
+
+// Growable device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    Status ensure(size_t want) {
+        if (want <= bytes && p) return {};
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (want == 0) want = 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) return {LHL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+        bytes = want;
+        return {};
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Control words of the reduction's step loop (device memory, ping-pong by step
+// parity so a step's consumers never race its producer).
+template <typename T>
+struct RedCtl {
+    int p;        // pivot row of the step (0-based)
+    int skip;     // 1 when the pivot is zero: the step's updates are skipped
+    int nchunk;   // column chunks of the trailing GEMV partials
+    int pad;
+    T piv;        // the pivot value A[p, k] (post-interchange A[k+1, k])
+    T pk;         // post-interchange A[k+1, k+1]
+};
+
+}  // namespace lhl
+
+// The shift state (LHLShift{TG}, src:84-96): the packed U of sigma*I + tau*H,
+// the multipliers, interchange flags, reciprocal pivots, one plane per real
+// component.
+struct lhl_shift_state {
+    lhl_handle *h = nullptr;
+    int64_t n = 0;
+    int is_complex = 0;
+    int dtype = LHL_F64;
+    int planes = 1;
+    lhl::DevBuf U, l, rdiag, swp, dinfo;
+    double sigma[2] = {0, 0}, tau[2] = {0, 0};
+    int64_t info = 0;
+    int loaded = 0;   // a shift has been factored for the current n
+};
+
+// The workspace (LHLWorkspace{T,Tr,TG}, src:144-162).
+struct lhl_handle {
+    int64_t n = 0;
+    int dtype = LHL_F64;        // element type T of the reduction and factors
+    int shift_complex = 0;      // element type of ws.shift
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t lda = 0;            // leading dimension of A and Ht
+    int64_t ldj = 0;            // leading dimension of the resident J
+    int j_dtype = LHL_F64;      // element type of the resident J
+    int reduced = 0;
+    lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
+    lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
+    lhl::DevBuf ybuf, rbuf, nbuf, xstage, bstage;  // solve / refine scratch
+    lhl_shift_state *own = nullptr;   // ws.shift
+    std::vector<lhl_shift_state *> held; // LHLShift states created on this handle
+    std::string err;
+};
+
+namespace lhl {
+
+// ---- launchers implemented in the translation units ----
+Status reduce_device(lhl_handle *h, int balance);          // lhl_reduce.hip
+Status balance_device(lhl_handle *h);                      // lhl_reduce.hip
+Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t nshift, const double *sig, const double *tau,
+                    int64_t *info_out);                    // lhl_shift.hip (one state)
+
+// Batched shift LU into a pool of states laid out back to back.
+struct ShiftPool {
+    int64_t n = 0, count = 0;
+    int planes = 1;
+    int64_t ustride = 0;   // elements between consecutive U's (per plane stride = ustride/planes)
+    DevBuf U, l, rdiag, swp, dinfo, params;
+};
+Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const double *sig,
+                         const double *tau, int is_complex);
+
+// The solve: x (device, element type xdt, nplanes planes of n x ncol, ld ldx
+// per plane) <- W^-1 x, with U from `uptr` etc.  accumulate = 1 adds the
+// solution to xout instead of overwriting it.
+struct SolveArgs {
+    const void *U, *l, *rdiag;
+    const uint8_t *swp;
+    int64_t ustride = 0;          // per shift, elements (both planes)
+    int64_t lstride = 0;          // per shift, elements of l / rdiag (both planes)
+    int64_t sstride = 0;          // per shift, swap bytes
+    int shift_complex = 0;
+    int64_t nshift = 1;
+    const int64_t *col_begin = nullptr;   // device: nshift+1 column offsets (nullptr: one shift, all cols)
+    int64_t ncol = 0;             // total right-hand-side columns
+    int x_complex = 0;            // columns hold complex values (planar in device memory)
+    int xdt = LHL_F64;            // element type of x in memory
+    const void *xin = nullptr;    // input columns (planar: re plane then im plane at +xplane)
+    void *xout = nullptr;
+    int64_t ldx = 0, xplane = 0;
+    int accumulate = 0;
+};
+Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_solve.hip
+Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane,
+                     int xdt, int x_complex);              // lhl_solve.hip
+
+// r = b - W x for ncol columns, W dense (w != nullptr, element type xdt) or
+// sigma_c*I + tau_c*J (per-column shift scalars, J resident).  All vectors are
+// planar device arrays of element type xdt; norms[c] = ||r_c||_inf,
+// norms[ncol+c] = ||b_c||_inf (double).
+struct ResidArgs {
+    const void *W = nullptr;
+    int64_t ldw = 0;
+    int w_complex = 0;
+    const double *colshift = nullptr;   // device: 4 doubles per column (sr, si, tr, ti)
+    const void *b = nullptr;
+    const void *x = nullptr;
+    void *r = nullptr;
+    int64_t ld = 0, plane = 0, ncol = 0;
+    int xdt = LHL_F64;
+    int x_complex = 0;
+    double *norms = nullptr;
+};
+Status residual_device(lhl_handle *h, const ResidArgs &a);  // lhl_solve.hip
+
+// Planar <-> interleaved conversions and precision casts (lhl_solve.hip).
+Status to_planar(lhl_handle *h, const void *src, int64_t ld_src, int src_dt, int src_complex,
+                 void *dst, int64_t ld_dst, int64_t plane_dst, int dst_dt, int64_t n, int64_t ncol);
+Status from_planar(lhl_handle *h, const void *src, int64_t ld_src, int64_t plane_src, int src_dt,
+                   void *dst, int64_t ld_dst, int dst_dt, int dst_complex, int64_t n, int64_t ncol);
+
+}  // namespace lhl

@@ -1,0 +1,360 @@
+"""GPU parity: liblhl_hip.so (through the host mirror and its C ABI) against the
+CPU oracle on the same seeded inputs.
+
+  * reduction: identical pivots (ipiv, perm) and balancing scales, factors within
+    1e-12*n relative (Frobenius) of the oracle's;
+  * shift LU: the GPU state (U, multipliers, swap, rdiag, info) equals the
+    oracle's bit for bit when both factor the same Ht (the reference's own
+    bitwise spec, test/core_tests.jl:115-233);
+  * solves and refinement: against the oracle and dense LAPACK within the
+    reference's tolerances (rtol 1e-9 solves, bwd bounds).
+"""
+import numpy as np
+import pytest
+
+from inputs import PRECOMPILE_J, bwd, hilbert_plus_I, near_nilpotent, randn, stiff, tie_matrix, wilkinson
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def check_reduction(L, O, J, balance=True, blocked=-1, dtype=np.float64, tol_n=1e-12):
+    n = J.shape[0]
+    ws = L.lhl(J, balance=balance, dtype=dtype)
+    red = O.reduce(J.astype(dtype), balance=balance, blocked=blocked)
+    assert np.array_equal(ws.ipiv - 1, red.ipiv), f"ipiv differs at n={n}"
+    assert np.array_equal(ws.perm - 1, red.perm)
+    assert np.array_equal(ws.scale, red.scale)
+    F = ws.factors
+    assert relerr(F.astype(np.float64), red.A.astype(np.float64)) <= tol_n * max(n, 1)
+    Ht = ws.Ht
+    assert np.array_equal(Ht, np.triu(F, -1).T)
+    return ws, red
+
+
+def test_smoke_precompile_inputs(lhl_hip, oracle):
+    say("case: precompile 4x4")
+    J = PRECOMPILE_J
+    ws, red = check_reduction(lhl_hip, oracle, J)
+    lhl_hip.lhl_shift_(ws, 1.0, -0.1)
+    x = lhl_hip.lhl_ldiv_(np.ones(4), ws)
+    assert np.allclose(x, np.linalg.solve(np.eye(4) - 0.1 * J, np.ones(4)), rtol=1e-14)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 17, 80, 128, 200, 301, 400])
+@pytest.mark.parametrize("balance", [True, False])
+def test_reduction_matches_oracle_unblocked(lhl_hip, oracle, n, balance):
+    say(f"case: reduction n={n} balance={balance}")
+    check_reduction(lhl_hip, oracle, randn(n, n), balance=balance, blocked=0)
+
+
+@pytest.mark.parametrize("n", [150, 200])
+def test_reduction_ties(lhl_hip, oracle, n):
+    say(f"case: tie reduction n={n}")
+    check_reduction(lhl_hip, oracle, tie_matrix(9, n, -1, 1), blocked=0)
+
+
+@pytest.mark.parametrize("n", [600, 1030])
+def test_reduction_blocked_sizes(lhl_hip, oracle, n):
+    say(f"case: reduction n={n} (oracle blocked)")
+    check_reduction(lhl_hip, oracle, randn(n, n), blocked=1)
+
+
+def test_reduction_float32(lhl_hip, oracle):
+    say("case: float32 reduction n=300")
+    J = randn(4, 300, dtype=np.float32)
+    ws = lhl_hip.lhl(J)
+    red = oracle.reduce(J, blocked=0)
+    assert np.array_equal(ws.ipiv - 1, red.ipiv)
+    assert relerr(ws.factors.astype(np.float64), red.A.astype(np.float64)) <= 1e-5
+
+
+def shift_state_equal(lhl_hip, oracle, ws, s, t, sh=None):
+    Ht = ws.Ht
+    if sh is None:
+        lhl_hip.lhl_shift_(ws, s, t)
+        st = ws.shift
+    else:
+        lhl_hip.lhl_shift_(sh, ws, s, t)
+        st = sh
+    if st.is_complex:
+        ref = oracle.shift_c(Ht, s, t)
+        refU = ref.U()
+        refl = ref.mult()
+        refrd = ref.rdiag + 1j * ref.rdiag_im
+    else:
+        ref = oracle.shift(Ht, s, t)
+        refU, refl, refrd = ref.U(), ref.mult(), ref.rdiag
+    assert st.info == ref.info
+    assert np.array_equal(st.swap, ref.swap.astype(bool))
+    assert np.array_equal(st.U, refU, equal_nan=True)
+    assert np.array_equal(st.multipliers, refl, equal_nan=True)
+    assert np.array_equal(st.rdiag, refrd, equal_nan=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("tie", [False, True])
+def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
+    for n in [1, 2, 3, 5, 17, 64, 255, 256, 257, 511, 512, 520, 1000]:
+        say(f"case: shift bitwise n={n} {np.dtype(dtype).name} tie={tie}")
+        J = tie_matrix(n, n, dtype=dtype) if tie else randn(n, n, dtype=dtype)
+        ws = lhl_hip.lhl(J)
+        for s, t in ((1, -0.1), (0, 1), (1, -3), (2, 0), (0, 0)):
+            shift_state_equal(lhl_hip, oracle, ws, s, t)
+
+
+@pytest.mark.parametrize("tie", [False, True])
+def test_complex_shift_bitwise(lhl_hip, oracle, tie):
+    for n in [1, 2, 5, 40, 100, 256, 511]:
+        say(f"case: complex shift bitwise n={n} tie={tie}")
+        J = tie_matrix(n, n) if tie else randn(n, n)
+        ws = lhl_hip.lhl(J, shift=np.complex128)
+        for s, t in ((0.4 + 0.7j, -1), (1, -0.1 + 0.2j), (0, 1j), (2j, 0), (0, 0)):
+            shift_state_equal(lhl_hip, oracle, ws, s, t)
+
+
+def test_singular_shift_reported(lhl_hip):
+    ws = lhl_hip.lhl(np.array([[1.0, 0.0], [0.0, 2.0]]))
+    lhl_hip.lhl_shift_(ws, 1, -1.0)
+    assert ws.info != 0
+    wd = lhl_hip.lhl(np.array([[1.0, 0.0], [0.0, 2.0]]), shift=np.complex128)
+    lhl_hip.lhl_shift_(wd, -1j, 1j)
+    assert wd.info != 0
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 40, 129, 260, 400])
+def test_shifted_solves(lhl_hip, oracle, n):
+    say(f"case: shifted solves n={n}")
+    J = randn(n, n)
+    b = np.random.default_rng(n + 7).standard_normal(n)
+    ws = lhl_hip.lhl(J)
+    red = oracle.reduce(J, blocked=0)
+    for s, t in ((1.0, 0.0), (1.0, -1e-8), (1.0, -0.05), (1.0, 1.7), (0.0, 1.0), (-3.0, 2.0)):
+        W = s * np.eye(n) + t * J
+        if np.linalg.matrix_rank(W) < n:
+            continue
+        lhl_hip.lhl_shift_(ws, s, t)
+        x = lhl_hip.lhl_ldiv_(b.copy(), ws)
+        assert relerr(x, np.linalg.solve(W, b)) <= 1e-9
+        xo = oracle.ldiv(red, oracle.shift(red.Ht, s, t), b)
+        assert relerr(x, xo) <= 1e-12 * max(n, 1) * max(1.0, np.linalg.cond(W) / 1e3)
+        assert bwd(W, x, b) < 50 * np.finfo(float).eps * max(1, n / 100)
+
+
+@pytest.mark.parametrize("n", [3, 40, 129, 300])
+def test_complex_solves(lhl_hip, n):
+    say(f"case: complex solves n={n}")
+    J = randn(n, n)
+    rng = np.random.default_rng(n + 7)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ws = lhl_hip.lhl(J, shift=np.complex128)
+    for s, t in ((0.4 + 0.7j, -1.0), (1.0, -0.05 + 0.3j), (0.0, 1j), (-3.0 + 2j, 2.0 - 1j), (1.0, -0.05)):
+        W = s * np.eye(n) + t * J
+        lhl_hip.lhl_shift_(ws, s, t)
+        x = lhl_hip.lhl_ldiv_(b.copy(), ws)
+        assert relerr(x, np.linalg.solve(W, b)) <= 1e-9
+        y = lhl_hip.lhl_refine_(x.copy(), W, b, ws, 1)
+        assert bwd(W, y, b) <= 2 * bwd(W, x, b) + np.finfo(float).eps
+        assert bwd(W, y, b) < 50 * np.finfo(float).eps
+
+
+def test_radau_real_and_complex_held_together(lhl_hip):
+    n = 80
+    J = randn(90, n)
+    br = np.random.default_rng(91).standard_normal(n)
+    rng = np.random.default_rng(92)
+    bc = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ws = lhl_hip.lhl(J)
+    sh = lhl_hip.LHLShift(ws, np.complex128)
+    for g, gc in ((0.1, 0.05 + 0.1j), (0.2, 0.1 - 0.3j)):
+        assert lhl_hip.lhl_shift_(ws, 1, -g) is ws
+        assert lhl_hip.lhl_shift_(sh, ws, 1, -gc) is sh
+        assert relerr(lhl_hip.lhl_ldiv_(br.copy(), ws), np.linalg.solve(np.eye(n) - g * J, br)) <= 1e-9
+        assert relerr(lhl_hip.lhl_ldiv_(bc.copy(), sh, ws), np.linalg.solve(np.eye(n) - gc * J, bc)) <= 1e-9
+        # complex rhs, real shift
+        assert relerr(lhl_hip.lhl_ldiv_(bc.copy(), ws), np.linalg.solve(np.eye(n) - g * J, bc)) <= 1e-9
+        Wc = np.eye(n) - gc * J
+        xr = lhl_hip.lhl_refine_(lhl_hip.lhl_ldiv_(bc.copy(), sh, ws), Wc, bc, sh, ws, 1)
+        assert relerr(xr, np.linalg.solve(Wc, bc)) <= 1e-12
+        assert ws.info == 0 and sh.info == 0 and sh.sigma == 1 and sh.tau == -gc
+
+
+def test_errors(lhl_hip):
+    n = 20
+    ws = lhl_hip.lhl(randn(93, n))
+    with pytest.raises(lhl_hip.ArgumentError):
+        lhl_hip.lhl_shift_(ws, 1, -0.3j)
+    wc = lhl_hip.lhl(randn(93, n), shift=np.complex128)
+    lhl_hip.lhl_shift_(wc, 1, -0.3j)
+    with pytest.raises(lhl_hip.ArgumentError):
+        lhl_hip.lhl_ldiv_(np.ones(n), wc)
+    lhl_hip.lhl_shift_(ws, 1, -0.3)
+    with pytest.raises(lhl_hip.DimensionMismatch):
+        lhl_hip.lhl_ldiv_(np.ones(n + 1), ws)
+    with pytest.raises(lhl_hip.DimensionMismatch):
+        lhl_hip.lhl_ldiv_(np.ones(n, dtype=complex), lhl_hip.LHLShift(n // 2, np.complex128), ws)
+    with pytest.raises(lhl_hip.DimensionMismatch):
+        lhl_hip.applyZ_(np.ones(n - 1), ws)
+    w32 = lhl_hip.lhl(randn(95, 3, dtype=np.float32))
+    with pytest.raises(lhl_hip.ArgumentError):
+        lhl_hip.lhl_shift_(lhl_hip.LHLShift(3, np.complex128), w32, 1, 1)
+    # complex with zero imaginary parts is fine
+    lhl_hip.lhl_shift_(ws, 1 + 0j, -0.3 + 0j)
+
+
+def test_refinement_families(lhl_hip):
+    n = 100
+    b = np.random.default_rng(42).standard_normal(n)
+    for name, J in (("randn", randn(1, n)), ("wilkinson", wilkinson(n))):
+        say(f"case: refine family {name}")
+        ws = lhl_hip.lhl(J)
+        assert np.abs(np.tril(ws.factors, -2)).max() <= 1 + np.finfo(float).eps
+        for g in (1e-3, 1.0, 1e3):
+            lhl_hip.lhl_shift_(ws, 1, -g)
+            if ws.info != 0:
+                continue
+            W = np.eye(n) - g * J
+            x = lhl_hip.lhl_ldiv_(b.copy(), ws)
+            assert lhl_hip.lhl_refine_(x, W, b, ws, 1) is x
+            assert bwd(W, x, b) < 1e-13
+
+
+def test_refinement_near_nilpotent(lhl_hip):
+    n = 60
+    J = near_nilpotent(17, n)
+    b = np.random.default_rng(18).standard_normal(n)
+    W = np.eye(n) - 0.5 * J
+    ws = lhl_hip.lhl(J)
+    lhl_hip.lhl_shift_(ws, 1, -0.5)
+    raw = lhl_hip.lhl_ldiv_(b.copy(), ws)
+    ref = lhl_hip.lhl_refine_(raw.copy(), W, b, ws, 1)
+    assert bwd(W, ref, b) <= bwd(W, raw, b)
+    assert bwd(W, ref, b) < 1e-13
+
+
+def test_applyZ_roundtrip(lhl_hip, oracle):
+    n = 130
+    J = randn(130, n)
+    ws = lhl_hip.lhl(J)
+    red = oracle.reduce(J, blocked=0)
+    x0 = np.random.default_rng(5).standard_normal(n)
+    z = lhl_hip.applyZ_(x0.copy(), ws)
+    zo = oracle.applyZ(red, np.asfortranarray(x0.copy()))
+    assert relerr(z, zo) <= 1e-12 * n
+    back = lhl_hip.applyZinv_(z.copy(), ws)
+    assert relerr(back, x0) <= 1e-12 * n
+
+
+def test_workspace_reuse_and_resize(lhl_hip):
+    n = 50
+    J = randn(3, n)
+    b = np.random.default_rng(4).standard_normal(n)
+    ws = lhl_hip.lhl(J)
+    for g in (0.02, 1e-6, 5.0, 0.02):
+        lhl_hip.lhl_shift_(ws, 1, -g)
+        assert relerr(lhl_hip.lhl_ldiv_(b.copy(), ws), np.linalg.solve(np.eye(n) - g * J, b)) <= 1e-9
+    J2 = randn(5, n)
+    assert lhl_hip.lhl_(ws, J2) is ws
+    lhl_hip.lhl_shift_(ws, 1, -0.3)
+    assert relerr(lhl_hip.lhl_ldiv_(b.copy(), ws), np.linalg.solve(np.eye(n) - 0.3 * J2, b)) <= 1e-9
+    J3 = randn(6, 2 * n)
+    lhl_hip.lhl_(ws, J3)
+    lhl_hip.lhl_shift_(ws, 1, -0.3)
+    b3 = np.random.default_rng(7).standard_normal(2 * n)
+    assert relerr(lhl_hip.lhl_ldiv_(b3.copy(), ws), np.linalg.solve(np.eye(2 * n) - 0.3 * J3, b3)) <= 1e-9
+
+
+def test_config1_readme_workflow(lhl_hip, oracle):
+    """J = randn(400,400), b = randn(400); lhl, then for gamma in (0.01, 0.013,
+    0.021): shift (1,-gamma), ldiv, refine 1 (README.md:40-52)."""
+    say("case: config 1")
+    n = 400
+    J = randn(400, n)
+    b = np.random.default_rng(401).standard_normal(n)
+    ws = lhl_hip.lhl(J)
+    red = oracle.reduce(J)
+    assert np.array_equal(ws.ipiv - 1, red.ipiv)
+    assert np.array_equal(ws.perm - 1, red.perm)
+    for g in (0.01, 0.013, 0.021):
+        W = np.eye(n) - g * J
+        lhl_hip.lhl_shift_(ws, 1, -g)
+        x = lhl_hip.lhl_ldiv_(b.copy(), ws)
+        sh = oracle.shift(red.Ht, 1.0, -g)
+        xo = oracle.ldiv(red, sh, b)
+        assert relerr(x, xo) <= 1e-12 * n
+        x = lhl_hip.lhl_refine_(x, W, b, ws, 1)
+        xo = oracle.refine(red, sh, W, b, xo, 1)
+        assert relerr(x, xo) <= 1e-12 * n
+        assert bwd(W, x, b) < 1e-15
+
+
+@pytest.mark.parametrize("n", [2000])
+def test_config2_stiff_fp64(lhl_hip, oracle, n):
+    say(f"case: config 2 stiff n={n}")
+    J = stiff(2000 + n, n)
+    ws = lhl_hip.lhl(J)
+    red = oracle.reduce(J)          # blocked oracle, as the reference at n >= 500
+    assert np.array_equal(ws.ipiv - 1, red.ipiv)
+    assert np.array_equal(ws.scale, red.scale)
+    assert relerr(ws.factors, red.A) <= 1e-12 * n
+    rng = np.random.default_rng(n)
+    for g in np.logspace(-4, -1, 8):
+        b = rng.standard_normal(n)
+        lhl_hip.lhl_shift_(ws, 1, -g)
+        x = lhl_hip.lhl_ldiv_(b.copy(), ws)
+        xo = oracle.ldiv(red, oracle.shift(red.Ht, 1.0, -g), b)
+        assert relerr(x, xo) <= 1e-12 * n
+        W = np.eye(n) - g * J
+        assert np.linalg.norm(b - W @ x, np.inf) / np.linalg.norm(b, np.inf) <= 1e-10
+
+
+def test_batched_matches_single(lhl_hip):
+    n = 300
+    J = stiff(11, n)
+    ws = lhl_hip.lhl(J)
+    gam = np.logspace(-4, -1, 6)
+    sig = np.ones(6)
+    tau = -gam
+    offs = np.array([0, 2, 3, 5, 6, 8, 10])
+    rng = np.random.default_rng(12)
+    B = np.asfortranarray(rng.standard_normal((n, 10)))
+    X, info, _ = lhl_hip.shift_solve_batched(ws, sig, tau, B, offs)
+    assert np.all(info == 0)
+    for s in range(6):
+        lhl_hip.lhl_shift_(ws, 1, -gam[s])
+        for c in range(offs[s], offs[s + 1]):
+            x = lhl_hip.lhl_ldiv_(B[:, c].copy(), ws)
+            assert relerr(X[:, c], x) <= 1e-13
+    # complex shifts s = i*omega on a real reduction, tau = -1
+    om = np.logspace(-2, 2, 4)
+    Bc = np.asfortranarray(rng.standard_normal((n, 8)) + 1j * rng.standard_normal((n, 8)))
+    Xc, info, _ = lhl_hip.shift_solve_batched(ws, 1j * om, -np.ones(4), Bc, np.arange(0, 9, 2))
+    for s in range(4):
+        W = 1j * om[s] * np.eye(n) - J
+        for c in range(2 * s, 2 * s + 2):
+            assert relerr(Xc[:, c], np.linalg.solve(W, Bc[:, c])) <= 1e-9
+
+
+def test_precision_form_refines_to_1e12(lhl_hip):
+    """FP32 factors of an FP64 J, FP64-residual refinement until ||r||/||b|| <= 1e-12."""
+    n = 1024
+    J = stiff(4096, n)
+    ws = lhl_hip.lhl(J, dtype=np.float32)
+    gam = np.logspace(-4, -1, 4)
+    offs = np.arange(0, 4 * 4 + 1, 4)
+    B = np.asfortranarray(np.random.default_rng(3).standard_normal((n, 16)))
+    X, info, iters = lhl_hip.shift_solve_batched(ws, np.ones(4), -gam, B, offs, precision="refine64", tol=1e-12,
+                                                 max_iter=30)
+    for s in range(4):
+        W = np.eye(n) - gam[s] * J
+        for c in range(offs[s], offs[s + 1]):
+            r = np.linalg.norm(B[:, c] - W @ X[:, c], np.inf) / np.linalg.norm(B[:, c], np.inf)
+            assert r <= 1e-12, (s, c, r, iters)
