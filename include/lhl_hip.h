@@ -150,6 +150,19 @@ int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh, void *U, void *l, ui
 /* Synchronize the handle's stream. */
 int lhl_sync(lhl_handle *h);
 
+/* ---- measurement (no reference counterpart; for benchmarks) ---- */
+
+/* The HIP stream (hipStream_t) every kernel of the handle runs on. */
+int lhl_get_stream(lhl_handle *h, void **stream);
+/* While enabled, every kernel launch is bracketed by HIP events on the
+ * handle's stream.  lhl_profile_read synchronizes, then returns the launches
+ * and the summed duration (ms) of one kernel family since the last reset:
+ * "balance", "reduce_copy", "reduce_pivot", "reduce_swap", "reduce_trailing",
+ * "reduce_finish", "shift", "solve", "applyz", "residual", "convert". */
+int lhl_profile_enable(lhl_handle *h, int on);
+int lhl_profile_reset(lhl_handle *h);
+int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,22 @@ Status do_solve_staged(lhl_handle *h, lhl_shift_state *sh, const void *xin, void
 
 lhl_shift_state *resolve_shift(lhl_handle *h, lhl_shift_state *sh) { return sh ? sh : h->own; }
 
+void prof_drain(lhl_handle *h) {
+    for (const ProfRec &r : h->prof.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) ms = 0.f;
+        auto &acc = h->prof.acc;
+        auto it = std::find_if(acc.begin(), acc.end(), [&](const auto &e) { return e.first == r.name; });
+        if (it == acc.end()) { acc.push_back({r.name, {0, 0.0}}); it = acc.end() - 1; }
+        it->second.first += 1;
+        it->second.second += ms;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    h->prof.pending.clear();
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -146,6 +162,7 @@ int lhl_destroy(lhl_handle *h) {
     if (!h) return LHL_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    prof_drain(h);
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
                       &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->ybuf, &h->rbuf, &h->nbuf,
                       &h->xstage, &h->bstage})
@@ -633,6 +650,39 @@ int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh_, void *U, void *l, u
         for (int64_t k = 0; k + 1 < n; k++) put(l, k, hl.data(), k, (int)lp);
     if (rdiag)
         for (int64_t k = 0; k < n; k++) put(rdiag, k, hr.data(), k, (int)lp);
+    return LHL_OK;
+}
+
+int lhl_get_stream(lhl_handle *h, void **stream) {
+    if (!h || !stream) return LHL_ERR_ARG;
+    *stream = (void *)h->stream;
+    return LHL_OK;
+}
+
+int lhl_profile_enable(lhl_handle *h, int on) {
+    if (!h) return LHL_ERR_ARG;
+    h->prof.on = on != 0;
+    return LHL_OK;
+}
+
+int lhl_profile_reset(lhl_handle *h) {
+    if (!h) return LHL_ERR_ARG;
+    if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) return LHL_ERR_HIP;
+    prof_drain(h);
+    h->prof.acc.clear();
+    return LHL_OK;
+}
+
+int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms) {
+    if (!h || !kernel) return LHL_ERR_ARG;
+    if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) return LHL_ERR_HIP;
+    prof_drain(h);
+    int64_t c = 0;
+    double t = 0;
+    for (const auto &e : h->prof.acc)
+        if (e.first == kernel) { c = e.second.first; t = e.second.second; }
+    if (launches) *launches = c;
+    if (total_ms) *total_ms = t;
     return LHL_OK;
 }
 

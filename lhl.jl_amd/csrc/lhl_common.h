@@ -81,6 +81,19 @@ struct RedCtl {
     T pk;         // post-interchange A[k+1, k+1]
 };
 
+// Opt-in kernel timing: while on, every launch through LHL_LAUNCH is
+// bracketed by two HIP events on the handle's stream; durations are summed per
+// kernel family when read.
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+struct Profiler {
+    bool on = false;
+    std::vector<ProfRec> pending;
+    std::vector<std::pair<std::string, std::pair<int64_t, double>>> acc;
+};
+
 }  // namespace lhl
 
 // The shift state (LHLShift{TG}, src:84-96): the packed U of sigma*I + tau*H,
@@ -112,12 +125,34 @@ struct lhl_handle {
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf ybuf, rbuf, nbuf, xstage, bstage;  // solve / refine scratch
+    lhl::Profiler prof;
     lhl_shift_state *own = nullptr;   // ws.shift
     std::vector<lhl_shift_state *> held; // LHLShift states created on this handle
     std::string err;
 };
 
 namespace lhl {
+
+struct ProfScope {
+    lhl_handle *h;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(lhl_handle *h_, const char *n_) : h(h_), name(n_) {
+        if (h->prof.on && hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, h->stream);
+    }
+    ~ProfScope() {
+        if (a && hipEventCreate(&b) == hipSuccess) {
+            (void)hipEventRecord(b, h->stream);
+            h->prof.pending.push_back(ProfRec{name, a, b});
+        }
+    }
+};
+
+#define LHL_LAUNCH(h, name, kern, grid, block, lds, ...)                            \
+    do {                                                                           \
+        ::lhl::ProfScope ps_((h), (name));                                         \
+        hipLaunchKernelGGL(kern, grid, block, lds, (h)->stream, __VA_ARGS__);      \
+    } while (0)
 
 // ---- launchers implemented in the translation units ----
 Status reduce_device(lhl_handle *h, int balance);          // lhl_reduce.hip

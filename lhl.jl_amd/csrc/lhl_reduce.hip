@@ -404,7 +404,7 @@ Status balance_typed(lhl_handle *h) {
     for (int sweep = 0; sweep < 20; sweep++) {
         LHL_HIP_TRY(hipMemsetAsync(h->flag.p, 0, sizeof(int), h->stream));
         for (int i0 = 0; i0 < n; i0 += kBalB)
-            hipLaunchKernelGGL(k_bal_block<TJ>, dim3(G), dim3(256), 0, h->stream, J, h->ldj, n, i0, d, isc,
+            LHL_LAUNCH(h, "balance", k_bal_block<TJ>, dim3(G), dim3(256), 0, J, h->ldj, n, i0, d, isc,
                                h->part.as<TJ>(), h->counter.as<unsigned>(), h->flag.as<int>());
         LHL_HIP_TRY(hipGetLastError());
         int changed = 0;
@@ -439,17 +439,17 @@ Status steps_typed(lhl_handle *h) {
     const int pblocks = std::max(1, std::min(kRedMaxBlocks, (n + kRedThreads - 1) / kRedThreads));
     const int rowblocks = (n + kRedThreads - 1) / kRedThreads;
     for (int k = 0; k + 2 < n; k++) {
-        hipLaunchKernelGGL(k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, h->stream, A, lda, n, k,
+        LHL_LAUNCH(h, "reduce_pivot", k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, A, lda, n, k,
                            (const T *)h->part.as<T>(), ldp, ctl, cand_v, cand_i, cand_x, vk1, counter, ipiv,
                            perm, 0);
-        hipLaunchKernelGGL(k_swap<T>, dim3((n + 255) / 256), dim3(256), 0, h->stream, A, lda, n, k,
+        LHL_LAUNCH(h, "reduce_swap", k_swap<T>, dim3((n + 255) / 256), dim3(256), 0, A, lda, n, k,
                            (const RedCtl<T> *)ctl);
         const int nch = (n - k - 2 + kChunk - 1) / kChunk;
-        hipLaunchKernelGGL(k_trailing<T>, dim3(rowblocks, nch), dim3(kRedThreads), 0, h->stream, A, lda, n,
+        LHL_LAUNCH(h, "reduce_trailing", k_trailing<T>, dim3(rowblocks, nch), dim3(kRedThreads), 0, A, lda, n,
                            k, (const RedCtl<T> *)ctl, h->part.as<T>(), ldp);
     }
     if (n >= 3)
-        hipLaunchKernelGGL(k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, h->stream, A, lda, n, n - 2,
+        LHL_LAUNCH(h, "reduce_pivot", k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, A, lda, n, n - 2,
                            (const T *)h->part.as<T>(), ldp, ctl, cand_v, cand_i, cand_x, vk1, counter, ipiv,
                            perm, 1);
     LHL_HIP_TRY(hipGetLastError());
@@ -461,11 +461,11 @@ Status finish_typed(lhl_handle *h) {
     const int n = (int)h->n;
     dim3 tb(32, 8);
     dim3 gr((n + 31) / 32, (n + 31) / 32);
-    hipLaunchKernelGGL(k_ht_fill<T>, gr, tb, 0, h->stream, (const T *)h->A.as<T>(), h->lda, h->Ht.as<T>(),
+    LHL_LAUNCH(h, "reduce_finish", k_ht_fill<T>, gr, tb, 0, (const T *)h->A.as<T>(), h->lda, h->Ht.as<T>(),
                        h->lda, n);
-    hipLaunchKernelGGL(k_iperm, dim3((n + 255) / 256), dim3(256), 0, h->stream, (const int *)h->perm.as<int>(),
+    LHL_LAUNCH(h, "reduce_finish", k_iperm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->perm.as<int>(),
                        h->iperm.as<int>(), n);
-    hipLaunchKernelGGL(k_cast_scale<T>, dim3((n + 255) / 256), dim3(256), 0, h->stream,
+    LHL_LAUNCH(h, "reduce_finish", k_cast_scale<T>, dim3((n + 255) / 256), dim3(256), 0,
                        (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(),
                        h->scaleT.as<T>(), h->iscaleT.as<T>(), n);
     LHL_HIP_TRY(hipGetLastError());
@@ -475,7 +475,7 @@ Status finish_typed(lhl_handle *h) {
 template <typename TJ, typename T>
 Status copy_scale_typed(lhl_handle *h, int balanced) {
     const int n = (int)h->n;
-    hipLaunchKernelGGL((k_copy_scale<TJ, T>), dim3((n + 255) / 256, n), dim3(256), 0, h->stream,
+    LHL_LAUNCH(h, "reduce_copy", (k_copy_scale<TJ, T>), dim3((n + 255) / 256, n), dim3(256), 0,
                        (const TJ *)h->J.as<TJ>(), h->ldj, h->A.as<T>(), h->lda, n,
                        (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(), balanced);
     LHL_HIP_TRY(hipGetLastError());
@@ -492,9 +492,9 @@ Status balance_device(lhl_handle *h) {
 Status reduce_device(lhl_handle *h, int balance) {
     const int n = (int)h->n;
     if (n == 0) return {};
-    hipLaunchKernelGGL(k_fill1, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->scale64.as<double>(),
+    LHL_LAUNCH(h, "reduce_copy", k_fill1, dim3((n + 255) / 256), dim3(256), 0, h->scale64.as<double>(),
                        h->iscale64.as<double>(), n);
-    hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->perm.as<int>(), n);
+    LHL_LAUNCH(h, "reduce_copy", k_iota, dim3((n + 255) / 256), dim3(256), 0, h->perm.as<int>(), n);
     LHL_HIP_TRY(hipGetLastError());
     if (balance) LHL_TRY(balance_device(h));
     if (h->j_dtype == LHL_F64) {

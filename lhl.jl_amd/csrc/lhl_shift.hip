@@ -261,28 +261,28 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
 }
 
 template <typename T, int NPER>
-void launch_shift(int cplx, int64_t nshift, hipStream_t st, const T *Ht, int64_t ldh, int n,
+void launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                   const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                   int64_t ss, int *info) {
     if (cplx)
-        hipLaunchKernelGGL((k_shift_cplx<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, st, Ht, ldh,
-                           n, prm, U, us, l, rd, ls, sw, ss, info);
+        LHL_LAUNCH(h, "shift", (k_shift_cplx<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, Ht, ldh, n,
+                   prm, U, us, l, rd, ls, sw, ss, info);
     else
-        hipLaunchKernelGGL((k_shift_real<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, st, Ht, ldh,
-                           n, prm, U, us, l, rd, ls, sw, ss, info);
+        LHL_LAUNCH(h, "shift", (k_shift_real<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, Ht, ldh, n,
+                   prm, U, us, l, rd, ls, sw, ss, info);
 }
 
 template <typename T>
-Status launch_shift_n(int cplx, int64_t nshift, hipStream_t st, const T *Ht, int64_t ldh, int n,
+Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
     const int nper = (n + kShiftThreads - 1) / kShiftThreads;
-    if (nper <= 1) launch_shift<T, 1>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 2) launch_shift<T, 2>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 4) launch_shift<T, 4>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 8) launch_shift<T, 8>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 16) launch_shift<T, 16>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 32) launch_shift<T, 32>(cplx, nshift, st, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    if (nper <= 1) launch_shift<T, 1>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    else if (nper <= 2) launch_shift<T, 2>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    else if (nper <= 4) launch_shift<T, 4>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    else if (nper <= 8) launch_shift<T, 8>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    else if (nper <= 16) launch_shift<T, 16>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    else if (nper <= 32) launch_shift<T, 32>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
     else return {LHL_ERR_ARG, "lhl_shift: n above 8192 is not supported by the register-resident shift kernel"};
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -303,7 +303,7 @@ Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, co
                                hipMemcpyHostToDevice, h->stream));
     // the host vector must outlive the async copy
     LHL_HIP_TRY(hipStreamSynchronize(h->stream));
-    return launch_shift_n<T>(cplx, nshift, h->stream, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
+    return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
                              params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info);
 }
 

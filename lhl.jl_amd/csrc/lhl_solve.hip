@@ -506,7 +506,7 @@ Status launch_ldiv_t(lhl_handle *h, const SolveArgs &a, const Group *groups, int
     const size_t lds = sizeof(T) * (size_t)NR * PL * n;
     auto fn = k_ldiv<T, TX, NR, PL, UC>;
     LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fn, dim3(ngroups), dim3(kSolveThreads), lds, h->stream, n, (const T *)h->A.as<T>(), h->lda,
+    LHL_LAUNCH(h, "solve", fn, dim3(ngroups), dim3(kSolveThreads), lds, n, (const T *)h->A.as<T>(), h->lda,
                        (const int *)h->perm.as<int>(), (const int *)h->iperm.as<int>(),
                        (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(),
                        (const T *)a.U, a.ustride, (const T *)a.l, (const T *)a.rdiag, a.lstride, a.swp, a.sstride,
@@ -591,7 +591,7 @@ Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t l
         auto fn = k_applyz<T, TX, 1>;
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
         const int groups = (int)(ncol * planes);
-        hipLaunchKernelGGL(fn, dim3(groups), dim3(kSolveThreads), lds1, h->stream, n, inverse,
+        LHL_LAUNCH(h, "applyz", fn, dim3(groups), dim3(kSolveThreads), lds1, n, inverse,
                            (const T *)h->A.as<T>(), h->lda, (const int *)h->perm.as<int>(),
                            (const int *)h->iperm.as<int>(), (const double *)h->scale64.as<double>(),
                            (const double *)h->iscale64.as<double>(), (TX *)x, ldx, xplane, planes, (int)ncol);
@@ -599,7 +599,7 @@ Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t l
         auto fn = k_applyz<T, TX, NV>;
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const int groups = (int)((ncol * planes + NV - 1) / NV);
-        hipLaunchKernelGGL(fn, dim3(groups), dim3(kSolveThreads), lds, h->stream, n, inverse,
+        LHL_LAUNCH(h, "applyz", fn, dim3(groups), dim3(kSolveThreads), lds, n, inverse,
                            (const T *)h->A.as<T>(), h->lda, (const int *)h->perm.as<int>(),
                            (const int *)h->iperm.as<int>(), (const double *)h->scale64.as<double>(),
                            (const double *)h->iscale64.as<double>(), (TX *)x, ldx, xplane, planes, (int)ncol);
@@ -627,10 +627,10 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
         ldw = h->ldj;
         wc = 0;
     }
-    hipLaunchKernelGGL((k_residual<TW, TX, NC>), grid, dim3(256), 0, h->stream, n, W, ldw, wc, wplane,
+    LHL_LAUNCH(h, "residual", (k_residual<TW, TX, NC>), grid, dim3(256), 0, n, W, ldw, wc, wplane,
                        a.W ? nullptr : a.colshift, (const TX *)a.b, (const TX *)a.x, (TX *)a.r, a.ld, a.plane,
                        planes, (int)a.ncol);
-    hipLaunchKernelGGL(k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, h->stream, n, (const TX *)a.r,
+    LHL_LAUNCH(h, "residual", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
                        (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms);
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -673,7 +673,7 @@ Status to_planar(lhl_handle *h, const void *src, int64_t ld_src, int src_dt, int
     if (n == 0 || ncol == 0) return {};
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)ncol);
 #define TP(TS, TD)                                                                                        \
-    hipLaunchKernelGGL((k_to_planar<TS, TD>), grid, dim3(256), 0, h->stream, (const TS *)src, ld_src,     \
+    LHL_LAUNCH(h, "convert", (k_to_planar<TS, TD>), grid, dim3(256), 0, (const TS *)src, ld_src,     \
                        src_complex, (TD *)dst, ld_dst, plane_dst, (int)n)
     if (src_dt == LHL_F64 && dst_dt == LHL_F64) TP(double, double);
     else if (src_dt == LHL_F64) TP(double, float);
@@ -689,7 +689,7 @@ Status from_planar(lhl_handle *h, const void *src, int64_t ld_src, int64_t plane
     if (n == 0 || ncol == 0) return {};
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)ncol);
 #define FP(TS, TD)                                                                                        \
-    hipLaunchKernelGGL((k_from_planar<TS, TD>), grid, dim3(256), 0, h->stream, (const TS *)src, ld_src,   \
+    LHL_LAUNCH(h, "convert", (k_from_planar<TS, TD>), grid, dim3(256), 0, (const TS *)src, ld_src,   \
                        plane_src, (TD *)dst, ld_dst, dst_complex, (int)n)
     if (src_dt == LHL_F64 && dst_dt == LHL_F64) FP(double, double);
     else if (src_dt == LHL_F64) FP(double, float);

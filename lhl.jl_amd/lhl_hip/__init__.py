@@ -35,7 +35,8 @@ import numpy as np
 __all__ = [
     "ArgumentError", "DimensionMismatch", "LHLError", "LHLWorkspace", "LHLShift", "lhl", "lhl_",
     "lhl_reduce_", "lhl_shift_", "lhl_ldiv_", "lhl_refine_", "applyZ_", "applyZinv_",
-    "shift_solve_batched", "library_path", "load_library",
+    "shift_solve_batched", "library_path", "load_library", "profile_enable", "profile_reset", "profile_read",
+    "sync",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -96,6 +97,10 @@ def load_library():
         "lhl_get_ht": ([vp, vp], ci),
         "lhl_get_shift_state": ([vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)], ci),
         "lhl_sync": ([vp], ci),
+        "lhl_get_stream": ([vp, ctypes.POINTER(vp)], ci),
+        "lhl_profile_enable": ([vp, ci], ci),
+        "lhl_profile_reset": ([vp], ci),
+        "lhl_profile_read": ([vp, ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -579,10 +584,11 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
     if sig.shape != tau.shape:
         raise DimensionMismatch("sigmas and taus differ in length")
     S = sig.shape[0]
-    cplx = bool(np.any(sig.imag != 0) or np.any(tau.imag != 0) or np.iscomplexobj(B) and not _is_torch(B))
+    cplx = bool(np.any(sig.imag != 0) or np.any(tau.imag != 0))
     if _is_torch(B):
-        import torch
-        cplx = cplx or B.is_complex()
+        cplx = cplx or bool(B.is_complex())
+    else:
+        cplx = cplx or bool(np.iscomplexobj(B))
     offs = np.asarray(rhs_offsets, dtype=np.int64)
     if offs.shape != (S + 1,):
         raise DimensionMismatch("rhs_offsets must have one entry more than the shifts")
@@ -611,3 +617,25 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
         ctypes.c_void_p(xv.ptr), xv.ld, bv.on_device, prec, float(tol), int(max_iter), info.ctypes.data,
         iters.ctypes.data))
     return X, info, iters
+
+
+def sync(ws: LHLWorkspace):
+    """Wait for the workspace's stream."""
+    _check(ws._h, _lib.lhl_sync(ws._h))
+
+
+def profile_enable(ws: LHLWorkspace, on: bool = True):
+    """Bracket every kernel launch of ws with HIP events on its stream."""
+    _check(ws._h, _lib.lhl_profile_enable(ws._h, int(bool(on))))
+
+
+def profile_reset(ws: LHLWorkspace):
+    _check(ws._h, _lib.lhl_profile_reset(ws._h))
+
+
+def profile_read(ws: LHLWorkspace, kernel: str):
+    """(launches, total milliseconds) of one kernel family since the last reset."""
+    c = ctypes.c_int64()
+    t = ctypes.c_double()
+    _check(ws._h, _lib.lhl_profile_read(ws._h, kernel.encode(), ctypes.byref(c), ctypes.byref(t)))
+    return int(c.value), float(t.value)

@@ -70,11 +70,19 @@ def test_reduction_blocked_sizes(lhl_hip, oracle, n):
 
 def test_reduction_float32(lhl_hip, oracle):
     say("case: float32 reduction n=300")
-    J = randn(4, 300, dtype=np.float32)
+    # Two Float32 reductions differ by summation order alone (~n eps in the
+    # factors), so each is compared to the Float64 reduction of the same J, as
+    # test/core_tests.jl:428-449 does: the GPU must be as accurate as the oracle.
+    n = 300
+    J = randn(4, n, dtype=np.float32)
     ws = lhl_hip.lhl(J)
     red = oracle.reduce(J, blocked=0)
+    r64 = oracle.reduce(J.astype(np.float64), blocked=0)
     assert np.array_equal(ws.ipiv - 1, red.ipiv)
-    assert relerr(ws.factors.astype(np.float64), red.A.astype(np.float64)) <= 1e-5
+    assert np.array_equal(red.ipiv, r64.ipiv)
+    e_gpu = relerr(ws.factors.astype(np.float64), r64.A)
+    e_orc = relerr(red.A.astype(np.float64), r64.A)
+    assert e_gpu <= 4 * e_orc + 100 * n * np.finfo(np.float32).eps
 
 
 def shift_state_equal(lhl_hip, oracle, ws, s, t, sh=None):
