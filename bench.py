@@ -57,10 +57,12 @@ def algorithmic(kernel, n, S, ncol, es):
     if kernel == "shift":
         # Ht's upper Hessenberg part read once, S packed U's written
         return (tri + n) * es + S * tri * es, "bytes", "hbm"
-    if kernel == "solve":
-        # every shift's U read once, the multipliers L read for the forward and
-        # the backward sweep, the right-hand sides read and written (FP64)
-        return S * tri * es + 2 * (n * n / 2) * es + 2 * ncol * n * 8, "bytes", "hbm"
+    if kernel == "hess":
+        # every shift's packed U read once, the batch's columns read and written
+        return S * tri * es + 2 * ncol * n * es, "bytes", "hbm"
+    if kernel in ("trsm", "trmm"):
+        # the multipliers' triangle and the columns once each: n^2 ncol FMAs
+        return 2.0 * (n * n / 2) * ncol, "flops", "valu"
     if kernel == "residual":
         # J x for every column in FP64: 2 n^2 flops per column
         return 2.0 * n * n * ncol, "flops", "mfma"
@@ -155,15 +157,20 @@ def main():
     lhl_hip.profile_reset(ws32)
     lhl_hip.profile_enable(ws32, True)
     step(ws32)
-    fams = ["shift", "solve", "residual", "convert"]
+    fams = ["shift", "gather", "trsm", "hess", "trmm", "residual", "convert"]
     prof = {k: lhl_hip.profile_read(ws32, k) for k in fams}
     lhl_hip.profile_enable(ws32, False)
     log("bench: kernels " + ", ".join(f"{k} {c}x {t:.2f} ms" for k, (c, t) in prof.items()))
-    dom = max(("shift", "solve", "residual"), key=lambda k: prof[k][1])
+    dom = max(("shift", "trsm", "hess", "trmm", "residual"), key=lambda k: prof[k][1])
     c, t = prof[dom]
     amt, unit, bound = algorithmic(dom, n, S, ncol, 4)
     avg_s = (t / c) / 1e3 if c else float("nan")
-    if bound == "hbm":
+    if bound == "valu":
+        achieved = amt / (t / 1e3) / 1e12      # all launches of the family in one pass
+        roof = {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                "frac": achieved / 157.3, "traffic": None, "kernel": dom, "launches": c, "avg_ms": t / c,
+                "algorithmic_flops": amt}
+    elif bound == "hbm":
         # one solve launch per refinement round covers the whole batch
         achieved = amt / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -71,7 +71,7 @@ Status stage_out(lhl_handle *h, const DevBuf &buf, int64_t ld, int64_t plane, in
 }
 
 int shift_planes_lstride(const lhl_shift_state *sh, int64_t n, int64_t &us, int64_t &ls, int64_t &ss) {
-    const int64_t uplane = ((upack_len(n) + 63) / 64) * 64;
+    const int64_t uplane = upack_plane(n);
     const int64_t lplane = ((n + 63) / 64) * 64;
     us = uplane * sh->planes;
     ls = lplane * sh->planes;
@@ -164,7 +164,7 @@ int lhl_destroy(lhl_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     prof_drain(h);
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
-                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->ybuf, &h->rbuf, &h->nbuf,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->Linv, &h->ybuf, &h->rbuf, &h->nbuf,
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {

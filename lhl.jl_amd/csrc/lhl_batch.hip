@@ -1,0 +1,471 @@
+// The solves for many right-hand sides at once (SciML/LHL.jl lhl_ldiv!,
+// src:3052-3123, applied to a batch), and the refinement residual.
+//
+// x <- Z (sigma I + tau H)^-1 Z^-1 x with Z = D P L runs as four launches over
+// all columns of the batch:
+//   gather   Y = P^-1 D^-1 X into the working precision           (src:2130-2140)
+//   L^-1     a blocked triangular solve over 64-index blocks: one launch per
+//            block, whose workgroups each take a (row tile, 64 columns) pair;
+//            every workgroup re-solves the block's 64x64 triangle for its
+//            columns (cheap, from LDS) and the workgroups below the block
+//            subtract the block's rank-64 product from their rows.  The block's
+//            own tile writes its finished rows to a second buffer, so no
+//            workgroup reads a row another one is rewriting.
+//   H solve  per shift: the Hessenberg forward chain and the blocked back
+//            substitution with the columns resident in LDS (lhl_solve.hip).
+//   L, D P   y <- L y is a triangular matrix product without recurrence: one
+//            launch of (row tile, column tile) workgroups, whose epilogue
+//            scatters x[perm[p]] = y[p] * d[perm[p]] (src:3064-3068).
+// In y' = y[1:n] the multipliers form the strictly lower triangle
+// L'[i, m] = A[i+1, m] (i > m): step m of the reference's sweeps.
+#include "lhl_common.h"
+
+namespace lhl {
+
+constexpr int kT = 64;   // tile edge
+
+// Y[i + q*ldy] = x[perm[i], col] * iscale[perm[i]] for internal column q = col*planes + pl;
+// row 0 is also copied to Y2 (the second buffer of the triangular solve).
+template <typename T, typename TX>
+__global__ void k_gather(int n, const int *__restrict__ perm, const double *__restrict__ iscale,
+                         const TX *__restrict__ x, int64_t ldx, int64_t xplane, int planes, T *__restrict__ Y,
+                         T *__restrict__ Y2, int64_t ldy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= n) return;
+    const int col = q / planes, pl = q % planes;
+    const int p = perm[i];
+    const T v = (T)(x[p + (int64_t)col * ldx + pl * xplane] * (TX)iscale[p]);
+    Y[i + (int64_t)q * ldy] = v;
+    if (i == 0) Y2[(int64_t)q * ldy] = v;
+}
+
+// Linv[b] = (L'[64b:64b+64, 64b:64b+64])^-1, column-major 64x64 per block: the
+// explicit inverse of each unit lower triangular diagonal block (the
+// reference forms its nb x nb panel inverse the same way, src:1942-1954).
+template <typename T>
+__global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int64_t lda, T *__restrict__ Linv) {
+    const int N1 = n - 1;
+    const int b = blockIdx.x;
+    const int r0 = b * kT;
+    const int nb = min(kT, N1 - r0);
+    __shared__ T Ls[kT][kT + 1];
+    for (int t = threadIdx.x; t < kT * kT; t += 64) {
+        const int i = t % kT, m = t / kT;
+        Ls[i][m] = (i > m && i < nb) ? A[(r0 + i + 1) + (int64_t)(r0 + m) * lda] : (T)0;
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    T x[kT];
+#pragma unroll
+    for (int i = 0; i < kT; i++) x[i] = (i == c) ? (T)1 : (T)0;
+#pragma unroll
+    for (int m = 0; m < kT - 1; m++) {
+        const T xm = x[m];
+#pragma unroll
+        for (int i = m + 1; i < kT; i++) x[i] = __builtin_fma(-Ls[i][m], xm, x[i]);
+    }
+    T *dst = Linv + (int64_t)b * kT * kT + (int64_t)c * kT;
+#pragma unroll
+    for (int i = 0; i < kT; i++) dst[i] = x[i];
+}
+
+// One block step b of y' <- L'^-1 y': the block's rows become Linv[b] * y'_b
+// (redundantly in every workgroup of the step, a 64x64x64 product); the
+// block's own tile stores them to Yo, the tiles below subtract L'[I, b] * y'_b.
+template <typename T>
+__global__ void __launch_bounds__(256) k_trsm_step(int n, const T *__restrict__ A, int64_t lda,
+                                                   const T *__restrict__ Linv, T *__restrict__ Yw,
+                                                   T *__restrict__ Yo, int64_t ldy, int ncols, int b) {
+    const int N1 = n - 1;
+    const int I = b + blockIdx.x;
+    const int c0 = blockIdx.y * kT;
+    const int r0 = b * kT;
+    const int nb = min(kT, N1 - r0);
+    const int tid = threadIdx.x;
+    __shared__ T Ls[kT][kT + 1];
+    __shared__ T Ys[kT][kT + 1];
+    const T *Lb = Linv + (int64_t)b * kT * kT;
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int m = t % kT, c = t / kT;
+        Ys[m][c] = (m < nb && c0 + c < ncols) ? Yw[(r0 + m + 1) + (int64_t)(c0 + c) * ldy] : (T)0;
+        Ls[m][c] = Lb[m + (int64_t)c * kT];     // Ls[i][m'] = Linv[b](i, m')
+    }
+    __syncthreads();
+    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    for (int m = 0; m < nb; m++) {
+        T lv[4], yv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
+#pragma unroll
+        for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) Ys[ti + a][tc + c] = acc[a][c];
+    if (I == b) {
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if (ti + a < nb && c0 + tc + c < ncols) Yo[(r0 + ti + a + 1) + (int64_t)(c0 + tc + c) * ldy] = acc[a][c];
+        return;
+    }
+    const int ri = I * kT;
+    const int ni = min(kT, N1 - ri);
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int i = t % kT, m = t / kT;
+        Ls[i][m] = (i < ni && m < nb) ? A[(ri + i + 1) + (int64_t)(r0 + m) * lda] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    for (int m = 0; m < nb; m++) {
+        T lv[4], yv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
+#pragma unroll
+        for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int col = c0 + tc + c;
+        if (col >= ncols) continue;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int i = ti + a;
+            if (i < ni) {
+                T *dst = Yw + (ri + i + 1) + (int64_t)col * ldy;
+                *dst = *dst - acc[a][c];
+            }
+        }
+    }
+}
+
+// y_new' = y' + L' y' for one (row tile, column tile), scattered to
+// x[perm[p], col] (+)= y_new[p] * d[perm[p]], p = i' + 1; the row-0 tile also
+// scatters p = 0.
+template <typename T, typename TX>
+__global__ void __launch_bounds__(256) k_trmm_scatter(int n, const T *__restrict__ A, int64_t lda,
+                                                      const T *__restrict__ Y, int64_t ldy, int ncols, int planes,
+                                                      const int *__restrict__ perm, const double *__restrict__ scale,
+                                                      TX *__restrict__ x, int64_t ldx, int64_t xplane,
+                                                      int accumulate) {
+    const int N1 = n - 1;
+    const int I = blockIdx.x;
+    const int c0 = blockIdx.y * kT;
+    const int ri = I * kT;
+    const int ni = min(kT, N1 - ri);
+    const int tid = threadIdx.x;
+    __shared__ T Ls[kT][kT + 1];
+    __shared__ T Ys[kT][kT + 1];
+    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    for (int K = 0; K <= I; K++) {
+        const int rk = K * kT;
+        const int nk = min(kT, N1 - rk);
+        __syncthreads();
+        for (int t = tid; t < kT * kT; t += 256) {
+            const int i = t % kT, m = t / kT;
+            const bool ok = i < ni && m < nk && (K < I || m < i);
+            Ls[i][m] = ok ? A[(ri + i + 1) + (int64_t)(rk + m) * lda] : (T)0;
+            const int mm = t % kT, c = t / kT;
+            Ys[mm][c] = (mm < nk && c0 + c < ncols) ? Y[(rk + mm + 1) + (int64_t)(c0 + c) * ldy] : (T)0;
+        }
+        __syncthreads();
+        for (int m = 0; m < nk; m++) {
+            T lv[4], yv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
+#pragma unroll
+            for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int q = c0 + tc + c;
+        if (q >= ncols) continue;
+        const int col = q / planes, pl = q % planes;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int i = ti + a;
+            if (i >= ni) continue;
+            const int p = ri + i + 1;
+            const T yn = Y[p + (int64_t)q * ldy] + acc[a][c];
+            const int dst_row = perm[p];
+            TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+            const TX v = (TX)yn * (TX)scale[dst_row];
+            if (accumulate) *dst += v;
+            else *dst = v;
+        }
+    }
+    if (I == 0 && tid < kT && c0 + tid < ncols) {
+        const int q = c0 + tid;
+        const int col = q / planes, pl = q % planes;
+        const int dst_row = perm[0];
+        TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+        const TX v = (TX)Y[(int64_t)q * ldy] * (TX)scale[dst_row];
+        if (accumulate) *dst += v;
+        else *dst = v;
+    }
+}
+
+// ---- the refinement residual: JX = J X (FP64 tiled GEMM), then
+// r = b - (sigma x + tau JX) per column with its shift, and column norms.
+// X, B, R are in the staged planar layout (ld, plane); JX uses the internal
+// column order q = col*planes + pl with leading dimension ldq.
+template <typename TW, typename TX>
+__global__ void __launch_bounds__(256) k_gemm_jx(int n, const TW *__restrict__ J, int64_t ldj,
+                                                 const TX *__restrict__ X, int64_t ld, int64_t plane, int planes,
+                                                 int ncols, TX *__restrict__ JX, int64_t ldq) {
+    constexpr int KC = 32;
+    const int r0 = blockIdx.x * kT, c0 = blockIdx.y * kT;
+    const int tid = threadIdx.x;
+    __shared__ TX Js[KC][kT + 1];   // Js[k][i] = J[r0+i, k0+k]
+    __shared__ TX Xs[KC][kT + 1];   // Xs[k][c] = X[k0+k, q=c0+c]
+    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
+    TX acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    for (int k0 = 0; k0 < n; k0 += KC) {
+        __syncthreads();
+        for (int t = tid; t < KC * kT; t += 256) {
+            const int i = t % kT, k = t / kT;
+            Js[k][i] = (r0 + i < n && k0 + k < n) ? (TX)J[(r0 + i) + (int64_t)(k0 + k) * ldj] : (TX)0;
+            const int kk = t % KC, c = t / KC;
+            const int q = c0 + c;
+            TX v = 0;
+            if (k0 + kk < n && q < ncols) {
+                const int col = q / planes, pl = q % planes;
+                v = X[(k0 + kk) + (int64_t)col * ld + pl * plane];
+            }
+            Xs[kk][c] = v;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < KC; k++) {
+            TX jv[4], xv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) jv[a] = Js[k][ti + a];
+#pragma unroll
+            for (int c = 0; c < 4; c++) xv[c] = Xs[k][tc + c];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(jv[a], xv[c], acc[a][c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int q = c0 + tc + c;
+        if (q >= ncols) continue;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            if (r0 + ti + a < n) JX[(r0 + ti + a) + (int64_t)q * ldq] = acc[a][c];
+    }
+}
+
+// dense W x for a user-supplied W (planar complex when w_complex): same GEMM with
+// the complex combination done by k_resid_combine.
+template <typename TX>
+__global__ void k_resid_combine(int n, const TX *__restrict__ JX, int64_t ldq, const TX *__restrict__ JXi,
+                                const TX *__restrict__ X, const TX *__restrict__ B, TX *__restrict__ R, int64_t ld,
+                                int64_t plane, int planes, const double *__restrict__ colshift, int w_complex) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (i >= n) return;
+    const int64_t o = i + (int64_t)col * ld;
+    if (colshift) {
+        const TX sr = (TX)colshift[4 * col], si = (TX)colshift[4 * col + 1];
+        const TX tr = (TX)colshift[4 * col + 2], ti = (TX)colshift[4 * col + 3];
+        const TX xr = X[o], xi = planes > 1 ? X[o + plane] : (TX)0;
+        const TX jr = JX[i + (int64_t)(col * planes) * ldq];
+        const TX ji = planes > 1 ? JX[i + (int64_t)(col * planes + 1) * ldq] : (TX)0;
+        const TX wr = (sr * xr - si * xi) + (tr * jr - ti * ji);
+        const TX wi = (sr * xi + si * xr) + (tr * ji + ti * jr);
+        R[o] = B[o] - wr;
+        if (planes > 1) R[o + plane] = B[o + plane] - wi;
+    } else if (w_complex) {
+        // JX = Wr X (both planes), JXi = Wi X (both planes): W x = (Wr xr - Wi xi) + i (Wr xi + Wi xr)
+        const TX a = JX[i + (int64_t)(col * 2) * ldq], b = JX[i + (int64_t)(col * 2 + 1) * ldq];
+        const TX c = JXi[i + (int64_t)(col * 2) * ldq], d = JXi[i + (int64_t)(col * 2 + 1) * ldq];
+        R[o] = B[o] - (a - d);
+        R[o + plane] = B[o + plane] - (b + c);
+    } else {
+        for (int pl = 0; pl < planes; pl++)
+            R[o + pl * plane] = B[o + pl * plane] - JX[i + (int64_t)(col * planes + pl) * ldq];
+    }
+}
+
+
+// norms[c] = ||r_c||_inf, norms[ncol + c] = ||b_c||_inf (complex: the modulus)
+template <typename TX>
+__global__ void k_colnorms(int n, const TX *__restrict__ r, const TX *__restrict__ b, int64_t ld, int64_t plane,
+                           int planes, int ncol, double *__restrict__ norms) {
+    const int c = blockIdx.x;
+    double mr = 0, mb = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int64_t o = i + (int64_t)c * ld;
+        double vr, vb;
+        if (planes > 1) {
+            vr = hypot((double)r[o], (double)r[o + plane]);
+            vb = hypot((double)b[o], (double)b[o + plane]);
+        } else {
+            vr = fabs((double)r[o]);
+            vb = fabs((double)b[o]);
+        }
+        mr = fmax(mr, vr);
+        mb = fmax(mb, vb);
+    }
+    __shared__ double sr[256], sb[256];
+    sr[threadIdx.x] = mr;
+    sb[threadIdx.x] = mb;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + o]);
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        norms[c] = sr[0];
+        norms[ncol + c] = sb[0];
+    }
+}
+
+namespace {
+
+inline int64_t ldy_for(int64_t n) { return std::max<int64_t>(64, ((n + 63) / 64) * 64); }
+
+template <typename T, typename TX>
+Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
+    const int n = (int)h->n;
+    const int planes = a.x_complex ? 2 : 1;
+    const int Q = (int)(a.ncol * planes);
+    const int64_t ldy = ldy_for(n);
+    LHL_TRY(h->ybuf.ensure(sizeof(T) * (size_t)ldy * Q));
+    LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
+    T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
+    const T *A = h->A.as<T>();
+    const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
+    LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
+               (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo, ldy);
+    const int N1 = n - 1;
+    const int nbt = (N1 + kT - 1) / kT;
+    const unsigned ct = (unsigned)((Q + kT - 1) / kT);
+    const T *Linv = h->Linv.as<T>();
+    for (int b = 0; b < nbt; b++)
+        LHL_LAUNCH(h, "trsm", k_trsm_step<T>, dim3((unsigned)(nbt - b), ct), dim3(256), 0, n, A, h->lda, Linv, Yw, Yo,
+                   ldy, Q, b);
+    LHL_HIP_TRY(hipGetLastError());
+    LHL_TRY(hess_device(h, a, Yo, ldy));
+    LHL_LAUNCH(h, "trmm", (k_trmm_scatter<T, TX>), dim3((unsigned)std::max(nbt, 1), ct), dim3(256), 0, n, A, h->lda,
+               (const T *)Yo, ldy, Q, planes, (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(),
+               (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+template <typename TW, typename TX>
+Status residual_typed(lhl_handle *h, const ResidArgs &a) {
+    const int n = (int)h->n;
+    const int planes = a.x_complex ? 2 : 1;
+    const int Q = (int)(a.ncol * planes);
+    const int64_t ldq = ldy_for(n);
+    LHL_TRY(h->ybuf.ensure(sizeof(TX) * (size_t)ldq * Q));
+    TX *JX = h->ybuf.as<TX>();
+    TX *JXi = nullptr;
+    const dim3 gg((unsigned)((n + kT - 1) / kT), (unsigned)((Q + kT - 1) / kT));
+    if (a.W) {
+        LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)a.W, a.ldw, (const TX *)a.x,
+                   a.ld, a.plane, planes, Q, JX, ldq);
+        if (a.w_complex) {
+            LHL_TRY(h->rbuf.ensure(sizeof(TX) * (size_t)ldq * Q));
+            JXi = h->rbuf.as<TX>();
+            const TW *Wi = (const TW *)a.W + a.ldw * (int64_t)n;
+            LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, Wi, a.ldw, (const TX *)a.x, a.ld,
+                       a.plane, planes, Q, JXi, ldq);
+        }
+    } else {
+        LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)h->J.p, h->ldj,
+                   (const TX *)a.x, a.ld, a.plane, planes, Q, JX, ldq);
+    }
+    const dim3 gc((unsigned)((n + 255) / 256), (unsigned)a.ncol);
+    LHL_LAUNCH(h, "residual", k_resid_combine<TX>, gc, dim3(256), 0, n, (const TX *)JX, ldq, (const TX *)JXi,
+               (const TX *)a.x, (const TX *)a.b, (TX *)a.r, a.ld, a.plane, planes, a.W ? nullptr : a.colshift,
+               a.W ? a.w_complex : 0);
+    LHL_LAUNCH(h, "residual", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
+               (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+template <typename T>
+Status linv_typed(lhl_handle *h) {
+    const int n = (int)h->n;
+    const int nbt = (n - 1 + kT - 1) / kT;
+    LHL_TRY(h->Linv.ensure(sizeof(T) * (size_t)std::max(nbt, 1) * kT * kT));
+    if (nbt > 0)
+        LHL_LAUNCH(h, "reduce_finish", k_linv<T>, dim3((unsigned)nbt), dim3(64), 0, n, (const T *)h->A.as<T>(), h->lda,
+                   h->Linv.as<T>());
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+}  // namespace
+
+Status linv_device(lhl_handle *h) {
+    if (h->n < 2) return {};
+    return h->dtype == LHL_F64 ? linv_typed<double>(h) : linv_typed<float>(h);
+}
+
+Status solve_device(lhl_handle *h, const SolveArgs &a) {
+    if (h->n == 0 || a.ncol == 0) return {};
+    if (h->dtype == LHL_F64) {
+        if (a.xdt != LHL_F64) return {LHL_ERR_ARG, "lhl_ldiv: a Float32 vector cannot take a Float64 solve"};
+        return solve_batch_typed<double, double>(h, a);
+    }
+    if (a.xdt == LHL_F64) return solve_batch_typed<float, double>(h, a);
+    return solve_batch_typed<float, float>(h, a);
+}
+
+Status residual_device(lhl_handle *h, const ResidArgs &a) {
+    if (h->n == 0 || a.ncol == 0) return {};
+    const int wdt = a.W ? a.xdt : h->j_dtype;
+    if (a.xdt == LHL_F64) {
+        if (wdt == LHL_F64) return residual_typed<double, double>(h, a);
+        return residual_typed<float, double>(h, a);
+    }
+    if (wdt == LHL_F64) return residual_typed<double, float>(h, a);
+    return residual_typed<float, float>(h, a);
+}
+
+}  // namespace lhl

@@ -12,6 +12,16 @@
 
 namespace lhl {
 
+// A workgroup barrier that orders LDS only: s_waitcnt lgkmcnt(0) + s_barrier.
+// Outstanding global loads (prefetches) and stores stay in flight across it;
+// use it only where no thread reads global memory another thread of the same
+// workgroup wrote in the kernel.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 
@@ -51,6 +61,9 @@ inline int64_t pad_ld(int64_t n, int dtype) {
 // Offset of row k of the packed upper-triangular U (row k holds U[k, k..n-1]).
 __host__ __device__ inline int64_t upack_off(int64_t n, int64_t k) { return k * n - (k * (k - 1)) / 2; }
 inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
+// Elements per plane of one shift's U: the packed triangle plus 256 spare
+// elements at the end (the shift kernel's dump slots), rounded to 64.
+inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 256 + 63) / 64) * 64; }
 
 // Growable device allocation.
 struct DevBuf {
@@ -124,6 +137,7 @@ struct lhl_handle {
     int reduced = 0;
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
+    lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L (solve)
     lhl::DevBuf ybuf, rbuf, nbuf, xstage, bstage;  // solve / refine scratch
     lhl::Profiler prof;
     lhl_shift_state *own = nullptr;   // ws.shift
@@ -190,7 +204,11 @@ struct SolveArgs {
     int64_t ldx = 0, xplane = 0;
     int accumulate = 0;
 };
-Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_solve.hip
+Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
+// the inverses of L's 64x64 diagonal blocks, formed once per reduction
+Status linv_device(lhl_handle *h);                          // lhl_batch.hip
+// the per-shift Hessenberg solves on the internal column layout Y (q = col*planes + pl)
+Status hess_device(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy);   // lhl_solve.hip
 Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane,
                      int xdt, int x_complex);              // lhl_solve.hip
 
@@ -211,7 +229,7 @@ struct ResidArgs {
     int x_complex = 0;
     double *norms = nullptr;
 };
-Status residual_device(lhl_handle *h, const ResidArgs &a);  // lhl_solve.hip
+Status residual_device(lhl_handle *h, const ResidArgs &a);  // lhl_batch.hip
 
 // Planar <-> interleaved conversions and precision casts (lhl_solve.hip).
 Status to_planar(lhl_handle *h, const void *src, int64_t ld_src, int src_dt, int src_complex,

@@ -511,7 +511,7 @@ Status reduce_device(lhl_handle *h, int balance) {
         LHL_TRY(steps_typed<float>(h));
         LHL_TRY(finish_typed<float>(h));
     }
-    return {};
+    return linv_device(h);
 }
 
 }  // namespace lhl

@@ -28,6 +28,12 @@ struct ShiftParam {
 };
 
 // ---------------------------------------------------------------- real shifts
+// The step body is straight-line code: every load is unconditional (clamped,
+// in-bounds addresses), both eliminations are formed and a select keeps the
+// decided one, and stores of lanes with nothing to store go to a per-shift
+// dump area after U.  With no branch between a load and its use, the
+// compiler waits only for the load it needs, so the next row of H streams in
+// while the current step runs.
 template <typename T, int NPER>
 __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
@@ -39,11 +45,13 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
     const int s = blockIdx.x;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
+    T *dump = U + ustride - kShiftThreads;        // kShiftThreads spare elements per shift
     T *lv = lbase + (int64_t)s * lstride;
     T *rd = rdbase + (int64_t)s * lstride;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x;
     __shared__ T slot[2];
+    __shared__ T ldump[kShiftThreads];
 
     T r[NPER], h[NPER];
 #pragma unroll
@@ -52,62 +60,58 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
     }
     if (tid == 0) r[0] = r[0] + sigma;
-    // prefetch row 1 of H (column 1 of Ht)
 #pragma unroll
-    for (int m = 0; m < NPER; m++) {
-        const int j = tid + m * kShiftThreads;
-        h[m] = (n > 1 && j < n) ? Ht[j + ldh] : (T)0;
-    }
+    for (int m = 0; m < NPER; m++) h[m] = Ht[min(tid + m * kShiftThreads, n - 1) + (int64_t)min(1, n - 1) * ldh];
     if (tid == 0) slot[0] = r[0];
+    // b = tau * H[k+1, k] of the next step, loaded one step ahead
+    T bnext = tau * Ht[(int64_t)min(1, n - 1) * ldh];
     __syncthreads();
     int info = 0;
     for (int k = 0; k < n - 1; k++) {
         const T a = slot[k & 1];
-        const T b = tau * Ht[k + (int64_t)(k + 1) * ldh];
-        const bool sw_k = dabs(b) > dabs(a);
-        T l;
-        if (sw_k) l = a / b;
-        else if (a == (T)0) { if (info == 0) info = k + 1; l = (T)0; }
-        else l = b / a;
-        T *Urow = U + upack_off(n, k) - k;     // Urow[j] = U[k, j]
-        if (tid == 0) {
-            const T ukk = sw_k ? b : a;
-            Urow[k] = ukk;
-            rd[k] = (T)1 / ukk;
-            lv[k] = l;
-            sw[k] = (uint8_t)sw_k;
-        }
-        // prefetch row k+2 of H for the next step
+        const T b = bnext;
+        bnext = tau * Ht[min(k + 1, n - 1) + (int64_t)min(k + 2, n - 1) * ldh];
         T hn[NPER];
-        const bool more = k + 2 < n;
+        const int64_t coln = (int64_t)min(k + 2, n - 1) * ldh;
+#pragma unroll
+        for (int m = 0; m < NPER; m++) hn[m] = Ht[min(tid + m * kShiftThreads, n - 1) + coln];
+        // the decision (src:2545-2565): swap iff |b| > |a|; l = a/b, b/a, or 0 for a zero pivot
+        const bool sw_k = dabs(b) > dabs(a);
+        const bool zero = !sw_k && a == (T)0;
+        const T num = sw_k ? a : b, den = sw_k ? b : a;
+        const T q = num / den;
+        const T l = zero ? (T)0 : q;
+        info = (zero && info == 0) ? k + 1 : info;
+        const T ukk = sw_k ? b : a;
+        const T rdk = (T)1 / ukk;
+        T *Urow = U + upack_off(n, k) - k;     // Urow[j] = U[k, j]
+        const bool t0 = tid == 0;
+        *(t0 ? Urow + k : dump + tid) = ukk;
+        *(t0 ? rd + k : dump + tid) = rdk;
+        *(t0 ? lv + k : dump + tid) = l;
+        *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        // both eliminations with the reference's operation order (src:2550-2572)
+        const T ls = l * sigma;
 #pragma unroll
         for (int m = 0; m < NPER; m++) {
             const int j = tid + m * kShiftThreads;
-            hn[m] = (more && j > k + 1 && j < n) ? Ht[j + (int64_t)(k + 2) * ldh] : (T)0;
-        }
-#pragma unroll
-        for (int m = 0; m < NPER; m++) {
-            const int j = tid + m * kShiftThreads;
-            if (j > k && j < n) {
-                T u;
-                if (sw_k) {
-                    const T g = tau * h[m];
-                    T rj = r[m] - l * g;
-                    u = g;
-                    if (j == k + 1) { u = g + sigma; rj = rj - l * sigma; }
-                    r[m] = rj;
-                } else {
-                    u = r[m];
-                    T rj = tau * h[m] - l * r[m];
-                    if (j == k + 1) rj = rj + sigma;
-                    r[m] = rj;
-                }
-                Urow[j] = u;
-                if (j == k + 1) slot[(k + 1) & 1] = r[m];
-            }
+            const bool active = j > k && j < n;
+            const bool first = j == k + 1;
+            const T rj = r[m];
+            const T g = tau * h[m];
+            const T rs = rj - l * g;               // swap: pending row minus l * (new row)
+            const T rn = g - l * rj;               // no swap: new row minus l * pending row
+            const T us = first ? g + sigma : g;
+            const T r_s = first ? rs - ls : rs;
+            const T r_n = first ? rn + sigma : rn;
+            const T u = sw_k ? us : rj;
+            const T rnew = sw_k ? r_s : r_n;
+            r[m] = active ? rnew : rj;
+            *(active ? Urow + j : dump + tid) = u;
+            *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
             h[m] = hn[m];
         }
-        __syncthreads();
+        lds_barrier();   // the slot is the only data shared between steps
     }
     if (n >= 1) {
         // the last diagonal: U[n-1, n-1] = r[n-1]
@@ -151,6 +155,24 @@ __device__ __forceinline__ void cdiv(T a, T b, T c, T d, T &p, T &q) {
     else { cdiv1(b, a, d, c, p, q); q = -q; }
 }
 
+// cdiv with selects in place of its branches: each chosen result is computed
+// by the same operations as cdiv's branch, so the two agree bit for bit.
+template <typename T>
+__device__ __forceinline__ void cdiv_sel(T a, T b, T c, T d, T &p, T &q) {
+#pragma clang fp contract(off)
+    const bool sw = dabs(d) > dabs(c);
+    const T A = sw ? b : a, B = sw ? a : b, C = sw ? d : c, D = sw ? c : d;
+    const T r = D / C;
+    const T t = (T)1 / (C + D * r);
+    const T br = B * r;
+    const T p1 = (A + br) * t, p2 = A * t + (B * t) * r, p3 = (A + D * (B / C)) * t;
+    const T ar = -A * r;
+    const T q1 = (B + ar) * t, q2 = B * t + (-A * t) * r, q3 = (B + D * (-A / C)) * t;
+    p = (r != (T)0) ? ((br != (T)0) ? p1 : p2) : p3;
+    const T qq = (r != (T)0) ? ((ar != (T)0) ? q1 : q2) : q3;
+    q = sw ? -qq : qq;
+}
+
 template <typename T, int NPER>
 __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
@@ -163,11 +185,13 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     const T sr = prm[s].sr, si = prm[s].si, tr = prm[s].tr, ti = prm[s].ti;
     const int64_t up = ustride / 2, lp = lstride / 2;
     T *Ur = Ubase + (int64_t)s * ustride, *Ui = Ur + up;
+    T *dump = Ur + ustride - kShiftThreads;      // spare tail of the imaginary plane
     T *lr_ = lbase + (int64_t)s * lstride, *li_ = lr_ + lp;
     T *rdr = rdbase + (int64_t)s * lstride, *rdi = rdr + lp;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x;
     __shared__ T slot[2][2];
+    __shared__ T ldump[2][kShiftThreads];
 
     T rr[NPER], ri[NPER], h[NPER];
 #pragma unroll
@@ -179,68 +203,68 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     }
     if (tid == 0) { rr[0] = rr[0] + sr; ri[0] = ri[0] + si; }
 #pragma unroll
-    for (int m = 0; m < NPER; m++) {
-        const int j = tid + m * kShiftThreads;
-        h[m] = (n > 1 && j < n) ? Ht[j + ldh] : (T)0;
-    }
+    for (int m = 0; m < NPER; m++) h[m] = Ht[min(tid + m * kShiftThreads, n - 1) + (int64_t)min(1, n - 1) * ldh];
     if (tid == 0) { slot[0][0] = rr[0]; slot[0][1] = ri[0]; }
+    T hnext = Ht[(int64_t)min(1, n - 1) * ldh];
     __syncthreads();
     int info = 0;
     for (int k = 0; k < n - 1; k++) {
         const T ar = slot[k & 1][0], ai = slot[k & 1][1];
-        const T hk = Ht[k + (int64_t)(k + 1) * ldh];
+        const T hk = hnext;
+        hnext = Ht[min(k + 1, n - 1) + (int64_t)min(k + 2, n - 1) * ldh];
+        T hn[NPER];
+        const int64_t coln = (int64_t)min(k + 2, n - 1) * ldh;
+#pragma unroll
+        for (int m = 0; m < NPER; m++) hn[m] = Ht[min(tid + m * kShiftThreads, n - 1) + coln];
         const T br = tr * hk, bi = ti * hk;
         const bool sw_k = (dabs(br) + dabs(bi)) > (dabs(ar) + dabs(ai));
-        T lre, lim;
-        if (sw_k) cdiv(ar, ai, br, bi, lre, lim);
-        else if (ar == (T)0 && ai == (T)0) { if (info == 0) info = k + 1; lre = 0; lim = 0; }
-        else cdiv(br, bi, ar, ai, lre, lim);
+        const bool zero = !sw_k && ar == (T)0 && ai == (T)0;
+        T qr, qi;
+        cdiv_sel(sw_k ? ar : br, sw_k ? ai : bi, sw_k ? br : ar, sw_k ? bi : ai, qr, qi);
+        const T lre = zero ? (T)0 : qr, lim = zero ? (T)0 : qi;
+        info = (zero && info == 0) ? k + 1 : info;
         const int64_t ro = upack_off(n, k) - k;
-        if (tid == 0) {
-            const T ur = sw_k ? br : ar, ui = sw_k ? bi : ai;
-            Ur[ro + k] = ur; Ui[ro + k] = ui;
-            T qr, qi;
-            cdiv((T)1, (T)0, ur, ui, qr, qi);
-            rdr[k] = qr; rdi[k] = qi;
-            lr_[k] = lre; li_[k] = lim;
-            sw[k] = (uint8_t)sw_k;
-        }
-        T hn[NPER];
-        const bool more = k + 2 < n;
+        const T ukr = sw_k ? br : ar, uki = sw_k ? bi : ai;
+        T dr, di;
+        cdiv_sel((T)1, (T)0, ukr, uki, dr, di);
+        const bool t0 = tid == 0;
+        *(t0 ? Ur + ro + k : dump + tid) = ukr;
+        *(t0 ? Ui + ro + k : dump + tid) = uki;
+        *(t0 ? rdr + k : dump + tid) = dr;
+        *(t0 ? rdi + k : dump + tid) = di;
+        *(t0 ? lr_ + k : dump + tid) = lre;
+        *(t0 ? li_ + k : dump + tid) = lim;
+        *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        // l * sigma for the swap branch's first element
+        const T qsr = lre * sr - lim * si, qsi = lre * si + lim * sr;
 #pragma unroll
         for (int m = 0; m < NPER; m++) {
             const int j = tid + m * kShiftThreads;
-            hn[m] = (more && j > k + 1 && j < n) ? Ht[j + (int64_t)(k + 2) * ldh] : (T)0;
-        }
-#pragma unroll
-        for (int m = 0; m < NPER; m++) {
-            const int j = tid + m * kShiftThreads;
-            if (j > k && j < n) {
-                T ur, ui;
-                const T gr = tr * h[m], gi = ti * h[m];
-                if (sw_k) {
-                    const T pr = lre * gr - lim * gi, pi = lre * gi + lim * gr;
-                    T xr = rr[m] - pr, xi = ri[m] - pi;
-                    ur = gr; ui = gi;
-                    if (j == k + 1) {
-                        ur = gr + sr; ui = gi + si;
-                        const T qr = lre * sr - lim * si, qi = lre * si + lim * sr;
-                        xr = xr - qr; xi = xi - qi;
-                    }
-                    rr[m] = xr; ri[m] = xi;
-                } else {
-                    ur = rr[m]; ui = ri[m];
-                    const T pr = lre * ur - lim * ui, pi = lre * ui + lim * ur;
-                    T xr = gr - pr, xi = gi - pi;
-                    if (j == k + 1) { xr = xr + sr; xi = xi + si; }
-                    rr[m] = xr; ri[m] = xi;
-                }
-                Ur[ro + j] = ur; Ui[ro + j] = ui;
-                if (j == k + 1) { slot[(k + 1) & 1][0] = rr[m]; slot[(k + 1) & 1][1] = ri[m]; }
-            }
+            const bool active = j > k && j < n;
+            const bool first = j == k + 1;
+            const T xr0 = rr[m], xi0 = ri[m];
+            const T gr = tr * h[m], gi = ti * h[m];
+            // swap: r - l*g, then - l*sigma on the first element; U row = g (+ sigma)
+            const T pr = lre * gr - lim * gi, pi = lre * gi + lim * gr;
+            const T s1r = xr0 - pr, s1i = xi0 - pi;
+            const T s2r = s1r - qsr, s2i = s1i - qsi;
+            const T usr = first ? gr + sr : gr, usi = first ? gi + si : gi;
+            // no swap: U row = r; r = g - l*r (+ sigma on the first element)
+            const T pr2 = lre * xr0 - lim * xi0, pi2 = lre * xi0 + lim * xr0;
+            const T n1r = gr - pr2, n1i = gi - pi2;
+            const T n2r = n1r + sr, n2i = n1i + si;
+            const T ur = sw_k ? usr : xr0, ui = sw_k ? usi : xi0;
+            const T xr = sw_k ? (first ? s2r : s1r) : (first ? n2r : n1r);
+            const T xi = sw_k ? (first ? s2i : s1i) : (first ? n2i : n1i);
+            rr[m] = active ? xr : xr0;
+            ri[m] = active ? xi : xi0;
+            *(active ? Ur + ro + j : dump + tid) = ur;
+            *(active ? Ui + ro + j : dump + tid) = ui;
+            *(first ? &slot[(k + 1) & 1][0] : &ldump[0][tid]) = xr;
+            *(first ? &slot[(k + 1) & 1][1] : &ldump[1][tid]) = xi;
             h[m] = hn[m];
         }
-        __syncthreads();
+        lds_barrier();   // the slot is the only data shared between steps
     }
     if (n >= 1) {
         const int own = (n - 1) % kShiftThreads, mm = (n - 1) / kShiftThreads;
@@ -316,7 +340,7 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
     pool.n = n;
     pool.count = nshift;
     pool.planes = is_complex ? 2 : 1;
-    const int64_t uplane = ((upack_len(n) + 63) / 64) * 64;
+    const int64_t uplane = upack_plane(n);
     pool.ustride = uplane * pool.planes;
     const int64_t lplane = ((n + 63) / 64) * 64;
     LHL_TRY(pool.U.ensure(es * (size_t)pool.ustride * (size_t)nshift));
@@ -340,7 +364,7 @@ Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t /*nshift*/, cons
     const int64_t n = h->n;
     const size_t es = elt_size(h->dtype);
     const int planes = sh->is_complex ? 2 : 1;
-    const int64_t uplane = ((upack_len(n) + 63) / 64) * 64;
+    const int64_t uplane = upack_plane(n);
     const int64_t lplane = ((n + 63) / 64) * 64;
     LHL_TRY(sh->U.ensure(es * (size_t)uplane * planes));
     LHL_TRY(sh->l.ensure(es * (size_t)lplane * planes));
