@@ -19,6 +19,7 @@
 // In y' = y[1:n] the multipliers form the strictly lower triangle
 // L'[i, m] = A[i+1, m] (i > m): step m of the reference's sweeps.
 #include "lhl_common.h"
+#include "lhl_mfma.h"
 
 namespace lhl {
 
@@ -71,7 +72,7 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
 }
 
 // One block step b of y' <- L'^-1 y': the block's rows become Linv[b] * y'_b
-// (redundantly in every workgroup of the step, a 64x64x64 product); the
+// (redundantly in every workgroup of the step: a 64x64x64 MFMA product); the
 // block's own tile stores them to Yo, the tiles below subtract L'[I, b] * y'_b.
 template <typename T>
 __global__ void __launch_bounds__(256) k_trsm_step(int n, const T *__restrict__ A, int64_t lda,
@@ -83,78 +84,45 @@ __global__ void __launch_bounds__(256) k_trsm_step(int n, const T *__restrict__ 
     const int r0 = b * kT;
     const int nb = min(kT, N1 - r0);
     const int tid = threadIdx.x;
-    __shared__ T Ls[kT][kT + 1];
-    __shared__ T Ys[kT][kT + 1];
+    __shared__ T As[kT][kT + kTilePad];   // As[m][i]: A operand (i, m)
+    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c]: B operand (m, c)
     const T *Lb = Linv + (int64_t)b * kT * kT;
     for (int t = tid; t < kT * kT; t += 256) {
-        const int m = t % kT, c = t / kT;
-        Ys[m][c] = (m < nb && c0 + c < ncols) ? Yw[(r0 + m + 1) + (int64_t)(c0 + c) * ldy] : (T)0;
-        Ls[m][c] = Lb[m + (int64_t)c * kT];     // Ls[i][m'] = Linv[b](i, m')
+        const int x = t % kT, y = t / kT;
+        As[y][x] = Lb[x + (int64_t)y * kT];
+        Bs[x][y] = (x < nb && c0 + y < ncols) ? Yw[(r0 + x + 1) + (int64_t)(c0 + y) * ldy] : (T)0;
     }
     __syncthreads();
-    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
-    T acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
-    for (int m = 0; m < nb; m++) {
-        T lv[4], yv[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
-#pragma unroll
-        for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
-    }
+    typename Mfma<T>::acc_t acc[2][2];
+    mfma_tile64_zero<T>(acc);
+    mfma_tile64<T>(As, Bs, kT, acc);
     __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) Ys[ti + a][tc + c] = acc[a][c];
-    if (I == b) {
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (ti + a < nb && c0 + tc + c < ncols) Yo[(r0 + ti + a + 1) + (int64_t)(c0 + tc + c) * ldy] = acc[a][c];
-        return;
-    }
+    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { Bs[i][c] = v; });   // y'_b, the next B operand
     const int ri = I * kT;
     const int ni = min(kT, N1 - ri);
-    for (int t = tid; t < kT * kT; t += 256) {
-        const int i = t % kT, m = t / kT;
-        Ls[i][m] = (i < ni && m < nb) ? A[(ri + i + 1) + (int64_t)(r0 + m) * lda] : (T)0;
-    }
+    if (I != b)
+        for (int t = tid; t < kT * kT; t += 256) {
+            const int x = t % kT, y = t / kT;
+            As[y][x] = (x < ni && y < nb) ? A[(ri + x + 1) + (int64_t)(r0 + y) * lda] : (T)0;
+        }
     __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
-    for (int m = 0; m < nb; m++) {
-        T lv[4], yv[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
-#pragma unroll
-        for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
+    if (I == b) {
+        for (int t = tid; t < kT * kT; t += 256) {
+            const int x = t % kT, y = t / kT;
+            if (x < nb && c0 + y < ncols) Yo[(r0 + x + 1) + (int64_t)(c0 + y) * ldy] = Bs[x][y];
+        }
+        return;
     }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int col = c0 + tc + c;
-        if (col >= ncols) continue;
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int i = ti + a;
-            if (i < ni) {
-                T *dst = Yw + (ri + i + 1) + (int64_t)col * ldy;
-                *dst = *dst - acc[a][c];
-            }
+    mfma_tile64_zero<T>(acc);
+    mfma_tile64<T>(As, Bs, kT, acc);
+    __syncthreads();
+    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
+    __syncthreads();
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int x = t % kT, y = t / kT;
+        if (x < ni && c0 + y < ncols) {
+            T *dst = Yw + (ri + x + 1) + (int64_t)(c0 + y) * ldy;
+            *dst = *dst - As[x][y];
         }
     }
 }
@@ -174,55 +142,38 @@ __global__ void __launch_bounds__(256) k_trmm_scatter(int n, const T *__restrict
     const int ri = I * kT;
     const int ni = min(kT, N1 - ri);
     const int tid = threadIdx.x;
-    __shared__ T Ls[kT][kT + 1];
-    __shared__ T Ys[kT][kT + 1];
-    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
-    T acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
+    __shared__ T As[kT][kT + kTilePad];   // As[m][i] = L'(ri+i, rk+m)
+    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c] = y'(rk+m, c0+c)
+    typename Mfma<T>::acc_t acc[2][2];
+    mfma_tile64_zero<T>(acc);
     for (int K = 0; K <= I; K++) {
         const int rk = K * kT;
         const int nk = min(kT, N1 - rk);
         __syncthreads();
         for (int t = tid; t < kT * kT; t += 256) {
-            const int i = t % kT, m = t / kT;
-            const bool ok = i < ni && m < nk && (K < I || m < i);
-            Ls[i][m] = ok ? A[(ri + i + 1) + (int64_t)(rk + m) * lda] : (T)0;
-            const int mm = t % kT, c = t / kT;
-            Ys[mm][c] = (mm < nk && c0 + c < ncols) ? Y[(rk + mm + 1) + (int64_t)(c0 + c) * ldy] : (T)0;
+            const int xx = t % kT, yy = t / kT;
+            const bool ok = xx < ni && yy < nk && (K < I || yy < xx);
+            As[yy][xx] = ok ? A[(ri + xx + 1) + (int64_t)(rk + yy) * lda] : (T)0;
+            Bs[xx][yy] = (xx < nk && c0 + yy < ncols) ? Y[(rk + xx + 1) + (int64_t)(c0 + yy) * ldy] : (T)0;
         }
         __syncthreads();
-        for (int m = 0; m < nk; m++) {
-            T lv[4], yv[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) lv[a] = Ls[ti + a][m];
-#pragma unroll
-            for (int c = 0; c < 4; c++) yv[c] = Ys[m][tc + c];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(lv[a], yv[c], acc[a][c]);
-        }
+        mfma_tile64<T>(As, Bs, kT, acc);
     }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int q = c0 + tc + c;
-        if (q >= ncols) continue;
+    __syncthreads();
+    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
+    __syncthreads();
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int xx = t % kT, yy = t / kT;
+        const int q = c0 + yy;
+        if (xx >= ni || q >= ncols) continue;
         const int col = q / planes, pl = q % planes;
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int i = ti + a;
-            if (i >= ni) continue;
-            const int p = ri + i + 1;
-            const T yn = Y[p + (int64_t)q * ldy] + acc[a][c];
-            const int dst_row = perm[p];
-            TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-            const TX v = (TX)yn * (TX)scale[dst_row];
-            if (accumulate) *dst += v;
-            else *dst = v;
-        }
+        const int p = ri + xx + 1;
+        const T yn = Y[p + (int64_t)q * ldy] + As[xx][yy];
+        const int dst_row = perm[p];
+        TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+        const TX v = (TX)yn * (TX)scale[dst_row];
+        if (accumulate) *dst += v;
+        else *dst = v;
     }
     if (I == 0 && tid < kT && c0 + tid < ncols) {
         const int q = c0 + tid;
@@ -243,52 +194,35 @@ template <typename TW, typename TX>
 __global__ void __launch_bounds__(256) k_gemm_jx(int n, const TW *__restrict__ J, int64_t ldj,
                                                  const TX *__restrict__ X, int64_t ld, int64_t plane, int planes,
                                                  int ncols, TX *__restrict__ JX, int64_t ldq) {
-    constexpr int KC = 32;
     const int r0 = blockIdx.x * kT, c0 = blockIdx.y * kT;
     const int tid = threadIdx.x;
-    __shared__ TX Js[KC][kT + 1];   // Js[k][i] = J[r0+i, k0+k]
-    __shared__ TX Xs[KC][kT + 1];   // Xs[k][c] = X[k0+k, q=c0+c]
-    const int ti = (tid % 16) * 4, tc = (tid / 16) * 4;
-    TX acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0;
-    for (int k0 = 0; k0 < n; k0 += KC) {
+    __shared__ TX As[kT][kT + kTilePad];   // As[k][i] = J[r0+i, k0+k]
+    __shared__ TX Bs[kT][kT + kTilePad];   // Bs[k][c] = X[k0+k, q = c0+c]
+    typename Mfma<TX>::acc_t acc[2][2];
+    mfma_tile64_zero<TX>(acc);
+    for (int k0 = 0; k0 < n; k0 += kT) {
         __syncthreads();
-        for (int t = tid; t < KC * kT; t += 256) {
-            const int i = t % kT, k = t / kT;
-            Js[k][i] = (r0 + i < n && k0 + k < n) ? (TX)J[(r0 + i) + (int64_t)(k0 + k) * ldj] : (TX)0;
-            const int kk = t % KC, c = t / KC;
-            const int q = c0 + c;
+        for (int t = tid; t < kT * kT; t += 256) {
+            const int xx = t % kT, yy = t / kT;
+            As[yy][xx] = (r0 + xx < n && k0 + yy < n) ? (TX)J[(r0 + xx) + (int64_t)(k0 + yy) * ldj] : (TX)0;
+            const int q = c0 + yy;
             TX v = 0;
-            if (k0 + kk < n && q < ncols) {
+            if (k0 + xx < n && q < ncols) {
                 const int col = q / planes, pl = q % planes;
-                v = X[(k0 + kk) + (int64_t)col * ld + pl * plane];
+                v = X[(k0 + xx) + (int64_t)col * ld + pl * plane];
             }
-            Xs[kk][c] = v;
+            Bs[xx][yy] = v;
         }
         __syncthreads();
-#pragma unroll 8
-        for (int k = 0; k < KC; k++) {
-            TX jv[4], xv[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) jv[a] = Js[k][ti + a];
-#pragma unroll
-            for (int c = 0; c < 4; c++) xv[c] = Xs[k][tc + c];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[a][c] = __builtin_fma(jv[a], xv[c], acc[a][c]);
-        }
+        mfma_tile64<TX>(As, Bs, kT, acc);
     }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int q = c0 + tc + c;
-        if (q >= ncols) continue;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-            if (r0 + ti + a < n) JX[(r0 + ti + a) + (int64_t)q * ldq] = acc[a][c];
+    __syncthreads();
+    mfma_tile64_each<TX>(acc, [&](int i, int c, TX v) { As[i][c] = v; });
+    __syncthreads();
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int xx = t % kT, yy = t / kT;
+        const int q = c0 + yy;
+        if (r0 + xx < n && q < ncols) JX[(r0 + xx) + (int64_t)q * ldq] = As[xx][yy];
     }
 }
 
