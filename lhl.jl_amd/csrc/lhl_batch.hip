@@ -186,6 +186,158 @@ __global__ void __launch_bounds__(256) k_trmm_scatter(int n, const T *__restrict
     }
 }
 
+
+// ---- U: the shifted Hessenberg's triangular factor, per shift -------------
+//
+// Uinv[s][b] = (U_s[64b:64b+64, 64b:64b+64])^-1, column-major, real plane then
+// imaginary plane (complex shifts): back substitution on the unit columns with
+// the reciprocal pivots, one column per thread.
+template <typename T, int UC>
+__global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase, int64_t ustride,
+                                             const T *__restrict__ rdbase, int64_t lstride, T *__restrict__ Uinv,
+                                             int64_t uistride) {
+#pragma clang fp contract(off)
+    constexpr int UP = UC ? 2 : 1;
+    const int b = blockIdx.x, s = blockIdx.y;
+    const int r0 = b * kT, nb = min(kT, n - r0);
+    const T *U = Ubase + (int64_t)s * ustride;
+    const T *rd = rdbase + (int64_t)s * lstride;
+    const int64_t up = ustride / 2, lp = lstride / 2;
+    __shared__ T Us[UP][kT][kT + 1];
+    __shared__ T ds[UP][kT];
+    for (int t = threadIdx.x; t < kT * kT; t += 64) {
+        const int c = t % kT, r = t / kT;
+        const bool ok = r < nb && c < nb && c > r;
+        const int64_t o = upack_off(n, ok ? r0 + r : 0) - (ok ? r0 + r : 0) + r0 + c;
+        Us[0][r][c] = ok ? U[o] : (T)0;
+        if constexpr (UC) Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
+    }
+    {
+        const int t = threadIdx.x;
+        ds[0][t] = t < nb ? rd[r0 + t] : (T)1;
+        if constexpr (UC) ds[UP - 1][t] = t < nb ? rd[r0 + t + lp] : (T)0;
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    T xr[kT], xi[kT];
+#pragma unroll
+    for (int i = kT - 1; i >= 0; i--) {
+        T ar = (i == c) ? (T)1 : (T)0, ai = 0;
+#pragma unroll
+        for (int j = i + 1; j < kT; j++) {
+            ar = __builtin_fma(-Us[0][i][j], xr[j], ar);
+            if constexpr (UC) {
+                ar = __builtin_fma(Us[UP - 1][i][j], xi[j], ar);
+                ai = __builtin_fma(-Us[0][i][j], xi[j], ai);
+                ai = __builtin_fma(-Us[UP - 1][i][j], xr[j], ai);
+            }
+        }
+        if constexpr (UC) {
+            const T dr = ds[0][i], di = ds[UP - 1][i];
+            xr[i] = (i <= c) ? ar * dr - ai * di : (T)0;
+            xi[i] = (i <= c) ? ar * di + ai * dr : (T)0;
+        } else {
+            xr[i] = (i <= c) ? ar * ds[0][i] : (T)0;
+            xi[i] = 0;
+        }
+    }
+    T *dst = Uinv + (int64_t)s * uistride + (int64_t)b * kT * kT * UP + (int64_t)c * kT;
+#pragma unroll
+    for (int i = 0; i < kT; i++) {
+        dst[i] = xr[i];
+        if constexpr (UC) dst[i + kT * kT] = xi[i];
+    }
+}
+
+struct UGroup {
+    int shift, q0, nq, pad;   // internal columns q0 .. q0+nq-1 (nq <= 16), all of one shift
+};
+
+// One block step b of the back substitution on every group: the block's rows
+// become Uinv[b] * y_b (redundantly per workgroup); the block's own tile
+// stores them to Yf, the tiles above subtract U[I, b] * x_b from Yw.
+// Output tile: 64 rows x 16 internal columns; wave w owns rows 16w..16w+15.
+template <typename T, int UC>
+__global__ void __launch_bounds__(256) k_utrsm_step(int n, const T *__restrict__ Ubase, int64_t ustride,
+                                                    const T *__restrict__ Uinv, int64_t uistride,
+                                                    const UGroup *__restrict__ groups, T *__restrict__ Yw,
+                                                    T *__restrict__ Yf, int64_t ldy, int b) {
+    constexpr int UP = UC ? 2 : 1;
+    constexpr int NC = 16;
+    const int I = blockIdx.x;
+    const UGroup g = groups[blockIdx.y];
+    const int rb = b * kT, nbk = min(kT, n - rb);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    __shared__ T As[UP][kT][kT + kTilePad];   // As[.][k][i]: A operand (i, k)
+    __shared__ T Bs[kT][NC + 1];              // Bs[k][c]:   B operand (k, c)
+    const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
+    for (int t = tid; t < kT * kT; t += 256) {
+        const int x = t % kT, y = t / kT;
+        As[0][y][x] = Ui[x + (int64_t)y * kT];
+        if constexpr (UC) As[UP - 1][y][x] = Ui[kT * kT + x + (int64_t)y * kT];
+    }
+    for (int t = tid; t < kT * NC; t += 256) {
+        const int x = t % kT, y = t / kT;
+        Bs[x][y] = (x < nbk && y < g.nq) ? Yw[(rb + x) + (int64_t)(g.q0 + y) * ldy] : (T)0;
+    }
+    __syncthreads();
+    const int li = lane & 15, lk = lane >> 4;
+    auto product = [&]() {
+        typename Mfma<T>::acc_t p = Mfma<T>::zero(), q = Mfma<T>::zero();
+#pragma unroll 4
+        for (int k0 = 0; k0 < kT; k0 += 4) {
+            const int k = k0 + lk;
+            const T bv = Bs[k][li];
+            p = Mfma<T>::mac(As[0][k][16 * w + li], bv, p);
+            if constexpr (UC) q = Mfma<T>::mac(As[UP - 1][k][16 * w + li], bv, q);
+        }
+        if constexpr (UC) {
+            // internal columns alternate re, im: re = Pr - Qi, im = Pi + Qr (neighbour lane holds col ^ 1)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const T qn = __shfl_xor(q[r], 1, 64);
+                p[r] = (li & 1) ? p[r] + qn : p[r] - qn;
+            }
+        }
+        return p;
+    };
+    typename Mfma<T>::acc_t xb = product();
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) Bs[16 * w + Mfma<T>::row(lane, r)][li] = xb[r];
+    const int ri = I * kT;
+    if (I != b) {
+        for (int t = tid; t < kT * kT; t += 256) {
+            const int k = t % kT, i = t / kT;
+            const int row = ri + i;
+            const int64_t o = upack_off(n, row) - row + rb + k;
+            const bool ok = k < nbk;
+            As[0][k][i] = ok ? Ubase[(int64_t)g.shift * ustride + o] : (T)0;
+            if constexpr (UC) As[UP - 1][k][i] = ok ? Ubase[(int64_t)g.shift * ustride + ustride / 2 + o] : (T)0;
+        }
+    }
+    __syncthreads();
+    if (I == b) {
+        for (int t = tid; t < kT * NC; t += 256) {
+            const int x = t % kT, y = t / kT;
+            if (x < nbk && y < g.nq) Yf[(rb + x) + (int64_t)(g.q0 + y) * ldy] = Bs[x][y];
+        }
+        return;
+    }
+    typename Mfma<T>::acc_t up = product();
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; r++) As[0][li][16 * w + Mfma<T>::row(lane, r)] = up[r];   // As[0][c][i] staging
+    __syncthreads();
+    for (int t = tid; t < kT * NC; t += 256) {
+        const int x = t % kT, y = t / kT;
+        if (y < g.nq) {
+            T *dst = Yw + (ri + x) + (int64_t)(g.q0 + y) * ldy;
+            *dst = *dst - As[0][y][x];
+        }
+    }
+}
+
 // ---- the refinement residual: JX = J X (FP64 tiled GEMM), then
 // r = b - (sigma x + tau JX) per column with its shift, and column norms.
 // X, B, R are in the staged planar layout (ld, plane); JX uses the internal
@@ -309,9 +461,27 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
     T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
     const T *A = h->A.as<T>();
+    // the back substitution's column groups (within one shift)
+    std::vector<int64_t> cb((size_t)a.nshift + 1);
+    if (a.col_begin) {
+        LHL_HIP_TRY(hipMemcpy(cb.data(), a.col_begin, sizeof(int64_t) * (a.nshift + 1), hipMemcpyDeviceToHost));
+    } else {
+        cb[0] = 0;
+        cb[1] = a.ncol;
+    }
+    std::vector<UGroup> ug;
+    for (int64_t s = 0; s < a.nshift; s++) {
+        for (int64_t q = cb[s] * planes; q < cb[s + 1] * planes; q += 16)
+            ug.push_back(UGroup{(int)s, (int)q, (int)std::min<int64_t>(16, cb[s + 1] * planes - q), 0});
+    }
+    LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
+    UGroup *dg = h->gbuf.as<UGroup>();
+    LHL_HIP_TRY(hipMemcpy(dg, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice));
+
     const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
     LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
                (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo, ldy);
+    // y' <- L'^-1 y'  (final rows in Yo)
     const int N1 = n - 1;
     const int nbt = (N1 + kT - 1) / kT;
     const unsigned ct = (unsigned)((Q + kT - 1) / kT);
@@ -319,11 +489,41 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     for (int b = 0; b < nbt; b++)
         LHL_LAUNCH(h, "trsm", k_trsm_step<T>, dim3((unsigned)(nbt - b), ct), dim3(256), 0, n, A, h->lda, Linv, Yw, Yo,
                    ldy, Q, b);
-    LHL_HIP_TRY(hipGetLastError());
+    // forward interchange chain in place on Yo (columns resident in LDS)
     LHL_TRY(hess_device(h, a, Yo, ldy));
+    // x <- U^-1 y per shift, blocks from the bottom (final rows in Yw)
+    const int nbu = (n + kT - 1) / kT;
+    const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
+    for (int b = nbu - 1; b >= 0; b--) {
+        if (a.shift_complex)
+            LHL_LAUNCH(h, "hess", (k_utrsm_step<T, 1>), dim3((unsigned)(b + 1), (unsigned)ug.size()), dim3(256), 0, n,
+                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, Yo, Yw, ldy, b);
+        else
+            LHL_LAUNCH(h, "hess", (k_utrsm_step<T, 0>), dim3((unsigned)(b + 1), (unsigned)ug.size()), dim3(256), 0, n,
+                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, Yo, Yw, ldy, b);
+    }
+    LHL_HIP_TRY(hipGetLastError());
     LHL_LAUNCH(h, "trmm", (k_trmm_scatter<T, TX>), dim3((unsigned)std::max(nbt, 1), ct), dim3(256), 0, n, A, h->lda,
-               (const T *)Yo, ldy, Q, planes, (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(),
+               (const T *)Yw, ldy, Q, planes, (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(),
                (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+template <typename T>
+Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride, int64_t nshift,
+                  int cplx, DevBuf &Uinv, int64_t *uistride) {
+    const int n = (int)h->n;
+    const int nbu = (n + kT - 1) / kT;
+    const int64_t uis = (int64_t)nbu * kT * kT * (cplx ? 2 : 1);
+    LHL_TRY(Uinv.ensure(sizeof(T) * (size_t)uis * (size_t)nshift));
+    *uistride = uis;
+    if (cplx)
+        LHL_LAUNCH(h, "shift", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
+                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
+    else
+        LHL_LAUNCH(h, "shift", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
+                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -375,6 +575,13 @@ Status linv_typed(lhl_handle *h) {
 }
 
 }  // namespace
+
+Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride) {
+    if (h->n == 0 || nshift == 0) return {};
+    return h->dtype == LHL_F64 ? uinv_typed<double>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride)
+                               : uinv_typed<float>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride);
+}
 
 Status linv_device(lhl_handle *h) {
     if (h->n < 2) return {};

@@ -118,7 +118,7 @@ struct lhl_shift_state {
     int is_complex = 0;
     int dtype = LHL_F64;
     int planes = 1;
-    lhl::DevBuf U, l, rdiag, swp, dinfo;
+    lhl::DevBuf U, l, rdiag, swp, dinfo, Uinv;
     double sigma[2] = {0, 0}, tau[2] = {0, 0};
     int64_t info = 0;
     int loaded = 0;   // a shift has been factored for the current n
@@ -138,7 +138,7 @@ struct lhl_handle {
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L (solve)
-    lhl::DevBuf ybuf, rbuf, nbuf, xstage, bstage;  // solve / refine scratch
+    lhl::DevBuf ybuf, rbuf, nbuf, gbuf, xstage, bstage;  // solve / refine scratch
     lhl::Profiler prof;
     lhl_shift_state *own = nullptr;   // ws.shift
     std::vector<lhl_shift_state *> held; // LHLShift states created on this handle
@@ -179,7 +179,7 @@ struct ShiftPool {
     int64_t n = 0, count = 0;
     int planes = 1;
     int64_t ustride = 0;   // elements between consecutive U's (per plane stride = ustride/planes)
-    DevBuf U, l, rdiag, swp, dinfo, params;
+    DevBuf U, l, rdiag, swp, dinfo, params, Uinv;
 };
 Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const double *sig,
                          const double *tau, int is_complex);
@@ -189,6 +189,8 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
 // solution to xout instead of overwriting it.
 struct SolveArgs {
     const void *U, *l, *rdiag;
+    const void *Uinv = nullptr;   // inverses of U's 64x64 diagonal blocks (per shift, uistride apart)
+    int64_t uistride = 0;
     const uint8_t *swp;
     int64_t ustride = 0;          // per shift, elements (both planes)
     int64_t lstride = 0;          // per shift, elements of l / rdiag (both planes)
@@ -205,6 +207,10 @@ struct SolveArgs {
     int accumulate = 0;
 };
 Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
+// inverses of the 64x64 diagonal blocks of nshift packed U's (after a shift LU);
+// returns the per-shift stride of the result in *uistride
+Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride);   // lhl_batch.hip
 // the inverses of L's 64x64 diagonal blocks, formed once per reduction
 Status linv_device(lhl_handle *h);                          // lhl_batch.hip
 // the per-shift Hessenberg solves on the internal column layout Y (q = col*planes + pl)

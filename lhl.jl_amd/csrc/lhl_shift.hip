@@ -349,12 +349,16 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
     LHL_TRY(pool.swp.ensure((size_t)lplane * (size_t)nshift));
     LHL_TRY(pool.dinfo.ensure(sizeof(int) * (size_t)nshift));
     if (h->dtype == LHL_F64)
-        return shift_into<double>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<double>(), pool.ustride,
-                                  pool.l.as<double>(), pool.rdiag.as<double>(), lplane * pool.planes,
-                                  pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>());
-    return shift_into<float>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<float>(), pool.ustride,
-                             pool.l.as<float>(), pool.rdiag.as<float>(), lplane * pool.planes,
-                             pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>());
+        LHL_TRY(shift_into<double>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<double>(), pool.ustride,
+                                   pool.l.as<double>(), pool.rdiag.as<double>(), lplane * pool.planes,
+                                   pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>()));
+    else
+        LHL_TRY(shift_into<float>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<float>(), pool.ustride,
+                                  pool.l.as<float>(), pool.rdiag.as<float>(), lplane * pool.planes,
+                                  pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>()));
+    int64_t uis = 0;
+    return uinv_device(h, pool.U.p, pool.ustride, pool.rdiag.p, lplane * pool.planes, nshift, is_complex, pool.Uinv,
+                       &uis);
 }
 
 // One shift state (ws.shift or an LHLShift): storage is the pool layout with
@@ -387,6 +391,11 @@ Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t /*nshift*/, cons
                                    sh->swp.as<uint8_t>(), lplane, sh->dinfo.as<int>());
     }
     if (!st.ok()) { params.release(); return st; }
+    if (n > 0) {
+        int64_t uis = 0;
+        st = uinv_device(h, sh->U.p, uplane * planes, sh->rdiag.p, lplane * planes, 1, sh->is_complex, sh->Uinv, &uis);
+        if (!st.ok()) { params.release(); return st; }
+    }
     int info = 0;
     if (n > 0) {
         hipError_t e = hipMemcpyAsync(&info, sh->dinfo.p, sizeof(int), hipMemcpyDeviceToHost, h->stream);

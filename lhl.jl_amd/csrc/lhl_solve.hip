@@ -4,7 +4,9 @@
 // vectors (src:2069-2126), and the planar layout conversions.
 //
 // k_hess: one workgroup owns NR right-hand sides (PL planes each: 2 for
-// complex values) of one shift with the columns resident in LDS:
+// complex values) of one shift with the columns resident in LDS (the batched
+// solve runs it for the forward chain only; the back substitution of a batch
+// runs as blocked MFMA steps over all shifts at once):
 //   forward      the interchange/elimination chain, one lane per column, with
 //                the flags and multipliers staged through LDS;
 //   back subst.  right-looking over 32-row blocks from the bottom: the
@@ -35,7 +37,7 @@ struct Group {
     int pad;
 };
 
-template <typename T, int NR, int PL, int UC>
+template <typename T, int NR, int PL, int UC, bool BACK>
 __global__ void __launch_bounds__(kSolveThreads) k_hess(
     int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restrict__ lbase, const T *__restrict__ rdbase,
     int64_t lstride, const uint8_t *__restrict__ swbase, int64_t sstride, const Group *__restrict__ groups,
@@ -110,6 +112,13 @@ __global__ void __launch_bounds__(kSolveThreads) k_hess(
         for (int pl = 0; pl < PL; pl++) y(r * PL + pl, n - 1) = ca[pl];
     __syncthreads();
 
+    if constexpr (!BACK) {
+        for (int i = tid; i < n; i += kSolveThreads)
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                if (v / PL < nc) gcol(v)[i] = y(v, i);
+        return;
+    }
     // ---- back substitution, right-looking over kHB-row blocks from the bottom:
     // solve the block's triangle from LDS, then every row above subtracts its
     // contiguous U segment times the block's finished values.
@@ -178,36 +187,48 @@ __global__ void __launch_bounds__(kSolveThreads) k_hess(
         }
         __syncthreads();
         if (j0 > 0) {
-            // here nb == kHB: only the top block is partial, and it has no rows above
-            for (int i = tid; i < j0; i += kSolveThreads) {
-                const T *Ur = U + upack_off(n, i) - i + j0;   // Ur[c] = U[i, j0+c]
-                T acc[NV];
+            // here nb == kHB: only the top block is partial, and it has no rows above.
+            // RU rows per thread at a time, all their U segments loaded before use.
+            constexpr int RU = (UC ? 1 : 2) * (sizeof(T) == 4 ? 2 : 1);
+            for (int i0 = tid; i0 < j0; i0 += RU * kSolveThreads) {
+                T uu[RU][kHB], ui[RU][kHB];
 #pragma unroll
-                for (int q = 0; q < NV; q++) acc[q] = 0;
-                T uu[kHB], ui[kHB];
+                for (int ru = 0; ru < RU; ru++) {
+                    const int i = i0 + ru * kSolveThreads;
+                    const int ii = i < j0 ? i : 0;
+                    const T *Ur = U + upack_off(n, ii) - ii + j0;   // Ur[c] = U[ii, j0+c]
 #pragma unroll
-                for (int c = 0; c < kHB; c++) {
-                    uu[c] = Ur[c];
-                    if constexpr (UC) ui[c] = Ur[c + up];
-                }
-#pragma unroll
-                for (int c = 0; c < kHB; c++) {
-                    if constexpr (UC) {
-#pragma unroll
-                        for (int rr = 0; rr < NR; rr++) {
-                            const T xr = y(rr * PL, j0 + c), xi = y(rr * PL + 1, j0 + c);
-                            acc[rr * PL] = __builtin_fma(uu[c], xr, acc[rr * PL]);
-                            acc[rr * PL] = __builtin_fma(-ui[c], xi, acc[rr * PL]);
-                            acc[rr * PL + 1] = __builtin_fma(uu[c], xi, acc[rr * PL + 1]);
-                            acc[rr * PL + 1] = __builtin_fma(ui[c], xr, acc[rr * PL + 1]);
-                        }
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(uu[c], y(q, j0 + c), acc[q]);
+                    for (int c = 0; c < kHB; c++) {
+                        uu[ru][c] = Ur[c];
+                        if constexpr (UC) ui[ru][c] = Ur[c + up];
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < NV; q++) y(q, i) -= acc[q];
+                for (int ru = 0; ru < RU; ru++) {
+                    const int i = i0 + ru * kSolveThreads;
+                    T acc[NV];
+#pragma unroll
+                    for (int q = 0; q < NV; q++) acc[q] = 0;
+#pragma unroll
+                    for (int c = 0; c < kHB; c++) {
+                        if constexpr (UC) {
+#pragma unroll
+                            for (int rr = 0; rr < NR; rr++) {
+                                const T xr = y(rr * PL, j0 + c), xi = y(rr * PL + 1, j0 + c);
+                                acc[rr * PL] = __builtin_fma(uu[ru][c], xr, acc[rr * PL]);
+                                acc[rr * PL] = __builtin_fma(-ui[ru][c], xi, acc[rr * PL]);
+                                acc[rr * PL + 1] = __builtin_fma(uu[ru][c], xi, acc[rr * PL + 1]);
+                                acc[rr * PL + 1] = __builtin_fma(ui[ru][c], xr, acc[rr * PL + 1]);
+                            }
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(uu[ru][c], y(q, j0 + c), acc[q]);
+                        }
+                    }
+                    if (i < j0)
+#pragma unroll
+                        for (int q = 0; q < NV; q++) y(q, i) -= acc[q];
+                }
             }
             __syncthreads();
         }
@@ -353,7 +374,7 @@ template <typename T, int NR, int PL, int UC>
 Status launch_hess_t(lhl_handle *h, const SolveArgs &a, const Group *groups, int ngroups, void *Y, int64_t ldy) {
     const int n = (int)h->n;
     const size_t lds = sizeof(T) * (size_t)NR * PL * n;
-    auto fn = k_hess<T, NR, PL, UC>;
+    auto fn = k_hess<T, NR, PL, UC, false>;   // the forward chain only; U is solved by the blocked kernels
     LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     LHL_LAUNCH(h, "hess", fn, dim3(ngroups), dim3(kSolveThreads), lds, n, (const T *)a.U, a.ustride, (const T *)a.l,
                (const T *)a.rdiag, a.lstride, a.swp, a.sstride, groups, (T *)Y, ldy);
@@ -384,15 +405,18 @@ Status hess_typed(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
     const int ng = (int)hg.size();
     Status st;
     if (a.shift_complex) {
-        if (nr >= 4) st = launch_hess_t<T, 4, 2, 1>(h, a, dg, ng, Y, ldy);
+        if (nr >= 8) st = launch_hess_t<T, 8, 2, 1>(h, a, dg, ng, Y, ldy);
+        else if (nr >= 4) st = launch_hess_t<T, 4, 2, 1>(h, a, dg, ng, Y, ldy);
         else if (nr >= 2) st = launch_hess_t<T, 2, 2, 1>(h, a, dg, ng, Y, ldy);
         else st = launch_hess_t<T, 1, 2, 1>(h, a, dg, ng, Y, ldy);
     } else if (a.x_complex) {
-        if (nr >= 4) st = launch_hess_t<T, 4, 2, 0>(h, a, dg, ng, Y, ldy);
+        if (nr >= 8) st = launch_hess_t<T, 8, 2, 0>(h, a, dg, ng, Y, ldy);
+        else if (nr >= 4) st = launch_hess_t<T, 4, 2, 0>(h, a, dg, ng, Y, ldy);
         else if (nr >= 2) st = launch_hess_t<T, 2, 2, 0>(h, a, dg, ng, Y, ldy);
         else st = launch_hess_t<T, 1, 2, 0>(h, a, dg, ng, Y, ldy);
     } else {
-        if (nr >= 4) st = launch_hess_t<T, 4, 1, 0>(h, a, dg, ng, Y, ldy);
+        if (nr >= 8) st = launch_hess_t<T, 8, 1, 0>(h, a, dg, ng, Y, ldy);
+        else if (nr >= 4) st = launch_hess_t<T, 4, 1, 0>(h, a, dg, ng, Y, ldy);
         else if (nr >= 2) st = launch_hess_t<T, 2, 1, 0>(h, a, dg, ng, Y, ldy);
         else st = launch_hess_t<T, 1, 1, 0>(h, a, dg, ng, Y, ldy);
     }
