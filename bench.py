@@ -58,9 +58,10 @@ def algorithmic(kernel, n, S, ncol, es):
         # Ht's upper Hessenberg part read once, S packed U's written
         return (tri + n) * es + S * tri * es, "bytes", "hbm"
     if kernel == "hess":
-        # every shift's packed U read once, the batch's columns read and written
-        return S * tri * es + 2 * ncol * n * es, "bytes", "hbm"
-    if kernel in ("trsm", "trmm"):
+        # the forward interchange chain: every column read and written once,
+        # each shift's multipliers and swap flags read once
+        return 2 * ncol * n * es + S * 2 * n * es, "bytes", "hbm"
+    if kernel in ("trsm", "utrsm", "trmm"):
         # the multipliers' triangle and the columns once each: n^2 ncol FMAs
         return 2.0 * (n * n / 2) * ncol, "flops", "valu"
     if kernel == "residual":
@@ -157,11 +158,11 @@ def main():
     lhl_hip.profile_reset(ws32)
     lhl_hip.profile_enable(ws32, True)
     step(ws32)
-    fams = ["shift", "gather", "trsm", "hess", "trmm", "residual", "convert"]
+    fams = ["shift", "gather", "trsm", "hess", "utrsm", "trmm", "residual", "convert"]
     prof = {k: lhl_hip.profile_read(ws32, k) for k in fams}
     lhl_hip.profile_enable(ws32, False)
     log("bench: kernels " + ", ".join(f"{k} {c}x {t:.2f} ms" for k, (c, t) in prof.items()))
-    dom = max(("shift", "trsm", "hess", "trmm", "residual"), key=lambda k: prof[k][1])
+    dom = max(("shift", "trsm", "hess", "utrsm", "trmm", "residual"), key=lambda k: prof[k][1])
     c, t = prof[dom]
     amt, unit, bound = algorithmic(dom, n, S, ncol, 4)
     avg_s = (t / c) / 1e3 if c else float("nan")

@@ -338,6 +338,232 @@ __global__ void __launch_bounds__(256) k_utrsm_step(int n, const T *__restrict__
     }
 }
 
+
+// ---- column-owner triangular solves ---------------------------------------
+// One workgroup owns kCN internal columns and walks every 64-row block in
+// order, left-looking: Y_b <- inv(T_bb) (Y_b - sum_K T[b,K] X_K).  Columns are
+// independent, so no workgroup waits on another and one launch covers the
+// whole solve.  kCW waves split the K sum (partial tiles reduced through LDS);
+// four of them then apply the precomputed 64x64 diagonal-block inverse.
+// Within each 16-wide K slab lane (i, q) takes k = 16j + 4q + t, t = 0..3, so
+// both operands are read as runs of four consecutive elements.
+constexpr int kCW = 8;
+constexpr int kCN = 16;
+
+// Four consecutive elements through one (float) or two (double) vector loads
+// that need only element alignment.
+template <typename T> struct V4T;
+template <> struct V4T<float> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
+template <> struct V4T<double> { typedef double type __attribute__((ext_vector_type(4), aligned(8))); };
+template <typename T> using V4 = typename V4T<T>::type;
+template <typename T> __device__ __forceinline__ V4<T> ld4u(const T *p) { return *(const V4<T> *)p; }
+
+// y' <- L'^-1 y': Yw holds the gathered columns (rows 1..n-1), Yo receives the
+// solution (row 0 is already there).
+template <typename T>
+__global__ void __launch_bounds__(64 * kCW) k_ltrsm_cols(int n, const T *__restrict__ A, int64_t lda,
+                                                         const T *__restrict__ Linv, const T *__restrict__ Yw,
+                                                         T *__restrict__ Yo, int64_t ldy, int Q) {
+    const int N1 = n - 1, nbt = (N1 + kT - 1) / kT;
+    const int c0 = blockIdx.x * kCN;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const T *Xc = Yo + (int64_t)min(c0 + li, Q - 1) * ldy + 1;   // solved rows m at Xc[m]
+    __shared__ T red[kCW][kT][kCN + 1];
+    __shared__ T Ys[kT][kCN + 1];
+    for (int b = 0; b < nbt; b++) {
+        const int r0 = b * kT, nb = min(kT, N1 - r0);
+        // MFMA row i of accumulator rr is block row 4i + rr, so each lane reads
+        // four consecutive rows of a column of L' as one run.
+        typename Mfma<T>::acc_t acc[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::zero();
+        const T *Ar = A + r0 + 4 * li + 1;   // L'(i, m) = A(i + 1, m)
+        const bool full = nb == kT;
+        for (int j = w; j < 4 * b; j += kCW) {
+            const int m0 = 16 * j + 4 * lk;
+            T av[4][4];
+            const V4<T> xv = ld4u(Xc + m0);
+            if (full) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const V4<T> a = ld4u(Ar + (int64_t)(m0 + t) * lda);
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) av[rr][t] = a[rr];
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++)
+                        av[rr][t] = A[min(r0 + 4 * li + rr, N1 - 1) + 1 + (int64_t)(m0 + t) * lda];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::mac(av[rr][t], xv[t], acc[rr]);
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) red[w][4 * Mfma<T>::row(lane, r) + rr][li] = acc[rr][r];
+        __syncthreads();
+        for (int t = threadIdx.x; t < kT * kCN; t += 64 * kCW) {
+            const int i = t % kT, c = t / kT;
+            T v = 0;
+            if (i < nb && c0 + c < Q) {
+                T sum = 0;
+#pragma unroll
+                for (int ww = 0; ww < kCW; ww++) sum += red[ww][i][c];
+                v = Yw[(r0 + i + 1) + (int64_t)(c0 + c) * ldy] - sum;
+            }
+            Ys[i][c] = v;
+        }
+        __syncthreads();
+        if (w < 4) {
+            const T *Lb = Linv + (int64_t)b * kT * kT + 16 * w + li;
+            typename Mfma<T>::acc_t x = Mfma<T>::zero();
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int k = 16 * sl + 4 * lk + t;
+                    x = Mfma<T>::mac(Lb[(int64_t)k * kT], Ys[k][li], x);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * w + Mfma<T>::row(lane, r);
+                if (i < nb && c0 + li < Q) Yo[(r0 + i + 1) + (int64_t)(c0 + li) * ldy] = x[r];
+            }
+        }
+        __syncthreads();   // the block's solved rows are visible to every wave
+    }
+}
+
+// x <- U_s^-1 y for one group of internal columns of shift s: Yin holds the
+// forward-chain output, Yf receives the solution.  Complex shifts: the
+// columns alternate (re, im) and U is planar (imaginary plane ustride/2 on).
+template <typename T, int UC>
+__global__ void __launch_bounds__(64 * kCW) k_utrsm_cols(int n, const T *__restrict__ Ubase, int64_t ustride,
+                                                         const T *__restrict__ Uinv, int64_t uistride,
+                                                         const UGroup *__restrict__ groups, const T *__restrict__ Yin,
+                                                         T *__restrict__ Yf, int64_t ldy) {
+    constexpr int UP = UC ? 2 : 1;
+    const UGroup g = groups[blockIdx.x];
+    const int nbu = (n + kT - 1) / kT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const T *Xc = Yf + (int64_t)(g.q0 + min(li, g.nq - 1)) * ldy;
+    const T *Us = Ubase + (int64_t)g.shift * ustride;
+    const int64_t up = ustride / 2;
+    __shared__ T red[kCW][kT][kCN + 1];
+    __shared__ T Ys[kT][kCN + 1];
+    for (int b = nbu - 1; b >= 0; b--) {
+        const int rb = b * kT, nbk = min(kT, n - rb);
+        typename Mfma<T>::acc_t acc[4], qc[4];
+        const T *Ur[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            acc[rr] = Mfma<T>::zero();
+            qc[rr] = Mfma<T>::zero();
+            const int row = min(rb + 16 * rr + li, n - 1);
+            Ur[rr] = Us + upack_off(n, row) - row;   // U(row, m) = Ur[m], m >= row
+        }
+        const int kbeg = rb + kT;
+        const int nsl = kbeg < n ? (n - kbeg + 15) / 16 : 0;
+        for (int j = w; j < nsl; j += kCW) {
+            const int m0 = kbeg + 16 * j + 4 * lk;
+            T bv[4], av[4][4], ai[4][4];
+            if (m0 + 3 < n) {
+                // one 4-element run per lane and row (dword aligned vector loads)
+                const V4<T> xv = ld4u(Xc + m0);
+#pragma unroll
+                for (int t = 0; t < 4; t++) bv[t] = xv[t];
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const V4<T> u = ld4u(Ur[rr] + m0);
+#pragma unroll
+                    for (int t = 0; t < 4; t++) av[rr][t] = u[t];
+                    if constexpr (UC) {
+                        const V4<T> v = ld4u(Ur[rr] + m0 + up);
+#pragma unroll
+                        for (int t = 0; t < 4; t++) ai[rr][t] = v[t];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int m = m0 + t;
+                    const bool ok = m < n;
+                    const int mc = ok ? m : n - 1;
+                    bv[t] = ok ? Xc[mc] : (T)0;
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        av[rr][t] = Ur[rr][mc];
+                        if constexpr (UC) ai[rr][t] = Ur[rr][mc + up];
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    acc[rr] = Mfma<T>::mac(av[rr][t], bv[t], acc[rr]);
+                    if constexpr (UC) qc[rr] = Mfma<T>::mac(ai[rr][t], bv[t], qc[rr]);
+                }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                T v = acc[rr][r];
+                if constexpr (UC) {
+                    // re = Ur*xr - Ui*xi, im = Ur*xi + Ui*xr (the neighbour lane holds col ^ 1)
+                    const T qn = __shfl_xor(qc[rr][r], 1, 64);
+                    v = (li & 1) ? v + qn : v - qn;
+                }
+                red[w][16 * rr + Mfma<T>::row(lane, r)][li] = v;
+            }
+        __syncthreads();
+        for (int t = threadIdx.x; t < kT * kCN; t += 64 * kCW) {
+            const int i = t % kT, c = t / kT;
+            T v = 0;
+            if (i < nbk && c < g.nq) {
+                T sum = 0;
+#pragma unroll
+                for (int ww = 0; ww < kCW; ww++) sum += red[ww][i][c];
+                v = Yin[(rb + i) + (int64_t)(g.q0 + c) * ldy] - sum;
+            }
+            Ys[i][c] = v;
+        }
+        __syncthreads();
+        if (w < 4) {
+            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP + 16 * w + li;
+            typename Mfma<T>::acc_t x = Mfma<T>::zero(), xq = Mfma<T>::zero();
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int k = 16 * sl + 4 * lk + t;
+                    const T yv = Ys[k][li];
+                    x = Mfma<T>::mac(Ui[(int64_t)k * kT], yv, x);
+                    if constexpr (UC) xq = Mfma<T>::mac(Ui[kT * kT + (int64_t)k * kT], yv, xq);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                T v = x[r];
+                if constexpr (UC) {
+                    const T qn = __shfl_xor(xq[r], 1, 64);
+                    v = (li & 1) ? v + qn : v - qn;
+                }
+                const int i = 16 * w + Mfma<T>::row(lane, r);
+                if (i < nbk && li < g.nq) Yf[(rb + i) + (int64_t)(g.q0 + li) * ldy] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- the refinement residual: JX = J X (FP64 tiled GEMM), then
 // r = b - (sigma x + tau JX) per column with its shift, and column norms.
 // X, B, R are in the staged planar layout (ld, plane); JX uses the internal
@@ -486,22 +712,21 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int nbt = (N1 + kT - 1) / kT;
     const unsigned ct = (unsigned)((Q + kT - 1) / kT);
     const T *Linv = h->Linv.as<T>();
-    for (int b = 0; b < nbt; b++)
-        LHL_LAUNCH(h, "trsm", k_trsm_step<T>, dim3((unsigned)(nbt - b), ct), dim3(256), 0, n, A, h->lda, Linv, Yw, Yo,
-                   ldy, Q, b);
+    if (nbt > 0)
+        LHL_LAUNCH(h, "trsm", k_ltrsm_cols<T>, dim3((unsigned)((Q + kCN - 1) / kCN)), dim3(64 * kCW), 0, n, A, h->lda,
+                   Linv, (const T *)Yw, Yo, ldy, Q);
     // forward interchange chain in place on Yo (columns resident in LDS)
     LHL_TRY(hess_device(h, a, Yo, ldy));
-    // x <- U^-1 y per shift, blocks from the bottom (final rows in Yw)
+    // x <- U^-1 y per shift (solution in Yw)
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
-    for (int b = nbu - 1; b >= 0; b--) {
-        if (a.shift_complex)
-            LHL_LAUNCH(h, "hess", (k_utrsm_step<T, 1>), dim3((unsigned)(b + 1), (unsigned)ug.size()), dim3(256), 0, n,
-                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, Yo, Yw, ldy, b);
-        else
-            LHL_LAUNCH(h, "hess", (k_utrsm_step<T, 0>), dim3((unsigned)(b + 1), (unsigned)ug.size()), dim3(256), 0, n,
-                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, Yo, Yw, ldy, b);
-    }
+    const dim3 gu((unsigned)ug.size());
+    if (a.shift_complex)
+        LHL_LAUNCH(h, "utrsm", (k_utrsm_cols<T, 1>), gu, dim3(64 * kCW), 0, n, (const T *)a.U, a.ustride,
+                   (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+    else
+        LHL_LAUNCH(h, "utrsm", (k_utrsm_cols<T, 0>), gu, dim3(64 * kCW), 0, n, (const T *)a.U, a.ustride,
+                   (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
     LHL_HIP_TRY(hipGetLastError());
     LHL_LAUNCH(h, "trmm", (k_trmm_scatter<T, TX>), dim3((unsigned)std::max(nbt, 1), ct), dim3(256), 0, n, A, h->lda,
                (const T *)Yw, ldy, Q, planes, (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(),

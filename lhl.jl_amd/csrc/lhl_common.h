@@ -61,9 +61,9 @@ inline int64_t pad_ld(int64_t n, int dtype) {
 // Offset of row k of the packed upper-triangular U (row k holds U[k, k..n-1]).
 __host__ __device__ inline int64_t upack_off(int64_t n, int64_t k) { return k * n - (k * (k - 1)) / 2; }
 inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
-// Elements per plane of one shift's U: the packed triangle plus 256 spare
-// elements at the end (the shift kernel's dump slots), rounded to 64.
-inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 256 + 63) / 64) * 64; }
+// Elements per plane of one shift's U: the packed triangle plus 1024 spare
+// elements at the end (the shift kernel's dump slots, one per thread), rounded to 64.
+inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 63) / 64) * 64; }
 
 // Growable device allocation.
 struct DevBuf {

@@ -2,7 +2,7 @@
 // src:2456-2594; bit-for-bit spec test/core_tests.jl:119-171).
 //
 // One workgroup per shift.  The pending row r (the row of G not yet chosen as
-// a pivot row) lives in registers: thread t owns elements j = t + m*256.  Step
+// a pivot row) lives in registers: thread t owns elements j = t + m*1024.  Step
 // k compares r[k] with tau*H[k+1,k], writes U row k (coalesced, packed rows
 // of U) and updates the owned r[j], j > k, against row k+1 of H (a
 // contiguous column of Ht).  The owner of r[k+1] publishes it through a
@@ -17,7 +17,7 @@
 namespace lhl {
 namespace {
 
-constexpr int kShiftThreads = 256;
+constexpr int kShiftThreads = 1024;   // 16 waves: few pending-row elements per thread per step
 
 template <typename T> __device__ __forceinline__ T dabs(T x) { return x < 0 ? -x : x; }
 
@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         const T ls = l * sigma;
 #pragma unroll
         for (int m = 0; m < NPER; m++) {
+            if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
             const int j = tid + m * kShiftThreads;
             const bool active = j > k && j < n;
             const bool first = j == k + 1;
@@ -239,6 +240,7 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
         const T qsr = lre * sr - lim * si, qsi = lre * si + lim * sr;
 #pragma unroll
         for (int m = 0; m < NPER; m++) {
+            if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
             const int j = tid + m * kShiftThreads;
             const bool active = j > k && j < n;
             const bool first = j == k + 1;
@@ -306,8 +308,7 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     else if (nper <= 4) launch_shift<T, 4>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
     else if (nper <= 8) launch_shift<T, 8>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
     else if (nper <= 16) launch_shift<T, 16>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 32) launch_shift<T, 32>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else return {LHL_ERR_ARG, "lhl_shift: n above 8192 is not supported by the register-resident shift kernel"};
+    else return {LHL_ERR_ARG, "lhl_shift: n above 16384 is not supported by the register-resident shift kernel"};
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
