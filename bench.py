@@ -37,6 +37,7 @@ for _p in (ROOT, os.path.join(ROOT, "lhl.jl_amd")):
 METRIC = "solves/s @n=4096 (FP32+refine vs FP64, rel resid≤1e-12) + lhl() time"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def stiff(seed, n):
@@ -50,26 +51,64 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+KERNELS = ["shift", "uinv", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "gemm_jx", "resid_combine", "colnorms",
+           "convert"]
+PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
+        "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
+
+
 def algorithmic(kernel, n, S, ncol, es):
-    """Algorithmic bytes (or flops) one launch of a kernel family moves for the
-    whole batch: (amount, unit, bound)."""
+    """Algorithmic work of ONE launch of a kernel over the whole batch (S
+    shifts, ncol columns, factor element size es): (bytes, flops, bound).
+    bytes counts each operand once (what an ideal kernel moves to or from
+    HBM); flops counts the multiply-adds as 2."""
     tri = n * (n + 1) / 2
+    col = n * ncol * es                      # one pass over the batch's columns
+    nb = (n + 63) // 64
     if kernel == "shift":
-        # Ht's upper Hessenberg part read once, S packed U's written
-        return (tri + n) * es + S * tri * es, "bytes", "hbm"
-    if kernel == "hess":
-        # the forward interchange chain: every column read and written once,
-        # each shift's multipliers and swap flags read once
-        return 2 * ncol * n * es + S * 2 * n * es, "bytes", "hbm"
-    if kernel in ("trsm", "utrsm", "trmm"):
-        # the multipliers' triangle and the columns once each: n^2 ncol FMAs
-        return 2.0 * (n * n / 2) * ncol, "flops", "valu"
-    if kernel == "residual":
-        # J x for every column in FP64: 2 n^2 flops per column
-        return 2.0 * n * n * ncol, "flops", "mfma"
-    if kernel == "reduce_trailing":
-        return None
-    return None
+        # H's upper Hessenberg part read, each shift's packed U, multipliers,
+        # reciprocal pivots and swap flags written; n^2/2 multiply-adds per shift
+        return (tri + n) * es + S * (tri + 3 * n) * es, S * n * n, "hbm"
+    if kernel == "uinv":
+        return S * nb * 64 * 64 * es * 2, S * nb * 2 * 64 ** 3 / 3, "hbm"
+    if kernel == "gather":
+        return col + ncol * n * 8 + n * 12, 0, "hbm"
+    if kernel == "trsm":
+        # L' (strict lower triangle) and its diagonal-block inverses read once,
+        # the columns read and written: n^2 flops per column
+        return n * n / 2 * es + nb * 64 * 64 * es + 2 * col, n * n * ncol, "mfma32" if es == 4 else "mfma64"
+    if kernel == "fwd_chain":
+        return 2 * col + S * 2 * n * es, 4 * n * ncol, "hbm"
+    if kernel == "utrsm":
+        # every shift's packed U and diagonal-block inverses read once
+        return S * tri * es + S * nb * 64 * 64 * es + 2 * col, n * n * ncol, "hbm"
+    if kernel == "trmm":
+        # L' read once, the columns read, the scattered FP64 solution read-modify-written
+        return n * n / 2 * es + col + 2 * n * ncol * 8, n * n * ncol, "mfma32" if es == 4 else "mfma64"
+    if kernel == "gemm_jx":
+        # J x in FP64 for every column: J (n^2) read once, x read, J x written
+        return n * n * 8 + 2 * n * ncol * 8, 2.0 * n * n * ncol, "mfma64"
+    if kernel == "resid_combine":
+        return 4 * n * ncol * 8, 3 * n * ncol, "hbm"
+    if kernel == "colnorms":
+        return 2 * n * ncol * 8, 2 * n * ncol, "hbm"
+    if kernel == "convert":
+        return 2 * col, 0, "hbm"
+    return 0, 0, "hbm"
+
+
+def roofline_of(kernel, launches, total_ms, n, S, ncol, es):
+    """Roofline figures of one kernel from its live per-launch average."""
+    byts, flops, bound = algorithmic(kernel, n, S, ncol, es)
+    avg_s = total_ms / launches / 1e3
+    peak, unit = PEAK[bound]
+    if bound == "hbm":
+        achieved = byts / avg_s / 1e9
+    else:
+        achieved = flops / avg_s / 1e12
+    return {"bound": "hbm" if bound == "hbm" else "mfma", "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None, "kernel": kernel, "launches": launches,
+            "avg_ms": total_ms / launches, "algorithmic_bytes": byts, "algorithmic_flops": flops}
 
 
 def main():
@@ -135,15 +174,22 @@ def main():
         return info, iters
 
     results = {}
+    prof = {}
     for name, ws in (("fp64", ws64), ("fp32", ws32)):
         for _ in range(args.warmup):
             step(ws)
         torch.cuda.synchronize()
+        # every launch inside the timed region is bracketed by HIP events on
+        # the library's stream (the stream the kernels run on)
+        lhl_hip.profile_reset(ws)
+        lhl_hip.profile_enable(ws, True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             info, iters = step(ws)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / max(args.steps, 1)
+        lhl_hip.profile_enable(ws, False)
+        prof[name] = {k: lhl_hip.profile_read(ws, k) for k in KERNELS}
         # verify the accuracy bar on the device in FP64: r = b - (x - gamma J x)
         JX = Jd @ Xd
         g = torch.from_numpy(np.repeat(gam, R)).to(dev)
@@ -154,34 +200,20 @@ def main():
         log(f"bench: {name} step {1e3 * dt:.2f} ms, {ncol / dt:.0f} solves/s, max rel resid {rel:.2e}, "
             f"refine iters {int(iters.min())}-{int(iters.max())}")
 
-    # ---- live per-kernel timing of one FP32+refine step (outside the timed region)
-    lhl_hip.profile_reset(ws32)
-    lhl_hip.profile_enable(ws32, True)
-    step(ws32)
-    fams = ["shift", "gather", "trsm", "hess", "utrsm", "trmm", "residual", "convert"]
-    prof = {k: lhl_hip.profile_read(ws32, k) for k in fams}
-    lhl_hip.profile_enable(ws32, False)
-    log("bench: kernels " + ", ".join(f"{k} {c}x {t:.2f} ms" for k, (c, t) in prof.items()))
-    dom = max(("shift", "trsm", "hess", "utrsm", "trmm", "residual"), key=lambda k: prof[k][1])
-    c, t = prof[dom]
-    amt, unit, bound = algorithmic(dom, n, S, ncol, 4)
-    avg_s = (t / c) / 1e3 if c else float("nan")
-    if bound == "valu":
-        achieved = amt / (t / 1e3) / 1e12      # all launches of the family in one pass
-        roof = {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
-                "frac": achieved / 157.3, "traffic": None, "kernel": dom, "launches": c, "avg_ms": t / c,
-                "algorithmic_flops": amt}
-    elif bound == "hbm":
-        # one solve launch per refinement round covers the whole batch
-        achieved = amt / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "launches": c, "avg_ms": t / c, "algorithmic_bytes": amt}
-    else:
-        achieved = amt / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
-                "launches": c, "avg_ms": t / c, "algorithmic_flops": amt}
+    # ---- roofline of every kernel of the FP32+refine step; the dominant one leads
+    steps = max(args.steps, 1)
+    kernels = {}
+    for prec, es in (("fp32", 4), ("fp64", 8)):
+        kernels[prec] = {}
+        for k, (c, t) in prof[prec].items():
+            if c:
+                r = roofline_of(k, c, t, n, S, ncol, es)
+                r["ms_per_step"] = t / steps
+                kernels[prec][k] = r
+    log("bench: fp32 kernels " + ", ".join(f"{k} {v['launches'] // steps}x {v['ms_per_step']:.2f} ms"
+                                          for k, v in kernels["fp32"].items()))
+    dom = max(kernels["fp32"], key=lambda k: kernels["fp32"][k]["ms_per_step"])
+    roof = dict(kernels["fp32"][dom])
 
     # ---- CPU baseline: the C restatement of the reference algorithm (oracle/)
     cpu = None
@@ -247,8 +279,10 @@ def main():
         "max_rel_resid": {"fp32_refine": fp32["rel"], "fp64": fp64["rel"]},
         "refine_rounds": {"fp32": [min(fp32["iters"]), max(fp32["iters"])],
                           "fp64": [min(fp64["iters"]), max(fp64["iters"])]},
-        "kernel_ms_per_step_fp32": {k: v[1] for k, v in prof.items()},
+        "kernel_ms_per_step": {p: {k: v["ms_per_step"] for k, v in kernels[p].items()} for p in kernels},
         "roofline": roof,
+        "rooflines_fp32": {k: {f: v[f] for f in ("bound", "achieved", "unit", "frac", "avg_ms")}
+                           for k, v in kernels["fp32"].items()},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -156,9 +156,11 @@ int lhl_sync(lhl_handle *h);
 int lhl_get_stream(lhl_handle *h, void **stream);
 /* While enabled, every kernel launch is bracketed by HIP events on the
  * handle's stream.  lhl_profile_read synchronizes, then returns the launches
- * and the summed duration (ms) of one kernel family since the last reset:
- * "balance", "reduce_copy", "reduce_pivot", "reduce_swap", "reduce_trailing",
- * "reduce_finish", "shift", "solve", "applyz", "residual", "convert". */
+ * and the summed duration (ms) of one kernel since the last reset:
+ * reduction "balance", "reduce_copy", "reduce_pivot", "reduce_swap",
+ * "reduce_trailing", "reduce_finish"; shift "shift", "uinv"; solve "gather",
+ * "trsm", "fwd_chain", "utrsm", "trmm", "applyz", "convert"; refinement
+ * "gemm_jx", "resid_combine", "colnorms". */
 int lhl_profile_enable(lhl_handle *h, int on);
 int lhl_profile_reset(lhl_handle *h);
 int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms);

@@ -165,8 +165,9 @@ int lhl_destroy(lhl_handle *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     prof_drain(h);
+    blas_release(h);
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
-                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->Linv, &h->ybuf, &h->rbuf, &h->nbuf, &h->gbuf,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->Linv, &h->Lfull, &h->ybuf, &h->rbuf, &h->nbuf, &h->gbuf,
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {

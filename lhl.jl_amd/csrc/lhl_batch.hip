@@ -18,6 +18,9 @@
 //            scatters x[perm[p]] = y[p] * d[perm[p]] (src:3064-3068).
 // In y' = y[1:n] the multipliers form the strictly lower triangle
 // L'[i, m] = A[i+1, m] (i > m): step m of the reference's sweeps.
+#include <rocblas/rocblas.h>
+
+#include <type_traits>
 #include "lhl_common.h"
 #include "lhl_mfma.h"
 
@@ -71,118 +74,178 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
     for (int i = 0; i < kT; i++) dst[i] = x[i];
 }
 
-// One block step b of y' <- L'^-1 y': the block's rows become Linv[b] * y'_b
-// (redundantly in every workgroup of the step: a 64x64x64 MFMA product); the
-// block's own tile stores them to Yo, the tiles below subtract L'[I, b] * y'_b.
-template <typename T>
-__global__ void __launch_bounds__(256) k_trsm_step(int n, const T *__restrict__ A, int64_t lda,
-                                                   const T *__restrict__ Linv, T *__restrict__ Yw,
-                                                   T *__restrict__ Yo, int64_t ldy, int ncols, int b) {
-    const int N1 = n - 1;
-    const int I = b + blockIdx.x;
+// Lower triangular product over 64x64 tiles, T = L (lower part, L(i, m) =
+// Lm[(i + loff) + m * ldl]):
+//   SWEEP = 0:  Yo' = L Y'          (L = L'^-1 with its unit diagonal stored)
+//   SWEEP = 1:  y_new' = Y' + L' Y' (strict lower part), scattered to
+//               x[perm[p], col] (+)= y_new[p] * d[perm[p]], p = i + 1; the
+//               first row tile also scatters p = 0.
+// Y' is row p = i + 1 of Y.  A workgroup computes the row tiles I and
+// nbt-1-I of one column tile, so every workgroup runs nbt+1 K steps; the next
+// K step's operands are loaded into registers while the MFMAs consume LDS.
+template <typename T, typename TX, int SWEEP>
+__global__ void __launch_bounds__(256) k_trmm_lower(int n, const T *__restrict__ Lm, int64_t ldl, int loff,
+                                                    const T *__restrict__ Y, int64_t ldy, int ncols,
+                                                    T *__restrict__ Yo, int planes, const int *__restrict__ perm,
+                                                    const double *__restrict__ scale, TX *__restrict__ x, int64_t ldx,
+                                                    int64_t xplane, int accumulate) {
+    const int N1 = n - 1, nbt = (N1 + kT - 1) / kT;
     const int c0 = blockIdx.y * kT;
-    const int r0 = b * kT;
-    const int nb = min(kT, N1 - r0);
-    const int tid = threadIdx.x;
-    __shared__ T As[kT][kT + kTilePad];   // As[m][i]: A operand (i, m)
-    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c]: B operand (m, c)
-    const T *Lb = Linv + (int64_t)b * kT * kT;
-    for (int t = tid; t < kT * kT; t += 256) {
-        const int x = t % kT, y = t / kT;
-        As[y][x] = Lb[x + (int64_t)y * kT];
-        Bs[x][y] = (x < nb && c0 + y < ncols) ? Yw[(r0 + x + 1) + (int64_t)(c0 + y) * ldy] : (T)0;
-    }
-    __syncthreads();
-    typename Mfma<T>::acc_t acc[2][2];
-    mfma_tile64_zero<T>(acc);
-    mfma_tile64<T>(As, Bs, kT, acc);
-    __syncthreads();
-    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { Bs[i][c] = v; });   // y'_b, the next B operand
-    const int ri = I * kT;
-    const int ni = min(kT, N1 - ri);
-    if (I != b)
-        for (int t = tid; t < kT * kT; t += 256) {
-            const int x = t % kT, y = t / kT;
-            As[y][x] = (x < ni && y < nb) ? A[(ri + x + 1) + (int64_t)(r0 + y) * lda] : (T)0;
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    __shared__ T As[kT][kT + kTilePad];   // As[m][i] = L(ri+i, rk+m)
+    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c] = Y'(rk+m, c0+c)
+    const int cy = min(c0 + ty, ncols - 1);
+    for (int half = 0; half < 2; half++) {
+        const int I = half == 0 ? (int)blockIdx.x : nbt - 1 - (int)blockIdx.x;
+        if (half == 1 && I <= (int)blockIdx.x) break;
+        const int ri = I * kT, ni = min(kT, N1 - ri);
+        const T *Lrow = Lm + min(ri + tx, N1 - 1) + loff;
+        T ra[16], rb[16];
+        // operands of step K into registers: clamped addresses, out-of-range elements zero
+        auto load = [&](int K) {
+            const int rk = K * kT, nk = min(kT, N1 - rk);
+            const T *Yrow = Y + min(rk + tx, N1 - 1) + 1;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int y = ty + 4 * j;
+                const T av = Lrow[(int64_t)min(rk + y, N1 - 1) * ldl];
+                ra[j] = (tx < ni && y < nk) ? av : (T)0;
+                const T bv = Yrow[(int64_t)min(cy + 4 * j, ncols - 1) * ldy];
+                rb[j] = (tx < nk && c0 + y < ncols) ? bv : (T)0;
+            }
+        };
+        typename Mfma<T>::acc_t acc[2][2];
+        mfma_tile64_zero<T>(acc);
+        load(0);
+        for (int K = 0; K <= I; K++) {
+            __syncthreads();
+            const bool diag = K == I;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int y = ty + 4 * j;
+                const bool keep = !diag || (SWEEP ? y < tx : y <= tx);
+                As[y][tx] = keep ? ra[j] : (T)0;
+                Bs[tx][y] = rb[j];
+            }
+            __syncthreads();
+            if (K < I) load(K + 1);
+            mfma_tile64<T>(As, Bs, kT, acc);
         }
-    __syncthreads();
-    if (I == b) {
+        __syncthreads();
+        mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
+        __syncthreads();
         for (int t = tid; t < kT * kT; t += 256) {
-            const int x = t % kT, y = t / kT;
-            if (x < nb && c0 + y < ncols) Yo[(r0 + x + 1) + (int64_t)(c0 + y) * ldy] = Bs[x][y];
+            const int xx = t % kT, yy = t / kT;
+            const int q = c0 + yy;
+            if (xx >= ni || q >= ncols) continue;
+            const int p = ri + xx + 1;
+            if constexpr (SWEEP) {
+                const int col = q / planes, pl = q % planes;
+                const T yn = Y[p + (int64_t)q * ldy] + As[xx][yy];
+                const int dst_row = perm[p];
+                TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+                const TX v = (TX)yn * (TX)scale[dst_row];
+                if (accumulate) *dst += v;
+                else *dst = v;
+            } else {
+                Yo[p + (int64_t)q * ldy] = As[xx][yy];
+            }
         }
-        return;
-    }
-    mfma_tile64_zero<T>(acc);
-    mfma_tile64<T>(As, Bs, kT, acc);
-    __syncthreads();
-    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
-    __syncthreads();
-    for (int t = tid; t < kT * kT; t += 256) {
-        const int x = t % kT, y = t / kT;
-        if (x < ni && c0 + y < ncols) {
-            T *dst = Yw + (ri + x + 1) + (int64_t)(c0 + y) * ldy;
-            *dst = *dst - As[x][y];
+        if constexpr (SWEEP) {
+            if (I == 0 && tid < kT && c0 + tid < ncols) {
+                const int q = c0 + tid;
+                const int col = q / planes, pl = q % planes;
+                const int dst_row = perm[0];
+                TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+                const TX v = (TX)Y[(int64_t)q * ldy] * (TX)scale[dst_row];
+                if (accumulate) *dst += v;
+                else *dst = v;
+            }
         }
     }
 }
 
-// y_new' = y' + L' y' for one (row tile, column tile), scattered to
-// x[perm[p], col] (+)= y_new[p] * d[perm[p]], p = i' + 1; the row-0 tile also
-// scatters p = 0.
-template <typename T, typename TX>
-__global__ void __launch_bounds__(256) k_trmm_scatter(int n, const T *__restrict__ A, int64_t lda,
-                                                      const T *__restrict__ Y, int64_t ldy, int ncols, int planes,
-                                                      const int *__restrict__ perm, const double *__restrict__ scale,
-                                                      TX *__restrict__ x, int64_t ldx, int64_t xplane,
-                                                      int accumulate) {
-    const int N1 = n - 1;
-    const int I = blockIdx.x;
-    const int c0 = blockIdx.y * kT;
-    const int ri = I * kT;
-    const int ni = min(kT, N1 - ri);
-    const int tid = threadIdx.x;
-    __shared__ T As[kT][kT + kTilePad];   // As[m][i] = L'(ri+i, rk+m)
-    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c] = y'(rk+m, c0+c)
-    typename Mfma<T>::acc_t acc[2][2];
-    mfma_tile64_zero<T>(acc);
-    for (int K = 0; K <= I; K++) {
-        const int rk = K * kT;
-        const int nk = min(kT, N1 - rk);
+// The explicit inverse of L' (n-1 x n-1, unit lower): column j solves
+// L' z = e_j.  One workgroup per 16 columns, left-looking from the block
+// holding the first column (the rows above are zero and never stored); each
+// block is finished with its 64x64 diagonal inverse.  Lf is column-major with
+// leading dimension ldf.
+template <typename T>
+__global__ void __launch_bounds__(64 * 8) k_linv_full(int n, const T *__restrict__ A, int64_t lda,
+                                                      const T *__restrict__ Linv, T *__restrict__ Lf, int64_t ldf) {
+    constexpr int NW = 8, NC = 16;
+    const int N1 = n - 1, nbt = (N1 + kT - 1) / kT;
+    const int c0 = blockIdx.x * NC;
+    const int b0 = c0 / kT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const T *Xc = Lf + (int64_t)min(c0 + li, N1 - 1) * ldf;   // solved rows m at Xc[m]
+    __shared__ T red[NW][kT][NC + 1];
+    __shared__ T Ys[kT][NC + 1];
+    for (int b = b0; b < nbt; b++) {
+        const int r0 = b * kT, nb = min(kT, N1 - r0);
+        typename Mfma<T>::acc_t acc[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::zero();
+        const T *Ar = A + r0 + 4 * li + 1;   // MFMA row i of accumulator rr is block row 4i + rr
+        const bool full = nb == kT;
+        for (int j = 4 * b0 + w; j < 4 * b; j += NW) {
+            const int m0 = 16 * j + 4 * lk;
+            T av[4][4];
+            const V4<T> xv = ld4u(Xc + m0);
+            if (full) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const V4<T> a = ld4u(Ar + (int64_t)(m0 + t) * lda);
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) av[rr][t] = a[rr];
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++)
+                        av[rr][t] = A[min(r0 + 4 * li + rr, N1 - 1) + 1 + (int64_t)(m0 + t) * lda];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::mac(av[rr][t], xv[t], acc[rr]);
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) red[w][4 * Mfma<T>::row(lane, r) + rr][li] = acc[rr][r];
         __syncthreads();
-        for (int t = tid; t < kT * kT; t += 256) {
-            const int xx = t % kT, yy = t / kT;
-            const bool ok = xx < ni && yy < nk && (K < I || yy < xx);
-            As[yy][xx] = ok ? A[(ri + xx + 1) + (int64_t)(rk + yy) * lda] : (T)0;
-            Bs[xx][yy] = (xx < nk && c0 + yy < ncols) ? Y[(rk + xx + 1) + (int64_t)(c0 + yy) * ldy] : (T)0;
+        for (int t = threadIdx.x; t < kT * NC; t += 64 * NW) {
+            const int i = t % kT, c = t / kT;
+            T v = 0;
+            if (i < nb && c0 + c < N1) {
+                T sum = 0;
+#pragma unroll
+                for (int ww = 0; ww < NW; ww++) sum += red[ww][i][c];
+                v = (r0 + i == c0 + c ? (T)1 : (T)0) - sum;
+            }
+            Ys[i][c] = v;
         }
         __syncthreads();
-        mfma_tile64<T>(As, Bs, kT, acc);
-    }
-    __syncthreads();
-    mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
-    __syncthreads();
-    for (int t = tid; t < kT * kT; t += 256) {
-        const int xx = t % kT, yy = t / kT;
-        const int q = c0 + yy;
-        if (xx >= ni || q >= ncols) continue;
-        const int col = q / planes, pl = q % planes;
-        const int p = ri + xx + 1;
-        const T yn = Y[p + (int64_t)q * ldy] + As[xx][yy];
-        const int dst_row = perm[p];
-        TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-        const TX v = (TX)yn * (TX)scale[dst_row];
-        if (accumulate) *dst += v;
-        else *dst = v;
-    }
-    if (I == 0 && tid < kT && c0 + tid < ncols) {
-        const int q = c0 + tid;
-        const int col = q / planes, pl = q % planes;
-        const int dst_row = perm[0];
-        TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-        const TX v = (TX)Y[(int64_t)q * ldy] * (TX)scale[dst_row];
-        if (accumulate) *dst += v;
-        else *dst = v;
+        if (w < 4) {
+            const T *Lb = Linv + (int64_t)b * kT * kT + 16 * w + li;
+            typename Mfma<T>::acc_t xo = Mfma<T>::zero();
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int k = 16 * sl + 4 * lk + t;
+                    xo = Mfma<T>::mac(Lb[(int64_t)k * kT], Ys[k][li], xo);
+                }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * w + Mfma<T>::row(lane, r);
+                if (i < nb && c0 + li < N1) Lf[(r0 + i) + (int64_t)(c0 + li) * ldf] = xo[r];
+            }
+        }
+        __syncthreads();   // the block's rows are visible to every wave
     }
 }
 
@@ -250,321 +313,176 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
 }
 
 struct UGroup {
-    int shift, q0, nq, pad;   // internal columns q0 .. q0+nq-1 (nq <= 16), all of one shift
+    int shift, q0, nq, pad;   // internal columns q0 .. q0+nq-1 (nq <= kNC), all of one shift; nq = 0: idle slot
 };
 
-// One block step b of the back substitution on every group: the block's rows
-// become Uinv[b] * y_b (redundantly per workgroup); the block's own tile
-// stores them to Yf, the tiles above subtract U[I, b] * x_b from Yw.
-// Output tile: 64 rows x 16 internal columns; wave w owns rows 16w..16w+15.
-template <typename T, int UC>
-__global__ void __launch_bounds__(256) k_utrsm_step(int n, const T *__restrict__ Ubase, int64_t ustride,
-                                                    const T *__restrict__ Uinv, int64_t uistride,
-                                                    const UGroup *__restrict__ groups, T *__restrict__ Yw,
-                                                    T *__restrict__ Yf, int64_t ldy, int b) {
-    constexpr int UP = UC ? 2 : 1;
-    constexpr int NC = 16;
-    const int I = blockIdx.x;
-    const UGroup g = groups[blockIdx.y];
-    const int rb = b * kT, nbk = min(kT, n - rb);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    __shared__ T As[UP][kT][kT + kTilePad];   // As[.][k][i]: A operand (i, k)
-    __shared__ T Bs[kT][NC + 1];              // Bs[k][c]:   B operand (k, c)
-    const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
-    for (int t = tid; t < kT * kT; t += 256) {
-        const int x = t % kT, y = t / kT;
-        As[0][y][x] = Ui[x + (int64_t)y * kT];
-        if constexpr (UC) As[UP - 1][y][x] = Ui[kT * kT + x + (int64_t)y * kT];
-    }
-    for (int t = tid; t < kT * NC; t += 256) {
-        const int x = t % kT, y = t / kT;
-        Bs[x][y] = (x < nbk && y < g.nq) ? Yw[(rb + x) + (int64_t)(g.q0 + y) * ldy] : (T)0;
-    }
-    __syncthreads();
-    const int li = lane & 15, lk = lane >> 4;
-    auto product = [&]() {
-        typename Mfma<T>::acc_t p = Mfma<T>::zero(), q = Mfma<T>::zero();
-#pragma unroll 4
-        for (int k0 = 0; k0 < kT; k0 += 4) {
-            const int k = k0 + lk;
-            const T bv = Bs[k][li];
-            p = Mfma<T>::mac(As[0][k][16 * w + li], bv, p);
-            if constexpr (UC) q = Mfma<T>::mac(As[UP - 1][k][16 * w + li], bv, q);
-        }
-        if constexpr (UC) {
-            // internal columns alternate re, im: re = Pr - Qi, im = Pi + Qr (neighbour lane holds col ^ 1)
+// ---- back substitution with U: kNC internal columns per workgroup ---------
+// With few columns per workgroup the batch spreads over every CU (1024
+// internal columns fill 256 CUs) and each shift's U streams from HBM once,
+// shared through the L2 of the XCD its workgroups are placed on.  VALU FMAs:
+// four columns per row leave no MFMA shape worth using.
+constexpr int kNC = 4;
+
+// Sum of v[0..V-1] over the 64 lanes, scattered: lane l returns the total of
+// v[l % V] (V a power of two <= 64): fold the lane halves down to V lanes,
+// then a butterfly reduce-scatter (63 exchanges in all).
+template <typename T, int V>
+__device__ __forceinline__ T wave_reduce_scatter(T (&v)[V]) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const T qn = __shfl_xor(q[r], 1, 64);
-                p[r] = (li & 1) ? p[r] + qn : p[r] - qn;
-            }
-        }
-        return p;
-    };
-    typename Mfma<T>::acc_t xb = product();
-    __syncthreads();
+    for (int span = 32; span >= V; span >>= 1)
 #pragma unroll
-    for (int r = 0; r < 4; r++) Bs[16 * w + Mfma<T>::row(lane, r)][li] = xb[r];
-    const int ri = I * kT;
-    if (I != b) {
-        for (int t = tid; t < kT * kT; t += 256) {
-            const int k = t % kT, i = t / kT;
-            const int row = ri + i;
-            const int64_t o = upack_off(n, row) - row + rb + k;
-            const bool ok = k < nbk;
-            As[0][k][i] = ok ? Ubase[(int64_t)g.shift * ustride + o] : (T)0;
-            if constexpr (UC) As[UP - 1][k][i] = ok ? Ubase[(int64_t)g.shift * ustride + ustride / 2 + o] : (T)0;
-        }
-    }
-    __syncthreads();
-    if (I == b) {
-        for (int t = tid; t < kT * NC; t += 256) {
-            const int x = t % kT, y = t / kT;
-            if (x < nbk && y < g.nq) Yf[(rb + x) + (int64_t)(g.q0 + y) * ldy] = Bs[x][y];
-        }
-        return;
-    }
-    typename Mfma<T>::acc_t up = product();
-    __syncthreads();
+        for (int j = 0; j < V; j++) v[j] += __shfl_xor(v[j], span, 64);
 #pragma unroll
-    for (int r = 0; r < 4; r++) As[0][li][16 * w + Mfma<T>::row(lane, r)] = up[r];   // As[0][c][i] staging
-    __syncthreads();
-    for (int t = tid; t < kT * NC; t += 256) {
-        const int x = t % kT, y = t / kT;
-        if (y < g.nq) {
-            T *dst = Yw + (ri + x) + (int64_t)(g.q0 + y) * ldy;
-            *dst = *dst - As[0][y][x];
+    for (int half = V / 2; half >= 1; half >>= 1) {
+        const bool hi = (lane & half) != 0;
+#pragma unroll
+        for (int j = 0; j < half; j++) {
+            const T keep = hi ? v[j + half] : v[j];
+            const T send = hi ? v[j] : v[j + half];
+            v[j] = keep + __shfl_xor(send, half, 64);
         }
     }
+    return v[0];
 }
 
+// x <- U_s^-1 y for kNC internal columns of one shift (two complex columns,
+// re/im interleaved, when UC).  U's rows are packed, so lanes run along a row
+// (each lane a run of four, coalesced) and each wave owns RW rows of the
+// 64-row block; per-lane partial sums are combined by a reduce-scatter.  All
+// loads of a 256-wide chunk are in flight before the first FMA.  Workgroups of
+// one shift are placed on one XCD by the host so U comes from HBM once per
+// shift.  RW = 16 (4 waves) for FP32, 8 (8 waves) for FP64.
+template <typename T> constexpr int utrsm4_rows() { return sizeof(T) == 4 ? 16 : 8; }
 
-// ---- column-owner triangular solves ---------------------------------------
-// One workgroup owns kCN internal columns and walks every 64-row block in
-// order, left-looking: Y_b <- inv(T_bb) (Y_b - sum_K T[b,K] X_K).  Columns are
-// independent, so no workgroup waits on another and one launch covers the
-// whole solve.  kCW waves split the K sum (partial tiles reduced through LDS);
-// four of them then apply the precomputed 64x64 diagonal-block inverse.
-// Within each 16-wide K slab lane (i, q) takes k = 16j + 4q + t, t = 0..3, so
-// both operands are read as runs of four consecutive elements.
-constexpr int kCW = 8;
-constexpr int kCN = 16;
-
-// Four consecutive elements through one (float) or two (double) vector loads
-// that need only element alignment.
-template <typename T> struct V4T;
-template <> struct V4T<float> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
-template <> struct V4T<double> { typedef double type __attribute__((ext_vector_type(4), aligned(8))); };
-template <typename T> using V4 = typename V4T<T>::type;
-template <typename T> __device__ __forceinline__ V4<T> ld4u(const T *p) { return *(const V4<T> *)p; }
-
-// y' <- L'^-1 y': Yw holds the gathered columns (rows 1..n-1), Yo receives the
-// solution (row 0 is already there).
-template <typename T>
-__global__ void __launch_bounds__(64 * kCW) k_ltrsm_cols(int n, const T *__restrict__ A, int64_t lda,
-                                                         const T *__restrict__ Linv, const T *__restrict__ Yw,
-                                                         T *__restrict__ Yo, int64_t ldy, int Q) {
-    const int N1 = n - 1, nbt = (N1 + kT - 1) / kT;
-    const int c0 = blockIdx.x * kCN;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const T *Xc = Yo + (int64_t)min(c0 + li, Q - 1) * ldy + 1;   // solved rows m at Xc[m]
-    __shared__ T red[kCW][kT][kCN + 1];
-    __shared__ T Ys[kT][kCN + 1];
-    for (int b = 0; b < nbt; b++) {
-        const int r0 = b * kT, nb = min(kT, N1 - r0);
-        // MFMA row i of accumulator rr is block row 4i + rr, so each lane reads
-        // four consecutive rows of a column of L' as one run.
-        typename Mfma<T>::acc_t acc[4];
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::zero();
-        const T *Ar = A + r0 + 4 * li + 1;   // L'(i, m) = A(i + 1, m)
-        const bool full = nb == kT;
-        for (int j = w; j < 4 * b; j += kCW) {
-            const int m0 = 16 * j + 4 * lk;
-            T av[4][4];
-            const V4<T> xv = ld4u(Xc + m0);
-            if (full) {
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const V4<T> a = ld4u(Ar + (int64_t)(m0 + t) * lda);
-#pragma unroll
-                    for (int rr = 0; rr < 4; rr++) av[rr][t] = a[rr];
-                }
-            } else {
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-#pragma unroll
-                    for (int rr = 0; rr < 4; rr++)
-                        av[rr][t] = A[min(r0 + 4 * li + rr, N1 - 1) + 1 + (int64_t)(m0 + t) * lda];
-            }
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) acc[rr] = Mfma<T>::mac(av[rr][t], xv[t], acc[rr]);
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) red[w][4 * Mfma<T>::row(lane, r) + rr][li] = acc[rr][r];
-        __syncthreads();
-        for (int t = threadIdx.x; t < kT * kCN; t += 64 * kCW) {
-            const int i = t % kT, c = t / kT;
-            T v = 0;
-            if (i < nb && c0 + c < Q) {
-                T sum = 0;
-#pragma unroll
-                for (int ww = 0; ww < kCW; ww++) sum += red[ww][i][c];
-                v = Yw[(r0 + i + 1) + (int64_t)(c0 + c) * ldy] - sum;
-            }
-            Ys[i][c] = v;
-        }
-        __syncthreads();
-        if (w < 4) {
-            const T *Lb = Linv + (int64_t)b * kT * kT + 16 * w + li;
-            typename Mfma<T>::acc_t x = Mfma<T>::zero();
-#pragma unroll
-            for (int sl = 0; sl < 4; sl++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int k = 16 * sl + 4 * lk + t;
-                    x = Mfma<T>::mac(Lb[(int64_t)k * kT], Ys[k][li], x);
-                }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = 16 * w + Mfma<T>::row(lane, r);
-                if (i < nb && c0 + li < Q) Yo[(r0 + i + 1) + (int64_t)(c0 + li) * ldy] = x[r];
-            }
-        }
-        __syncthreads();   // the block's solved rows are visible to every wave
-    }
-}
-
-// x <- U_s^-1 y for one group of internal columns of shift s: Yin holds the
-// forward-chain output, Yf receives the solution.  Complex shifts: the
-// columns alternate (re, im) and U is planar (imaginary plane ustride/2 on).
 template <typename T, int UC>
-__global__ void __launch_bounds__(64 * kCW) k_utrsm_cols(int n, const T *__restrict__ Ubase, int64_t ustride,
-                                                         const T *__restrict__ Uinv, int64_t uistride,
-                                                         const UGroup *__restrict__ groups, const T *__restrict__ Yin,
-                                                         T *__restrict__ Yf, int64_t ldy) {
+__global__ void __launch_bounds__(64 * (64 / utrsm4_rows<T>())) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restrict__ Uinv, int64_t uistride,
+         const UGroup *__restrict__ groups, const T *__restrict__ Yin, T *__restrict__ Yf, int64_t ldy) {
     constexpr int UP = UC ? 2 : 1;
+    constexpr int RW = utrsm4_rows<T>();
+    constexpr int NV = RW * kNC;           // partial sums per lane
+    constexpr int NT = 64 * (64 / RW);     // threads
     const UGroup g = groups[blockIdx.x];
+    if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
     const int nbu = (n + kT - 1) / kT;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const T *Xc = Yf + (int64_t)(g.q0 + min(li, g.nq - 1)) * ldy;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const T *Us = Ubase + (int64_t)g.shift * ustride;
     const int64_t up = ustride / 2;
-    __shared__ T red[kCW][kT][kCN + 1];
-    __shared__ T Ys[kT][kCN + 1];
+    const T *Xc[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; c++) Xc[c] = Yf + (int64_t)(g.q0 + min(c, g.nq - 1)) * ldy;
+    __shared__ T Ysum[kT][kNC];
+    __shared__ T Ys[kT][kNC];
     for (int b = nbu - 1; b >= 0; b--) {
         const int rb = b * kT, nbk = min(kT, n - rb);
-        typename Mfma<T>::acc_t acc[4], qc[4];
-        const T *Ur[4];
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            acc[rr] = Mfma<T>::zero();
-            qc[rr] = Mfma<T>::zero();
-            const int row = min(rb + 16 * rr + li, n - 1);
-            Ur[rr] = Us + upack_off(n, row) - row;   // U(row, m) = Ur[m], m >= row
-        }
         const int kbeg = rb + kT;
-        const int nsl = kbeg < n ? (n - kbeg + 15) / 16 : 0;
-        for (int j = w; j < nsl; j += kCW) {
-            const int m0 = kbeg + 16 * j + 4 * lk;
-            T bv[4], av[4][4], ai[4][4];
-            if (m0 + 3 < n) {
-                // one 4-element run per lane and row (dword aligned vector loads)
-                const V4<T> xv = ld4u(Xc + m0);
+        T v[NV];
 #pragma unroll
-                for (int t = 0; t < 4; t++) bv[t] = xv[t];
+        for (int j = 0; j < NV; j++) v[j] = 0;
+        if (kbeg < n) {
+            int64_t ro[RW];   // rows rb + RW*w + p (all < n here)
 #pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const V4<T> u = ld4u(Ur[rr] + m0);
+            for (int p = 0; p < RW; p++) {
+                const int row = rb + RW * w + p;
+                ro[p] = upack_off(n, row) - row;
+            }
+            // k-chunks of 256: lane l takes the run m0 + 4l .. m0 + 4l + 3
+            int m0 = kbeg;
+            for (; m0 + 256 <= n; m0 += 256) {
+                const int m = m0 + 4 * lane;
+                V4<T> xv[kNC], urv[RW], uiv[UC ? RW : 1];
 #pragma unroll
-                    for (int t = 0; t < 4; t++) av[rr][t] = u[t];
+                for (int c = 0; c < kNC; c++) xv[c] = ld4u(Xc[c] + m);
+#pragma unroll
+                for (int p = 0; p < RW; p++) {
+                    urv[p] = ld4u(Us + ro[p] + m);
+                    if constexpr (UC) uiv[p] = ld4u(Us + up + ro[p] + m);
+                }
+#pragma unroll
+                for (int p = 0; p < RW; p++) {
+                    const V4<T> ur = urv[p];
                     if constexpr (UC) {
-                        const V4<T> v = ld4u(Ur[rr] + m0 + up);
+                        const V4<T> ui = uiv[p];
 #pragma unroll
-                        for (int t = 0; t < 4; t++) ai[rr][t] = v[t];
+                        for (int t = 0; t < 4; t++) {
+                            v[4 * p + 0] = __builtin_fma(ur[t], xv[0][t], __builtin_fma(-ui[t], xv[1][t], v[4 * p + 0]));
+                            v[4 * p + 1] = __builtin_fma(ur[t], xv[1][t], __builtin_fma(ui[t], xv[0][t], v[4 * p + 1]));
+                            v[4 * p + 2] = __builtin_fma(ur[t], xv[2][t], __builtin_fma(-ui[t], xv[3][t], v[4 * p + 2]));
+                            v[4 * p + 3] = __builtin_fma(ur[t], xv[3][t], __builtin_fma(ui[t], xv[2][t], v[4 * p + 3]));
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+#pragma unroll
+                            for (int c = 0; c < kNC; c++) v[4 * p + c] = __builtin_fma(ur[t], xv[c][t], v[4 * p + c]);
                     }
+                }
+            }
+            // the last partial chunk: clamped addresses, out-of-range terms selected to zero
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int m = m0 + 4 * lane + t;
+                const bool ok = m < n;
+                const int mc = min(m, n - 1);
+                T xv[kNC];
+#pragma unroll
+                for (int c = 0; c < kNC; c++) {
+                    const T x = Xc[c][mc];
+                    xv[c] = ok ? x : (T)0;
+                }
+#pragma unroll
+                for (int p = 0; p < RW; p++) {
+                    const T u0 = Us[ro[p] + mc];
+                    const T ur = ok ? u0 : (T)0;
+                    if constexpr (UC) {
+                        const T u1 = Us[up + ro[p] + mc];
+                        const T ui = ok ? u1 : (T)0;
+                        v[4 * p + 0] = __builtin_fma(ur, xv[0], __builtin_fma(-ui, xv[1], v[4 * p + 0]));
+                        v[4 * p + 1] = __builtin_fma(ur, xv[1], __builtin_fma(ui, xv[0], v[4 * p + 1]));
+                        v[4 * p + 2] = __builtin_fma(ur, xv[2], __builtin_fma(-ui, xv[3], v[4 * p + 2]));
+                        v[4 * p + 3] = __builtin_fma(ur, xv[3], __builtin_fma(ui, xv[2], v[4 * p + 3]));
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < kNC; c++) v[4 * p + c] = __builtin_fma(ur, xv[c], v[4 * p + c]);
+                    }
+                }
+            }
+        }
+        const T tot = wave_reduce_scatter<T, NV>(v);   // lane l < NV: row RW*w + l/4, column l%4
+        if (lane < NV) Ysum[RW * w + (lane >> 2)][lane & 3] = tot;
+        __syncthreads();
+        for (int t = tid; t < kT * kNC; t += NT) {
+            const int i = t >> 2, c = t & 3;
+            Ys[i][c] = (i < nbk && c < g.nq) ? Yin[(rb + i) + (int64_t)(g.q0 + c) * ldy] - Ysum[i][c] : (T)0;
+        }
+        __syncthreads();
+        if (tid < kT * kNC) {
+            const int i = tid >> 2, c = tid & 3;
+            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP + i;
+            T x = 0;
+            if constexpr (UC) {
+                const int cr = c & ~1;   // the complex column's real part
+                const bool im = (c & 1) != 0;
+#pragma unroll 8
+                for (int k = 0; k < kT; k++) {
+                    const T ar = Ui[k * kT], ai = Ui[kT * kT + k * kT];
+                    const T yr = Ys[k][cr], yi = Ys[k][cr + 1];
+                    x = im ? __builtin_fma(ar, yi, __builtin_fma(ai, yr, x))
+                           : __builtin_fma(ar, yr, __builtin_fma(-ai, yi, x));
                 }
             } else {
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int m = m0 + t;
-                    const bool ok = m < n;
-                    const int mc = ok ? m : n - 1;
-                    bv[t] = ok ? Xc[mc] : (T)0;
-#pragma unroll
-                    for (int rr = 0; rr < 4; rr++) {
-                        av[rr][t] = Ur[rr][mc];
-                        if constexpr (UC) ai[rr][t] = Ur[rr][mc + up];
-                    }
-                }
+#pragma unroll 16
+                for (int k = 0; k < kT; k++) x = __builtin_fma(Ui[k * kT], Ys[k][c], x);
             }
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    acc[rr] = Mfma<T>::mac(av[rr][t], bv[t], acc[rr]);
-                    if constexpr (UC) qc[rr] = Mfma<T>::mac(ai[rr][t], bv[t], qc[rr]);
-                }
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                T v = acc[rr][r];
-                if constexpr (UC) {
-                    // re = Ur*xr - Ui*xi, im = Ur*xi + Ui*xr (the neighbour lane holds col ^ 1)
-                    const T qn = __shfl_xor(qc[rr][r], 1, 64);
-                    v = (li & 1) ? v + qn : v - qn;
-                }
-                red[w][16 * rr + Mfma<T>::row(lane, r)][li] = v;
-            }
-        __syncthreads();
-        for (int t = threadIdx.x; t < kT * kCN; t += 64 * kCW) {
-            const int i = t % kT, c = t / kT;
-            T v = 0;
-            if (i < nbk && c < g.nq) {
-                T sum = 0;
-#pragma unroll
-                for (int ww = 0; ww < kCW; ww++) sum += red[ww][i][c];
-                v = Yin[(rb + i) + (int64_t)(g.q0 + c) * ldy] - sum;
-            }
-            Ys[i][c] = v;
-        }
-        __syncthreads();
-        if (w < 4) {
-            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP + 16 * w + li;
-            typename Mfma<T>::acc_t x = Mfma<T>::zero(), xq = Mfma<T>::zero();
-#pragma unroll
-            for (int sl = 0; sl < 4; sl++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int k = 16 * sl + 4 * lk + t;
-                    const T yv = Ys[k][li];
-                    x = Mfma<T>::mac(Ui[(int64_t)k * kT], yv, x);
-                    if constexpr (UC) xq = Mfma<T>::mac(Ui[kT * kT + (int64_t)k * kT], yv, xq);
-                }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                T v = x[r];
-                if constexpr (UC) {
-                    const T qn = __shfl_xor(xq[r], 1, 64);
-                    v = (li & 1) ? v + qn : v - qn;
-                }
-                const int i = 16 * w + Mfma<T>::row(lane, r);
-                if (i < nbk && li < g.nq) Yf[(rb + i) + (int64_t)(g.q0 + li) * ldy] = v;
-            }
+            if (i < nbk && c < g.nq) Yf[(rb + i) + (int64_t)(g.q0 + c) * ldy] = x;
         }
         __syncthreads();
     }
 }
 
-// ---- the refinement residual: JX = J X (FP64 tiled GEMM), then
+// ---- the refinement residual: JX = J X (rocBLAS GEMM in one precision, the
+// tiled MFMA GEMM below for FP32 J with FP64 x), then
 // r = b - (sigma x + tau JX) per column with its shift, and column norms.
 // X, B, R are in the staged planar layout (ld, plane); JX uses the internal
 // column order q = col*planes + pl with leading dimension ldq.
@@ -695,11 +613,21 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         cb[0] = 0;
         cb[1] = a.ncol;
     }
-    std::vector<UGroup> ug;
+    // U solve groups: kNC internal columns of one shift each.  Workgroups are
+    // dealt to the 8 XCDs round-robin (blockIdx % 8), so all groups of a shift
+    // go to slots of one residue class and share that XCD's L2 copy of U;
+    // empty slots (nq = 0) pad the classes to equal length.
+    constexpr int kXcd = 8;
+    std::vector<std::vector<UGroup>> per(kXcd);
     for (int64_t s = 0; s < a.nshift; s++) {
-        for (int64_t q = cb[s] * planes; q < cb[s + 1] * planes; q += 16)
-            ug.push_back(UGroup{(int)s, (int)q, (int)std::min<int64_t>(16, cb[s + 1] * planes - q), 0});
+        for (int64_t q = cb[s] * planes; q < cb[s + 1] * planes; q += kNC)
+            per[s % kXcd].push_back(UGroup{(int)s, (int)q, (int)std::min<int64_t>(kNC, cb[s + 1] * planes - q), 0});
     }
+    size_t depth = 0;
+    for (const auto &v : per) depth = std::max(depth, v.size());
+    std::vector<UGroup> ug(depth * kXcd, UGroup{0, 0, 0, 0});
+    for (int x = 0; x < kXcd; x++)
+        for (size_t j = 0; j < per[x].size(); j++) ug[j * kXcd + x] = per[x][j];
     LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
     UGroup *dg = h->gbuf.as<UGroup>();
     LHL_HIP_TRY(hipMemcpy(dg, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice));
@@ -711,10 +639,11 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int N1 = n - 1;
     const int nbt = (N1 + kT - 1) / kT;
     const unsigned ct = (unsigned)((Q + kT - 1) / kT);
-    const T *Linv = h->Linv.as<T>();
+    // y' <- L'^-1 y' as a product with the explicit inverse formed by lhl()
     if (nbt > 0)
-        LHL_LAUNCH(h, "trsm", k_ltrsm_cols<T>, dim3((unsigned)((Q + kCN - 1) / kCN)), dim3(64 * kCW), 0, n, A, h->lda,
-                   Linv, (const T *)Yw, Yo, ldy, Q);
+        LHL_LAUNCH(h, "trsm", (k_trmm_lower<T, T, 0>), dim3((unsigned)((nbt + 1) / 2), ct), dim3(256), 0, n,
+                   (const T *)h->Lfull.as<T>(), h->ldlf, 0, (const T *)Yw, ldy, Q, Yo, planes, (const int *)nullptr,
+                   (const double *)nullptr, (T *)nullptr, (int64_t)0, (int64_t)0, 0);
     // forward interchange chain in place on Yo (columns resident in LDS)
     LHL_TRY(hess_device(h, a, Yo, ldy));
     // x <- U^-1 y per shift (solution in Yw)
@@ -722,15 +651,15 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
     const dim3 gu((unsigned)ug.size());
     if (a.shift_complex)
-        LHL_LAUNCH(h, "utrsm", (k_utrsm_cols<T, 1>), gu, dim3(64 * kCW), 0, n, (const T *)a.U, a.ustride,
-                   (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 1>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U, a.ustride, (const T *)a.Uinv,
+                   uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
     else
-        LHL_LAUNCH(h, "utrsm", (k_utrsm_cols<T, 0>), gu, dim3(64 * kCW), 0, n, (const T *)a.U, a.ustride,
-                   (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 0>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U, a.ustride, (const T *)a.Uinv,
+                   uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
     LHL_HIP_TRY(hipGetLastError());
-    LHL_LAUNCH(h, "trmm", (k_trmm_scatter<T, TX>), dim3((unsigned)std::max(nbt, 1), ct), dim3(256), 0, n, A, h->lda,
-               (const T *)Yw, ldy, Q, planes, (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(),
-               (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
+    LHL_LAUNCH(h, "trmm", (k_trmm_lower<T, TX, 1>), dim3((unsigned)std::max((nbt + 1) / 2, 1), ct), dim3(256), 0, n,
+               A, h->lda, 1, (const T *)Yw, ldy, Q, (T *)nullptr, planes, (const int *)h->perm.as<int>(),
+               (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -744,12 +673,50 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     LHL_TRY(Uinv.ensure(sizeof(T) * (size_t)uis * (size_t)nshift));
     *uistride = uis;
     if (cplx)
-        LHL_LAUNCH(h, "shift", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
+        LHL_LAUNCH(h, "uinv", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
                    ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
     else
-        LHL_LAUNCH(h, "shift", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
+        LHL_LAUNCH(h, "uinv", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
                    ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
     LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+Status blas_handle(lhl_handle *h, rocblas_handle *out) {
+    if (!h->blas) {
+        rocblas_handle b = nullptr;
+        if (rocblas_create_handle(&b) != rocblas_status_success) return {LHL_ERR_HIP, "rocblas_create_handle failed"};
+        if (rocblas_set_stream(b, h->stream) != rocblas_status_success ||
+            rocblas_set_pointer_mode(b, rocblas_pointer_mode_host) != rocblas_status_success) {
+            rocblas_destroy_handle(b);
+            return {LHL_ERR_HIP, "rocblas_set_stream failed"};
+        }
+        h->blas = b;
+    }
+    *out = (rocblas_handle)h->blas;
+    return {};
+}
+
+// JX[:, q] = W X[:, q] for every plane through rocBLAS (same element type for
+// W and X): one GEMM per plane, the output columns interleaved by plane.
+template <typename T>
+Status blas_jx(lhl_handle *h, const T *W, int64_t ldw, const T *X, int64_t ld, int64_t plane, int planes, int ncol,
+               T *JX, int64_t ldq) {
+    rocblas_handle b;
+    LHL_TRY(blas_handle(h, &b));
+    const int n = (int)h->n;
+    const T one = 1, zero = 0;
+    ProfScope ps(h, "gemm_jx");
+    for (int pl = 0; pl < planes; pl++) {
+        rocblas_status st;
+        if constexpr (std::is_same<T, double>::value)
+            st = rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
+                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
+        else
+            st = rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
+                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
+        if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS GEMM of the residual failed"};
+    }
     return {};
 }
 
@@ -763,25 +730,36 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     TX *JX = h->ybuf.as<TX>();
     TX *JXi = nullptr;
     const dim3 gg((unsigned)((n + kT - 1) / kT), (unsigned)((Q + kT - 1) / kT));
-    if (a.W) {
-        LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)a.W, a.ldw, (const TX *)a.x,
+    if constexpr (std::is_same<TW, TX>::value) {
+        // plain GEMM in one precision: the library's tuned kernels
+        const TW *W = a.W ? (const TW *)a.W : (const TW *)h->J.p;
+        const int64_t ldw = a.W ? a.ldw : h->ldj;
+        LHL_TRY(blas_jx<TX>(h, W, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol, JX, ldq));
+        if (a.W && a.w_complex) {
+            LHL_TRY(h->rbuf.ensure(sizeof(TX) * (size_t)ldq * Q));
+            JXi = h->rbuf.as<TX>();
+            LHL_TRY(blas_jx<TX>(h, W + ldw * (int64_t)n, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol,
+                                JXi, ldq));
+        }
+    } else if (a.W) {
+        LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)a.W, a.ldw, (const TX *)a.x,
                    a.ld, a.plane, planes, Q, JX, ldq);
         if (a.w_complex) {
             LHL_TRY(h->rbuf.ensure(sizeof(TX) * (size_t)ldq * Q));
             JXi = h->rbuf.as<TX>();
             const TW *Wi = (const TW *)a.W + a.ldw * (int64_t)n;
-            LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, Wi, a.ldw, (const TX *)a.x, a.ld,
+            LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, Wi, a.ldw, (const TX *)a.x, a.ld,
                        a.plane, planes, Q, JXi, ldq);
         }
     } else {
-        LHL_LAUNCH(h, "residual", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)h->J.p, h->ldj,
+        LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)h->J.p, h->ldj,
                    (const TX *)a.x, a.ld, a.plane, planes, Q, JX, ldq);
     }
     const dim3 gc((unsigned)((n + 255) / 256), (unsigned)a.ncol);
-    LHL_LAUNCH(h, "residual", k_resid_combine<TX>, gc, dim3(256), 0, n, (const TX *)JX, ldq, (const TX *)JXi,
+    LHL_LAUNCH(h, "resid_combine", k_resid_combine<TX>, gc, dim3(256), 0, n, (const TX *)JX, ldq, (const TX *)JXi,
                (const TX *)a.x, (const TX *)a.b, (TX *)a.r, a.ld, a.plane, planes, a.W ? nullptr : a.colshift,
                a.W ? a.w_complex : 0);
-    LHL_LAUNCH(h, "residual", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
+    LHL_LAUNCH(h, "colnorms", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
                (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms);
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -795,6 +773,14 @@ Status linv_typed(lhl_handle *h) {
     if (nbt > 0)
         LHL_LAUNCH(h, "reduce_finish", k_linv<T>, dim3((unsigned)nbt), dim3(64), 0, n, (const T *)h->A.as<T>(), h->lda,
                    h->Linv.as<T>());
+    // the explicit inverse of L' used by every solve (lower part; the rows
+    // above each column's diagonal block are never read)
+    const int64_t ldf = std::max<int64_t>(64, ((int64_t)(n - 1) + 63) / 64 * 64);
+    LHL_TRY(h->Lfull.ensure(sizeof(T) * (size_t)ldf * (size_t)std::max(n - 1, 1)));
+    h->ldlf = ldf;
+    if (nbt > 0)
+        LHL_LAUNCH(h, "reduce_finish", k_linv_full<T>, dim3((unsigned)((n - 1 + 15) / 16)), dim3(512), 0, n,
+                   (const T *)h->A.as<T>(), h->lda, (const T *)h->Linv.as<T>(), h->Lfull.as<T>(), ldf);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -821,6 +807,11 @@ Status solve_device(lhl_handle *h, const SolveArgs &a) {
     }
     if (a.xdt == LHL_F64) return solve_batch_typed<float, double>(h, a);
     return solve_batch_typed<float, float>(h, a);
+}
+
+void blas_release(lhl_handle *h) {
+    if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
+    h->blas = nullptr;
 }
 
 Status residual_device(lhl_handle *h, const ResidArgs &a) {

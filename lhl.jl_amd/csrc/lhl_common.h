@@ -25,6 +25,19 @@ __device__ __forceinline__ void lds_barrier() {
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 
+// LDS one workgroup may use on the current device (160 KB on gfx950).
+inline size_t lds_limit() {
+    static int cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        cached = v < kMaxLds ? v : kMaxLds;
+    }
+    return (size_t)cached;
+}
+
 // Status carried out of internal helpers; the ABI layer stores the message.
 struct Status {
     int code = LHL_OK;
@@ -137,9 +150,12 @@ struct lhl_handle {
     int reduced = 0;
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
-    lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L (solve)
+    lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
+    lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
+    int64_t ldlf = 0;
     lhl::DevBuf ybuf, rbuf, nbuf, gbuf, xstage, bstage;  // solve / refine scratch
     lhl::Profiler prof;
+    void *blas = nullptr;       // rocBLAS handle on `stream` (the residual's plain GEMM), created on first use
     lhl_shift_state *own = nullptr;   // ws.shift
     std::vector<lhl_shift_state *> held; // LHLShift states created on this handle
     std::string err;
@@ -235,6 +251,7 @@ struct ResidArgs {
     int x_complex = 0;
     double *norms = nullptr;
 };
+void blas_release(lhl_handle *h);                          // lhl_batch.hip
 Status residual_device(lhl_handle *h, const ResidArgs &a);  // lhl_batch.hip
 
 // Planar <-> interleaved conversions and precision casts (lhl_solve.hip).

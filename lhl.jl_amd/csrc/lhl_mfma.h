@@ -38,6 +38,14 @@ template <> struct Mfma<double> {
 
 constexpr int kTilePad = 4;   // LDS row padding of the staged operands
 
+// Four consecutive elements through one (float) or two (double) vector loads
+// that need only element alignment.
+template <typename T> struct V4T;
+template <> struct V4T<float> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
+template <> struct V4T<double> { typedef double type __attribute__((ext_vector_type(4), aligned(8))); };
+template <typename T> using V4 = typename V4T<T>::type;
+template <typename T> __device__ __forceinline__ V4<T> ld4u(const T *p) { return *(const V4<T> *)p; }
+
 // acc[bi][bj] += As[0:kk, quadrant rows] * Bs[0:kk, quadrant cols]; kk % 4 == 0.
 template <typename T>
 __device__ __forceinline__ void mfma_tile64(const T (*As)[64 + kTilePad], const T (*Bs)[64 + kTilePad], int kk,

@@ -349,18 +349,6 @@ __global__ void k_from_planar(const TS *__restrict__ src, int64_t lds, int64_t p
 
 // ---------------------------------------------------------------- launchers
 
-// LDS one workgroup may use on the current device (160 KB on gfx950).
-inline size_t lds_limit() {
-    static int cached = -1;
-    if (cached < 0) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-            v = 64 * 1024;
-        cached = v < kMaxLds ? v : kMaxLds;
-    }
-    return (size_t)cached;
-}
 
 // Right-hand sides per workgroup: as many as fit in LDS, at most 4.
 inline int pick_nr(int64_t n, int planes, size_t es) {
@@ -376,7 +364,7 @@ Status launch_hess_t(lhl_handle *h, const SolveArgs &a, const Group *groups, int
     const size_t lds = sizeof(T) * (size_t)NR * PL * n;
     auto fn = k_hess<T, NR, PL, UC, false>;   // the forward chain only; U is solved by the blocked kernels
     LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    LHL_LAUNCH(h, "hess", fn, dim3(ngroups), dim3(kSolveThreads), lds, n, (const T *)a.U, a.ustride, (const T *)a.l,
+    LHL_LAUNCH(h, "fwd_chain", fn, dim3(ngroups), dim3(kSolveThreads), lds, n, (const T *)a.U, a.ustride, (const T *)a.l,
                (const T *)a.rdiag, a.lstride, a.swp, a.sstride, groups, (T *)Y, ldy);
     LHL_HIP_TRY(hipGetLastError());
     return {};
