@@ -172,8 +172,17 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
     for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += gridDim.x * kRedThreads) {
         T v = col[i];
         if (add) {
+            // partials summed in chunk order; loads issued eight at a time
             T s = 0;
-            for (int c = 0; c < nch; c++) s += part[i + (int64_t)c * ldp];
+            int c = 0;
+            for (; c + 8 <= nch; c += 8) {
+                T pv[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) pv[u] = part[i + (int64_t)(c + u) * ldp];
+#pragma unroll
+                for (int u = 0; u < 8; u++) s += pv[u];
+            }
+            for (; c < nch; c++) s += part[i + (int64_t)c * ldp];
             v = v + s;
             col[i] = v;
         }
@@ -208,15 +217,25 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
         last = (atomicAdd(counter, 1u) == gridDim.x - 1);
     }
     __syncthreads();
-    if (!last || threadIdx.x != 0) return;
+    if (!last || threadIdx.x >= 64) return;
     __threadfence();
+    // one wave: lane b reads block b's candidate (gridDim.x <= 64), then a
+    // butterfly argmax with the same rule (larger value, ties to the smaller row)
+    const int lane = threadIdx.x;
     T gv = (T)-1, gx = 0;
     int gi = INT_MAX;
-    for (int b = 0; b < (int)gridDim.x; b++) {
-        T v2 = ld_coherent(&cand_v[b]);
-        int i2 = ld_coherent(&cand_i[b]);
-        if (v2 > gv || (v2 == gv && i2 < gi)) { gv = v2; gi = i2; gx = ld_coherent(&cand_x[b]); }
+    if (lane < (int)gridDim.x) {
+        gv = ld_coherent(&cand_v[lane]);
+        gi = ld_coherent(&cand_i[lane]);
+        gx = ld_coherent(&cand_x[lane]);
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        const T v2 = __shfl_xor(gv, o, 64);
+        const int i2 = __shfl_xor(gi, o, 64);
+        const T x2 = __shfl_xor(gx, o, 64);
+        if (v2 > gv || (v2 == gv && i2 < gi)) { gv = v2; gi = i2; gx = x2; }
+    }
+    if (lane != 0) return;
     const T a1 = ld_coherent(vk1);
     int p = k + 1;
     T piv = a1;
@@ -317,20 +336,19 @@ __global__ void __launch_bounds__(kRedThreads) k_trailing(T *__restrict__ A, int
     T *a = A + i + (int64_t)j0 * lda;
     const int m = j1 - j0;
     int t = 0;
+    // columns in batches of 16 loads issued together; the FMA order is the
+    // column order either way
     if (upd) {
-        for (; t + 4 <= m; t += 4) {
-            T a0 = a[(int64_t)t * lda], a1 = a[(int64_t)(t + 1) * lda];
-            T a2 = a[(int64_t)(t + 2) * lda], a3 = a[(int64_t)(t + 3) * lda];
-            a0 = __builtin_fma(li, rj[t], a0);
-            a1 = __builtin_fma(li, rj[t + 1], a1);
-            a2 = __builtin_fma(li, rj[t + 2], a2);
-            a3 = __builtin_fma(li, rj[t + 3], a3);
-            a[(int64_t)t * lda] = a0; a[(int64_t)(t + 1) * lda] = a1;
-            a[(int64_t)(t + 2) * lda] = a2; a[(int64_t)(t + 3) * lda] = a3;
-            acc = __builtin_fma(vj[t], a0, acc);
-            acc = __builtin_fma(vj[t + 1], a1, acc);
-            acc = __builtin_fma(vj[t + 2], a2, acc);
-            acc = __builtin_fma(vj[t + 3], a3, acc);
+        for (; t + 16 <= m; t += 16) {
+            T av[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) av[u] = a[(int64_t)(t + u) * lda];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                av[u] = __builtin_fma(li, rj[t + u], av[u]);
+                a[(int64_t)(t + u) * lda] = av[u];
+                acc = __builtin_fma(vj[t + u], av[u], acc);
+            }
         }
         for (; t < m; t++) {
             T a0 = __builtin_fma(li, rj[t], a[(int64_t)t * lda]);
@@ -338,13 +356,12 @@ __global__ void __launch_bounds__(kRedThreads) k_trailing(T *__restrict__ A, int
             acc = __builtin_fma(vj[t], a0, acc);
         }
     } else {
-        for (; t + 4 <= m; t += 4) {
-            T a0 = a[(int64_t)t * lda], a1 = a[(int64_t)(t + 1) * lda];
-            T a2 = a[(int64_t)(t + 2) * lda], a3 = a[(int64_t)(t + 3) * lda];
-            acc = __builtin_fma(vj[t], a0, acc);
-            acc = __builtin_fma(vj[t + 1], a1, acc);
-            acc = __builtin_fma(vj[t + 2], a2, acc);
-            acc = __builtin_fma(vj[t + 3], a3, acc);
+        for (; t + 16 <= m; t += 16) {
+            T av[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) av[u] = a[(int64_t)(t + u) * lda];
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc = __builtin_fma(vj[t + u], av[u], acc);
         }
         for (; t < m; t++) acc = __builtin_fma(vj[t], a[(int64_t)t * lda], acc);
     }
