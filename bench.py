@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU leg")
     ap.add_argument("--lhl-reps", type=int, default=2)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="MiB of a plain device copy run at the end (the HBM-counter calibration for --pmc runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -303,6 +305,16 @@ def main():
         out["lhl_speedup_vs_reference_published_1thread"] = (cpu["reference_published_lhl_1thread_s_scaled"] * 1e3 /
                                                             lhl_ms["fp64"])
     print(json.dumps(out), flush=True)
+    if args.calib_copy > 0:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+        # against its exact byte count calibrates the counters of the kernels above
+        nb = args.calib_copy * 1024 * 1024
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.calib_copy} MiB ({nb / 1024:.0f} KiB read, {nb / 1024:.0f} KiB written)")
 
 
 if __name__ == "__main__":

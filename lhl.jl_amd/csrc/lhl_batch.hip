@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
     for (int m = 0; m < kT - 1; m++) {
         const T xm = x[m];
 #pragma unroll
-        for (int i = m + 1; i < kT; i++) x[i] = __builtin_fma(-Ls[i][m], xm, x[i]);
+        for (int i = m + 1; i < kT; i++) x[i] = fma_t(-Ls[i][m], xm, x[i]);
     }
     T *dst = Linv + (int64_t)b * kT * kT + (int64_t)c * kT;
 #pragma unroll
@@ -288,11 +288,11 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
         T ar = (i == c) ? (T)1 : (T)0, ai = 0;
 #pragma unroll
         for (int j = i + 1; j < kT; j++) {
-            ar = __builtin_fma(-Us[0][i][j], xr[j], ar);
+            ar = fma_t(-Us[0][i][j], xr[j], ar);
             if constexpr (UC) {
-                ar = __builtin_fma(Us[UP - 1][i][j], xi[j], ar);
-                ai = __builtin_fma(-Us[0][i][j], xi[j], ai);
-                ai = __builtin_fma(-Us[UP - 1][i][j], xr[j], ai);
+                ar = fma_t(Us[UP - 1][i][j], xi[j], ar);
+                ai = fma_t(-Us[0][i][j], xi[j], ai);
+                ai = fma_t(-Us[UP - 1][i][j], xr[j], ai);
             }
         }
         if constexpr (UC) {
@@ -348,21 +348,33 @@ __device__ __forceinline__ T wave_reduce_scatter(T (&v)[V]) {
 
 // x <- U_s^-1 y for kNC internal columns of one shift (two complex columns,
 // re/im interleaved, when UC).  U's rows are packed, so lanes run along a row
-// (each lane a run of four, coalesced) and each wave owns RW rows of the
-// 64-row block; per-lane partial sums are combined by a reduce-scatter.  All
-// loads of a 256-wide chunk are in flight before the first FMA.  Workgroups of
+// (each lane a 16-byte run, coalesced) and each wave owns RW rows of the
+// 64-row block; per-lane partial sums are combined by a reduce-scatter.  The
+// chunk loop keeps two register sets: the loads of chunk c+1 are in flight
+// while chunk c's FMAs run.  The block's right-hand side and diagonal inverse
+// are fetched when the block starts and used after its K loop.  Workgroups of
 // one shift are placed on one XCD by the host so U comes from HBM once per
-// shift.  RW = 16 (4 waves) for FP32, 8 (8 waves) for FP64.
-template <typename T> constexpr int utrsm4_rows() { return sizeof(T) == 4 ? 16 : 8; }
+// shift.  RW = 8 rows per wave, 8 waves; complex shifts (twice the loads per
+// chunk) keep one register set.
+template <typename T> constexpr int utrsm4_rows() { return 8; }
+
+template <typename T> struct V16T;
+template <> struct V16T<float> { typedef float type __attribute__((ext_vector_type(4), aligned(4))); };
+template <> struct V16T<double> { typedef double type __attribute__((ext_vector_type(2), aligned(8))); };
+template <typename T> using V16 = typename V16T<T>::type;   // 16 bytes, element-aligned loads
+template <typename T> __device__ __forceinline__ V16<T> ld16u(const T *p) { return *(const V16<T> *)p; }
 
 template <typename T, int UC>
-__global__ void __launch_bounds__(64 * (64 / utrsm4_rows<T>())) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(64 * (64 / utrsm4_rows<T>()))
 k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restrict__ Uinv, int64_t uistride,
          const UGroup *__restrict__ groups, const T *__restrict__ Yin, T *__restrict__ Yf, int64_t ldy) {
     constexpr int UP = UC ? 2 : 1;
     constexpr int RW = utrsm4_rows<T>();
     constexpr int NV = RW * kNC;           // partial sums per lane
     constexpr int NT = 64 * (64 / RW);     // threads
+    constexpr int RL = 16 / sizeof(T);     // elements per lane per load
+    constexpr int CH = 64 * RL;            // chunk width
+    constexpr int NPRE = kT * kT * UP / NT;   // diagonal-inverse elements per thread
     const UGroup g = groups[blockIdx.x];
     if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
     const int nbu = (n + kT - 1) / kT;
@@ -374,9 +386,23 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
     for (int c = 0; c < kNC; c++) Xc[c] = Yf + (int64_t)(g.q0 + min(c, g.nq - 1)) * ldy;
     __shared__ T Ysum[kT][kNC];
     __shared__ T Ys[kT][kNC];
+    __shared__ T UiS[UP][kT][kT + 1];   // UiS[p][k][i] = Uinv_b(i, k), plane p
+    const int di = tid >> 2, dc = tid & 3;   // the (row, column) of the block's solve this thread owns (tid < 256)
     for (int b = nbu - 1; b >= 0; b--) {
         const int rb = b * kT, nbk = min(kT, n - rb);
         const int kbeg = rb + kT;
+        // the block's own operands, consumed after the K loop
+        T pre[NPRE];
+        {
+            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
+#pragma unroll
+            for (int u = 0; u < NPRE; u++) pre[u] = Ui[tid + NT * u];
+        }
+        T yin = 0;
+        if (tid < kT * kNC) {
+            const T yv = Yin[min(rb + di, n - 1) + (int64_t)(g.q0 + min(dc, g.nq - 1)) * ldy];
+            yin = (di < nbk && dc < g.nq) ? yv : (T)0;
+        }
         T v[NV];
 #pragma unroll
         for (int j = 0; j < NV; j++) v[j] = 0;
@@ -387,49 +413,73 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
                 const int row = rb + RW * w + p;
                 ro[p] = upack_off(n, row) - row;
             }
-            // k-chunks of 256: lane l takes the run m0 + 4l .. m0 + 4l + 3
-            int m0 = kbeg;
-            for (; m0 + 256 <= n; m0 += 256) {
-                const int m = m0 + 4 * lane;
-                V4<T> xv[kNC], urv[RW], uiv[UC ? RW : 1];
+            const int nfull = (n - kbeg) / CH;
+            V16<T> xA[kNC], uA[RW], iA[UC ? RW : 1], xB[kNC], uB[RW], iB[UC ? RW : 1];
+            // chunk c into a register set (c clamped to the last full chunk: such loads are never used)
+            auto load_set = [&](int c, V16<T>(&xs)[kNC], V16<T>(&us)[RW], V16<T>(&is)[UC ? RW : 1]) __attribute__((always_inline)) {
+                const int m = kbeg + min(c, nfull - 1) * CH + RL * lane;
 #pragma unroll
-                for (int c = 0; c < kNC; c++) xv[c] = ld4u(Xc[c] + m);
+                for (int cc = 0; cc < kNC; cc++) xs[cc] = ld16u(Xc[cc] + m);
 #pragma unroll
                 for (int p = 0; p < RW; p++) {
-                    urv[p] = ld4u(Us + ro[p] + m);
-                    if constexpr (UC) uiv[p] = ld4u(Us + up + ro[p] + m);
+                    us[p] = ld16u(Us + ro[p] + m);
+                    if constexpr (UC) is[p] = ld16u(Us + up + ro[p] + m);
                 }
+            };
+            auto fma_set = [&](const V16<T>(&xs)[kNC], const V16<T>(&us)[RW], const V16<T>(&is)[UC ? RW : 1]) __attribute__((always_inline)) {
 #pragma unroll
                 for (int p = 0; p < RW; p++) {
-                    const V4<T> ur = urv[p];
-                    if constexpr (UC) {
-                        const V4<T> ui = uiv[p];
 #pragma unroll
-                        for (int t = 0; t < 4; t++) {
-                            v[4 * p + 0] = __builtin_fma(ur[t], xv[0][t], __builtin_fma(-ui[t], xv[1][t], v[4 * p + 0]));
-                            v[4 * p + 1] = __builtin_fma(ur[t], xv[1][t], __builtin_fma(ui[t], xv[0][t], v[4 * p + 1]));
-                            v[4 * p + 2] = __builtin_fma(ur[t], xv[2][t], __builtin_fma(-ui[t], xv[3][t], v[4 * p + 2]));
-                            v[4 * p + 3] = __builtin_fma(ur[t], xv[3][t], __builtin_fma(ui[t], xv[2][t], v[4 * p + 3]));
+                    for (int t = 0; t < RL; t++) {
+                        const T ur = us[p][t];
+                        if constexpr (UC) {
+                            const T ui = is[p][t];
+                            v[4 * p + 0] = fma_t(ur, xs[0][t], fma_t(-ui, xs[1][t], v[4 * p + 0]));
+                            v[4 * p + 1] = fma_t(ur, xs[1][t], fma_t(ui, xs[0][t], v[4 * p + 1]));
+                            v[4 * p + 2] = fma_t(ur, xs[2][t], fma_t(-ui, xs[3][t], v[4 * p + 2]));
+                            v[4 * p + 3] = fma_t(ur, xs[3][t], fma_t(ui, xs[2][t], v[4 * p + 3]));
+                        } else {
+#pragma unroll
+                            for (int cc = 0; cc < kNC; cc++) v[4 * p + cc] = fma_t(ur, xs[cc][t], v[4 * p + cc]);
                         }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < 4; t++)
-#pragma unroll
-                            for (int c = 0; c < kNC; c++) v[4 * p + c] = __builtin_fma(ur[t], xv[c][t], v[4 * p + c]);
                     }
+                }
+            };
+            if (nfull > 0 && !UC) {
+                load_set(0, xA, uA, iA);
+                load_set(1, xB, uB, iB);
+                int c = 0;
+                // scheduling barriers keep each set's next loads after its FMAs,
+                // so only two sets are ever live
+                for (; c + 2 <= nfull; c += 2) {
+                    fma_set(xA, uA, iA);
+                    __builtin_amdgcn_sched_barrier(0);
+                    load_set(c + 2, xA, uA, iA);
+                    __builtin_amdgcn_sched_barrier(0);
+                    fma_set(xB, uB, iB);
+                    __builtin_amdgcn_sched_barrier(0);
+                    load_set(c + 3, xB, uB, iB);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (c < nfull) fma_set(xA, uA, iA);
+            } else {
+                for (int c = 0; c < nfull; c++) {
+                    load_set(c, xA, uA, iA);
+                    fma_set(xA, uA, iA);
                 }
             }
             // the last partial chunk: clamped addresses, out-of-range terms selected to zero
+            const int m0 = kbeg + nfull * CH;
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int m = m0 + 4 * lane + t;
+            for (int t = 0; t < RL; t++) {
+                const int m = m0 + RL * lane + t;
                 const bool ok = m < n;
                 const int mc = min(m, n - 1);
                 T xv[kNC];
 #pragma unroll
-                for (int c = 0; c < kNC; c++) {
-                    const T x = Xc[c][mc];
-                    xv[c] = ok ? x : (T)0;
+                for (int cc = 0; cc < kNC; cc++) {
+                    const T x = Xc[cc][mc];
+                    xv[cc] = ok ? x : (T)0;
                 }
 #pragma unroll
                 for (int p = 0; p < RW; p++) {
@@ -438,44 +488,45 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
                     if constexpr (UC) {
                         const T u1 = Us[up + ro[p] + mc];
                         const T ui = ok ? u1 : (T)0;
-                        v[4 * p + 0] = __builtin_fma(ur, xv[0], __builtin_fma(-ui, xv[1], v[4 * p + 0]));
-                        v[4 * p + 1] = __builtin_fma(ur, xv[1], __builtin_fma(ui, xv[0], v[4 * p + 1]));
-                        v[4 * p + 2] = __builtin_fma(ur, xv[2], __builtin_fma(-ui, xv[3], v[4 * p + 2]));
-                        v[4 * p + 3] = __builtin_fma(ur, xv[3], __builtin_fma(ui, xv[2], v[4 * p + 3]));
+                        v[4 * p + 0] = fma_t(ur, xv[0], fma_t(-ui, xv[1], v[4 * p + 0]));
+                        v[4 * p + 1] = fma_t(ur, xv[1], fma_t(ui, xv[0], v[4 * p + 1]));
+                        v[4 * p + 2] = fma_t(ur, xv[2], fma_t(-ui, xv[3], v[4 * p + 2]));
+                        v[4 * p + 3] = fma_t(ur, xv[3], fma_t(ui, xv[2], v[4 * p + 3]));
                     } else {
 #pragma unroll
-                        for (int c = 0; c < kNC; c++) v[4 * p + c] = __builtin_fma(ur, xv[c], v[4 * p + c]);
+                        for (int cc = 0; cc < kNC; cc++) v[4 * p + cc] = fma_t(ur, xv[cc], v[4 * p + cc]);
                     }
                 }
             }
         }
         const T tot = wave_reduce_scatter<T, NV>(v);   // lane l < NV: row RW*w + l/4, column l%4
         if (lane < NV) Ysum[RW * w + (lane >> 2)][lane & 3] = tot;
-        __syncthreads();
-        for (int t = tid; t < kT * kNC; t += NT) {
-            const int i = t >> 2, c = t & 3;
-            Ys[i][c] = (i < nbk && c < g.nq) ? Yin[(rb + i) + (int64_t)(g.q0 + c) * ldy] - Ysum[i][c] : (T)0;
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) {
+            const int e = tid + NT * u;   // plane p, column-major (i, k)
+            const int pl = e / (kT * kT), ik = e % (kT * kT);
+            UiS[pl][ik / kT][ik % kT] = pre[u];
         }
         __syncthreads();
+        if (tid < kT * kNC) Ys[di][dc] = yin - Ysum[di][dc];
+        __syncthreads();
         if (tid < kT * kNC) {
-            const int i = tid >> 2, c = tid & 3;
-            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP + i;
             T x = 0;
             if constexpr (UC) {
-                const int cr = c & ~1;   // the complex column's real part
-                const bool im = (c & 1) != 0;
+                const int cr = dc & ~1;   // the complex column's real part
+                const bool im = (dc & 1) != 0;
 #pragma unroll 8
                 for (int k = 0; k < kT; k++) {
-                    const T ar = Ui[k * kT], ai = Ui[kT * kT + k * kT];
+                    const T ar = UiS[0][k][di], ai = UiS[UP - 1][k][di];
                     const T yr = Ys[k][cr], yi = Ys[k][cr + 1];
-                    x = im ? __builtin_fma(ar, yi, __builtin_fma(ai, yr, x))
-                           : __builtin_fma(ar, yr, __builtin_fma(-ai, yi, x));
+                    x = im ? fma_t(ar, yi, fma_t(ai, yr, x))
+                           : fma_t(ar, yr, fma_t(-ai, yi, x));
                 }
             } else {
 #pragma unroll 16
-                for (int k = 0; k < kT; k++) x = __builtin_fma(Ui[k * kT], Ys[k][c], x);
+                for (int k = 0; k < kT; k++) x = fma_t(UiS[0][k][di], Ys[k][dc], x);
             }
-            if (i < nbk && c < g.nq) Yf[(rb + i) + (int64_t)(g.q0 + c) * ldy] = x;
+            if (di < nbk && dc < g.nq) Yf[(rb + di) + (int64_t)(g.q0 + dc) * ldy] = x;
         }
         __syncthreads();
     }

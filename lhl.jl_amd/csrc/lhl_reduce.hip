@@ -345,15 +345,15 @@ __global__ void __launch_bounds__(kRedThreads) k_trailing(T *__restrict__ A, int
             for (int u = 0; u < 16; u++) av[u] = a[(int64_t)(t + u) * lda];
 #pragma unroll
             for (int u = 0; u < 16; u++) {
-                av[u] = __builtin_fma(li, rj[t + u], av[u]);
+                av[u] = fma_t(li, rj[t + u], av[u]);
                 a[(int64_t)(t + u) * lda] = av[u];
-                acc = __builtin_fma(vj[t + u], av[u], acc);
+                acc = fma_t(vj[t + u], av[u], acc);
             }
         }
         for (; t < m; t++) {
-            T a0 = __builtin_fma(li, rj[t], a[(int64_t)t * lda]);
+            T a0 = fma_t(li, rj[t], a[(int64_t)t * lda]);
             a[(int64_t)t * lda] = a0;
-            acc = __builtin_fma(vj[t], a0, acc);
+            acc = fma_t(vj[t], a0, acc);
         }
     } else {
         for (; t + 16 <= m; t += 16) {
@@ -361,9 +361,9 @@ __global__ void __launch_bounds__(kRedThreads) k_trailing(T *__restrict__ A, int
 #pragma unroll
             for (int u = 0; u < 16; u++) av[u] = a[(int64_t)(t + u) * lda];
 #pragma unroll
-            for (int u = 0; u < 16; u++) acc = __builtin_fma(vj[t + u], av[u], acc);
+            for (int u = 0; u < 16; u++) acc = fma_t(vj[t + u], av[u], acc);
         }
-        for (; t < m; t++) acc = __builtin_fma(vj[t], a[(int64_t)t * lda], acc);
+        for (; t < m; t++) acc = fma_t(vj[t], a[(int64_t)t * lda], acc);
     }
     part[i + (int64_t)chunk * ldp] = acc;
 }

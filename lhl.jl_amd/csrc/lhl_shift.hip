@@ -32,8 +32,8 @@ struct ShiftParam {
 // in-bounds addresses), both eliminations are formed and a select keeps the
 // decided one, and stores of lanes with nothing to store go to a per-shift
 // dump area after U.  With no branch between a load and its use, the
-// compiler waits only for the load it needs, so the next row of H streams in
-// while the current step runs.
+// compiler waits only for the load it needs, so the rows of H prefetched
+// three steps ahead stream in while the current steps run.
 template <typename T, int NPER>
 __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
@@ -53,28 +53,32 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
     __shared__ T slot[2];
     __shared__ T ldump[kShiftThreads];
 
-    T r[NPER], h[NPER];
+    T r[NPER];
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
         const int j = tid + m * kShiftThreads;
         r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
     }
     if (tid == 0) r[0] = r[0] + sigma;
-#pragma unroll
-    for (int m = 0; m < NPER; m++) h[m] = Ht[min(tid + m * kShiftThreads, n - 1) + (int64_t)min(1, n - 1) * ldh];
     if (tid == 0) slot[0] = r[0];
-    // b = tau * H[k+1, k] of the next step, loaded one step ahead
-    T bnext = tau * Ht[(int64_t)min(1, n - 1) * ldh];
+    // Rows of H are prefetched three steps ahead into three register sets
+    // used in rotation (the k loop is unrolled by three so no set is copied):
+    // set X holds row `row` of H and the raw H[row, row-1] (b of step row-1).
+    T hA[NPER], hB[NPER], hC[NPER], bA, bB, bC;
+    auto load_row = [&](int row, T(&hd)[NPER], T &bd) {
+        const int64_t col = (int64_t)min(row, n - 1) * ldh;
+#pragma unroll
+        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * kShiftThreads, n - 1) + col];
+        bd = Ht[min(row - 1, n - 1) + col];
+    };
+    load_row(1, hA, bA);
+    load_row(2, hB, bB);
+    load_row(3, hC, bC);
     __syncthreads();
     int info = 0;
-    for (int k = 0; k < n - 1; k++) {
+    auto step = [&](int k, const T(&h)[NPER], T braw) {
         const T a = slot[k & 1];
-        const T b = bnext;
-        bnext = tau * Ht[min(k + 1, n - 1) + (int64_t)min(k + 2, n - 1) * ldh];
-        T hn[NPER];
-        const int64_t coln = (int64_t)min(k + 2, n - 1) * ldh;
-#pragma unroll
-        for (int m = 0; m < NPER; m++) hn[m] = Ht[min(tid + m * kShiftThreads, n - 1) + coln];
+        const T b = tau * braw;
         // the decision (src:2545-2565): swap iff |b| > |a|; l = a/b, b/a, or 0 for a zero pivot
         const bool sw_k = dabs(b) > dabs(a);
         const bool zero = !sw_k && a == (T)0;
@@ -110,10 +114,20 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
             r[m] = active ? rnew : rj;
             *(active ? Urow + j : dump + tid) = u;
             *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
-            h[m] = hn[m];
         }
         lds_barrier();   // the slot is the only data shared between steps
+    };
+    int k = 0;
+    for (; k + 3 <= n - 1; k += 3) {
+        step(k, hA, bA);
+        load_row(k + 4, hA, bA);
+        step(k + 1, hB, bB);
+        load_row(k + 5, hB, bB);
+        step(k + 2, hC, bC);
+        load_row(k + 6, hC, bC);
     }
+    if (k < n - 1) step(k, hA, bA);
+    if (k + 1 < n - 1) step(k + 1, hB, bB);
     if (n >= 1) {
         // the last diagonal: U[n-1, n-1] = r[n-1]
         const int own = (n - 1) % kShiftThreads, mm = (n - 1) / kShiftThreads;
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     __shared__ T slot[2][2];
     __shared__ T ldump[2][kShiftThreads];
 
-    T rr[NPER], ri[NPER], h[NPER];
+    T rr[NPER], ri[NPER];
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
         const int j = tid + m * kShiftThreads;
@@ -203,20 +217,22 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
         ri[m] = ti * hv;
     }
     if (tid == 0) { rr[0] = rr[0] + sr; ri[0] = ri[0] + si; }
-#pragma unroll
-    for (int m = 0; m < NPER; m++) h[m] = Ht[min(tid + m * kShiftThreads, n - 1) + (int64_t)min(1, n - 1) * ldh];
     if (tid == 0) { slot[0][0] = rr[0]; slot[0][1] = ri[0]; }
-    T hnext = Ht[(int64_t)min(1, n - 1) * ldh];
+    // rows of H three steps ahead in three register sets, as in k_shift_real
+    T hA[NPER], hB[NPER], hC[NPER], bA, bB, bC;
+    auto load_row = [&](int row, T(&hd)[NPER], T &bd) {
+        const int64_t col = (int64_t)min(row, n - 1) * ldh;
+#pragma unroll
+        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * kShiftThreads, n - 1) + col];
+        bd = Ht[min(row - 1, n - 1) + col];
+    };
+    load_row(1, hA, bA);
+    load_row(2, hB, bB);
+    load_row(3, hC, bC);
     __syncthreads();
     int info = 0;
-    for (int k = 0; k < n - 1; k++) {
+    auto step = [&](int k, const T(&h)[NPER], T hk) {
         const T ar = slot[k & 1][0], ai = slot[k & 1][1];
-        const T hk = hnext;
-        hnext = Ht[min(k + 1, n - 1) + (int64_t)min(k + 2, n - 1) * ldh];
-        T hn[NPER];
-        const int64_t coln = (int64_t)min(k + 2, n - 1) * ldh;
-#pragma unroll
-        for (int m = 0; m < NPER; m++) hn[m] = Ht[min(tid + m * kShiftThreads, n - 1) + coln];
         const T br = tr * hk, bi = ti * hk;
         const bool sw_k = (dabs(br) + dabs(bi)) > (dabs(ar) + dabs(ai));
         const bool zero = !sw_k && ar == (T)0 && ai == (T)0;
@@ -264,10 +280,20 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
             *(active ? Ui + ro + j : dump + tid) = ui;
             *(first ? &slot[(k + 1) & 1][0] : &ldump[0][tid]) = xr;
             *(first ? &slot[(k + 1) & 1][1] : &ldump[1][tid]) = xi;
-            h[m] = hn[m];
         }
         lds_barrier();   // the slot is the only data shared between steps
+    };
+    int k = 0;
+    for (; k + 3 <= n - 1; k += 3) {
+        step(k, hA, bA);
+        load_row(k + 4, hA, bA);
+        step(k + 1, hB, bB);
+        load_row(k + 5, hB, bB);
+        step(k + 2, hC, bC);
+        load_row(k + 6, hC, bC);
     }
+    if (k < n - 1) step(k, hA, bA);
+    if (k + 1 < n - 1) step(k + 1, hB, bB);
     if (n >= 1) {
         const int own = (n - 1) % kShiftThreads, mm = (n - 1) / kShiftThreads;
         if (tid == own) {

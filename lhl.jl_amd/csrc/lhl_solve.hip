@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_hess(
 #pragma unroll
                     for (int q = 0; q < NV; q++) {
                         const T xc = shfl(v[q], c);
-                        v[q] = __builtin_fma(-u, xc, v[q]);
+                        v[q] = fma_t(-u, xc, v[q]);
                     }
                 }
             }
@@ -215,14 +215,14 @@ __global__ void __launch_bounds__(kSolveThreads) k_hess(
 #pragma unroll
                             for (int rr = 0; rr < NR; rr++) {
                                 const T xr = y(rr * PL, j0 + c), xi = y(rr * PL + 1, j0 + c);
-                                acc[rr * PL] = __builtin_fma(uu[ru][c], xr, acc[rr * PL]);
-                                acc[rr * PL] = __builtin_fma(-ui[ru][c], xi, acc[rr * PL]);
-                                acc[rr * PL + 1] = __builtin_fma(uu[ru][c], xi, acc[rr * PL + 1]);
-                                acc[rr * PL + 1] = __builtin_fma(ui[ru][c], xr, acc[rr * PL + 1]);
+                                acc[rr * PL] = fma_t(uu[ru][c], xr, acc[rr * PL]);
+                                acc[rr * PL] = fma_t(-ui[ru][c], xi, acc[rr * PL]);
+                                acc[rr * PL + 1] = fma_t(uu[ru][c], xi, acc[rr * PL + 1]);
+                                acc[rr * PL + 1] = fma_t(ui[ru][c], xr, acc[rr * PL + 1]);
                             }
                         } else {
 #pragma unroll
-                            for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(uu[ru][c], y(q, j0 + c), acc[q]);
+                            for (int q = 0; q < NV; q++) acc[q] = fma_t(uu[ru][c], y(q, j0 + c), acc[q]);
                         }
                     }
                     if (i < j0)
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_applyz(int n, int inverse, co
                 for (int q = 0; q < NV; q++) v[q] = (lane < nb) ? y(q, row) : (T)0;
                 for (int kk = 0; kk < nb; kk++) {
                     const T lik = (lane > kk && lane < nb) ? A[row + (int64_t)(kb0 + kk) * lda] : (T)0;
-                    for (int q = 0; q < NV; q++) v[q] = __builtin_fma(-lik, shfl(v[q], kk), v[q]);
+                    for (int q = 0; q < NV; q++) v[q] = fma_t(-lik, shfl(v[q], kk), v[q]);
                 }
                 for (int q = 0; q < NV; q++) if (lane < nb) y(q, row) = v[q];
             }
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_applyz(int n, int inverse, co
                 for (int q = 0; q < NV; q++) acc[q] = 0;
                 for (int kk = 0; kk < nb; kk++) {
                     const T l = A[i + (int64_t)(kb0 + kk) * lda];
-                    for (int q = 0; q < NV; q++) acc[q] = __builtin_fma(l, y(q, kb0 + 1 + kk), acc[q]);
+                    for (int q = 0; q < NV; q++) acc[q] = fma_t(l, y(q, kb0 + 1 + kk), acc[q]);
                 }
                 for (int q = 0; q < NV; q++) y(q, i) -= acc[q];
             }
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(kSolveThreads) k_applyz(int n, int inverse, co
                     const int i = r0 + tid + m * kSolveThreads;
                     if (i < r1 && i >= k + 2) {
                         const T l = A[i + (int64_t)k * lda];
-                        for (int q = 0; q < NV; q++) acc[m][q] = __builtin_fma(l, y(q, k + 1), acc[m][q]);
+                        for (int q = 0; q < NV; q++) acc[m][q] = fma_t(l, y(q, k + 1), acc[m][q]);
                     }
                 }
             }
