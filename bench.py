@@ -51,8 +51,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-KERNELS = ["shift", "uinv", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "gemm_jx", "resid_combine", "colnorms",
-           "convert"]
+KERNELS = ["shift", "uinv", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "gemm_jx", "resid_combine",
+           "colnorms", "convert"]
 PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
         "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
 
@@ -83,8 +83,11 @@ def algorithmic(kernel, n, S, ncol, es):
         # every shift's packed U and diagonal-block inverses read once
         return S * tri * es + S * nb * 64 * 64 * es + 2 * col, n * n * ncol, "hbm"
     if kernel == "trmm":
-        # L' read once, the columns read, the scattered FP64 solution read-modify-written
-        return n * n / 2 * es + col + 2 * n * ncol * 8, n * n * ncol, "mfma32" if es == 4 else "mfma64"
+        # (I + L') y': L' read once, the columns read and written
+        return n * n / 2 * es + 2 * col, n * n * ncol, "mfma32" if es == 4 else "mfma64"
+    if kernel == "scatter":
+        # the swept columns read, the FP64 solution read-modify-written
+        return col + 2 * n * ncol * 8, 0, "hbm"
     if kernel == "gemm_jx":
         # J x in FP64 for every column: J (n^2) read once, x read, J x written
         return n * n * 8 + 2 * n * ncol * 8, 2.0 * n * n * ncol, "mfma64"

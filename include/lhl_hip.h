@@ -159,7 +159,7 @@ int lhl_get_stream(lhl_handle *h, void **stream);
  * and the summed duration (ms) of one kernel since the last reset:
  * reduction "balance", "reduce_copy", "reduce_pivot", "reduce_swap",
  * "reduce_trailing", "reduce_finish"; shift "shift", "uinv"; solve "gather",
- * "trsm", "fwd_chain", "utrsm", "trmm", "applyz", "convert"; refinement
+ * "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "applyz", "convert"; refinement
  * "gemm_jx", "resid_combine", "colnorms". */
 int lhl_profile_enable(lhl_handle *h, int on);
 int lhl_profile_reset(lhl_handle *h);

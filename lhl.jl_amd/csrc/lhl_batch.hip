@@ -74,95 +74,23 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
     for (int i = 0; i < kT; i++) dst[i] = x[i];
 }
 
-// Lower triangular product over 64x64 tiles, T = L (lower part, L(i, m) =
-// Lm[(i + loff) + m * ldl]):
-//   SWEEP = 0:  Yo' = L Y'          (L = L'^-1 with its unit diagonal stored)
-//   SWEEP = 1:  y_new' = Y' + L' Y' (strict lower part), scattered to
-//               x[perm[p], col] (+)= y_new[p] * d[perm[p]], p = i + 1; the
-//               first row tile also scatters p = 0.
-// Y' is row p = i + 1 of Y.  A workgroup computes the row tiles I and
-// nbt-1-I of one column tile, so every workgroup runs nbt+1 K steps; the next
-// K step's operands are loaded into registers while the MFMAs consume LDS.
-template <typename T, typename TX, int SWEEP>
-__global__ void __launch_bounds__(256) k_trmm_lower(int n, const T *__restrict__ Lm, int64_t ldl, int loff,
-                                                    const T *__restrict__ Y, int64_t ldy, int ncols,
-                                                    T *__restrict__ Yo, int planes, const int *__restrict__ perm,
-                                                    const double *__restrict__ scale, TX *__restrict__ x, int64_t ldx,
-                                                    int64_t xplane, int accumulate) {
-    const int N1 = n - 1, nbt = (N1 + kT - 1) / kT;
-    const int c0 = blockIdx.y * kT;
-    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    __shared__ T As[kT][kT + kTilePad];   // As[m][i] = L(ri+i, rk+m)
-    __shared__ T Bs[kT][kT + kTilePad];   // Bs[m][c] = Y'(rk+m, c0+c)
-    const int cy = min(c0 + ty, ncols - 1);
-    for (int half = 0; half < 2; half++) {
-        const int I = half == 0 ? (int)blockIdx.x : nbt - 1 - (int)blockIdx.x;
-        if (half == 1 && I <= (int)blockIdx.x) break;
-        const int ri = I * kT, ni = min(kT, N1 - ri);
-        const T *Lrow = Lm + min(ri + tx, N1 - 1) + loff;
-        T ra[16], rb[16];
-        // operands of step K into registers: clamped addresses, out-of-range elements zero
-        auto load = [&](int K) {
-            const int rk = K * kT, nk = min(kT, N1 - rk);
-            const T *Yrow = Y + min(rk + tx, N1 - 1) + 1;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int y = ty + 4 * j;
-                const T av = Lrow[(int64_t)min(rk + y, N1 - 1) * ldl];
-                ra[j] = (tx < ni && y < nk) ? av : (T)0;
-                const T bv = Yrow[(int64_t)min(cy + 4 * j, ncols - 1) * ldy];
-                rb[j] = (tx < nk && c0 + y < ncols) ? bv : (T)0;
-            }
-        };
-        typename Mfma<T>::acc_t acc[2][2];
-        mfma_tile64_zero<T>(acc);
-        load(0);
-        for (int K = 0; K <= I; K++) {
-            __syncthreads();
-            const bool diag = K == I;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int y = ty + 4 * j;
-                const bool keep = !diag || (SWEEP ? y < tx : y <= tx);
-                As[y][tx] = keep ? ra[j] : (T)0;
-                Bs[tx][y] = rb[j];
-            }
-            __syncthreads();
-            if (K < I) load(K + 1);
-            mfma_tile64<T>(As, Bs, kT, acc);
-        }
-        __syncthreads();
-        mfma_tile64_each<T>(acc, [&](int i, int c, T v) { As[i][c] = v; });
-        __syncthreads();
-        for (int t = tid; t < kT * kT; t += 256) {
-            const int xx = t % kT, yy = t / kT;
-            const int q = c0 + yy;
-            if (xx >= ni || q >= ncols) continue;
-            const int p = ri + xx + 1;
-            if constexpr (SWEEP) {
-                const int col = q / planes, pl = q % planes;
-                const T yn = Y[p + (int64_t)q * ldy] + As[xx][yy];
-                const int dst_row = perm[p];
-                TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-                const TX v = (TX)yn * (TX)scale[dst_row];
-                if (accumulate) *dst += v;
-                else *dst = v;
-            } else {
-                Yo[p + (int64_t)q * ldy] = As[xx][yy];
-            }
-        }
-        if constexpr (SWEEP) {
-            if (I == 0 && tid < kT && c0 + tid < ncols) {
-                const int q = c0 + tid;
-                const int col = q / planes, pl = q % planes;
-                const int dst_row = perm[0];
-                TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-                const TX v = (TX)Y[(int64_t)q * ldy] * (TX)scale[dst_row];
-                if (accumulate) *dst += v;
-                else *dst = v;
-            }
-        }
-    }
+// x[perm[p], col] (+)= y_new[p] * d[perm[p]]: the scatter that ends a solve
+// (src:3092-3107), from the swept rows y_new' (rows 1.. of Yn) and row 0 of Y
+// (the sweep leaves y[0] as it is).  Internal column q = col*planes + plane.
+template <typename T, typename TX>
+__global__ void k_scatter(int n, const T *__restrict__ Yn, const T *__restrict__ Y, int64_t ldy, int planes,
+                          const int *__restrict__ perm, const double *__restrict__ scale, TX *__restrict__ x,
+                          int64_t ldx, int64_t xplane, int accumulate) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (p >= n) return;
+    const int col = q / planes, pl = q % planes;
+    const T yn = p == 0 ? Y[(int64_t)q * ldy] : Yn[p + (int64_t)q * ldy];
+    const int dst_row = perm[p];
+    TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
+    const TX v = (TX)yn * (TX)scale[dst_row];
+    if (accumulate) *dst += v;
+    else *dst = v;
 }
 
 // The explicit inverse of L' (n-1 x n-1, unit lower): column j solves
@@ -646,6 +574,66 @@ namespace {
 
 inline int64_t ldy_for(int64_t n) { return std::max<int64_t>(64, ((n + 63) / 64) * 64); }
 
+
+Status blas_handle(lhl_handle *h, rocblas_handle *out) {
+    if (!h->blas) {
+        rocblas_handle b = nullptr;
+        if (rocblas_create_handle(&b) != rocblas_status_success) return {LHL_ERR_HIP, "rocblas_create_handle failed"};
+        if (rocblas_set_stream(b, h->stream) != rocblas_status_success ||
+            rocblas_set_pointer_mode(b, rocblas_pointer_mode_host) != rocblas_status_success) {
+            rocblas_destroy_handle(b);
+            return {LHL_ERR_HIP, "rocblas_set_stream failed"};
+        }
+        h->blas = b;
+    }
+    *out = (rocblas_handle)h->blas;
+    return {};
+}
+
+// JX[:, q] = W X[:, q] for every plane through rocBLAS (same element type for
+// W and X): one GEMM per plane, the output columns interleaved by plane.
+template <typename T>
+Status blas_jx(lhl_handle *h, const T *W, int64_t ldw, const T *X, int64_t ld, int64_t plane, int planes, int ncol,
+               T *JX, int64_t ldq) {
+    rocblas_handle b;
+    LHL_TRY(blas_handle(h, &b));
+    const int n = (int)h->n;
+    const T one = 1, zero = 0;
+    ProfScope ps(h, "gemm_jx");
+    for (int pl = 0; pl < planes; pl++) {
+        rocblas_status st;
+        if constexpr (std::is_same<T, double>::value)
+            st = rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
+                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
+        else
+            st = rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
+                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
+        if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS GEMM of the residual failed"};
+    }
+    return {};
+}
+
+// C = (unit lower part of L) * B for m rows and ncol columns through rocBLAS:
+// the two triangular products of the solve are plain library operations.
+template <typename T>
+Status blas_trmm_unit_lower(lhl_handle *h, const char *name, int m, int ncol, const T *L, int64_t ldl, const T *B,
+                            int64_t ldb, T *C, int64_t ldc) {
+    if (m <= 0 || ncol <= 0) return {};
+    rocblas_handle b;
+    LHL_TRY(blas_handle(h, &b));
+    const T one = 1;
+    ProfScope ps(h, name);
+    rocblas_status st;
+    if constexpr (std::is_same<T, double>::value)
+        st = rocblas_dtrmm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, m,
+                           ncol, &one, L, (int)ldl, B, (int)ldb, C, (int)ldc);
+    else
+        st = rocblas_strmm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, m,
+                           ncol, &one, L, (int)ldl, B, (int)ldb, C, (int)ldc);
+    if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS TRMM of the solve failed"};
+    return {};
+}
+
 template <typename T, typename TX>
 Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int n = (int)h->n;
@@ -689,12 +677,9 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     // y' <- L'^-1 y'  (final rows in Yo)
     const int N1 = n - 1;
     const int nbt = (N1 + kT - 1) / kT;
-    const unsigned ct = (unsigned)((Q + kT - 1) / kT);
     // y' <- L'^-1 y' as a product with the explicit inverse formed by lhl()
-    if (nbt > 0)
-        LHL_LAUNCH(h, "trsm", (k_trmm_lower<T, T, 0>), dim3((unsigned)((nbt + 1) / 2), ct), dim3(256), 0, n,
-                   (const T *)h->Lfull.as<T>(), h->ldlf, 0, (const T *)Yw, ldy, Q, Yo, planes, (const int *)nullptr,
-                   (const double *)nullptr, (T *)nullptr, (int64_t)0, (int64_t)0, 0);
+    LHL_TRY(blas_trmm_unit_lower<T>(h, "trsm", N1, Q, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
+                                    Yo + 1, ldy));
     // forward interchange chain in place on Yo (columns resident in LDS)
     LHL_TRY(hess_device(h, a, Yo, ldy));
     // x <- U^-1 y per shift (solution in Yw)
@@ -708,8 +693,10 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 0>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U, a.ustride, (const T *)a.Uinv,
                    uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
     LHL_HIP_TRY(hipGetLastError());
-    LHL_LAUNCH(h, "trmm", (k_trmm_lower<T, TX, 1>), dim3((unsigned)std::max((nbt + 1) / 2, 1), ct), dim3(256), 0, n,
-               A, h->lda, 1, (const T *)Yw, ldy, Q, (T *)nullptr, planes, (const int *)h->perm.as<int>(),
+    // y_new' = (I + L') y' (Yo is free again), then x = D P y_new
+    LHL_TRY(blas_trmm_unit_lower<T>(h, "trmm", N1, Q, A + 1, h->lda, (const T *)Yw + 1, ldy, Yo + 1, ldy));
+    LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), dim3((unsigned)((n + 255) / 256), (unsigned)Q), dim3(256), 0, n,
+               (const T *)Yo, (const T *)Yw, ldy, planes, (const int *)h->perm.as<int>(),
                (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -730,44 +717,6 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
                    ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
     LHL_HIP_TRY(hipGetLastError());
-    return {};
-}
-
-Status blas_handle(lhl_handle *h, rocblas_handle *out) {
-    if (!h->blas) {
-        rocblas_handle b = nullptr;
-        if (rocblas_create_handle(&b) != rocblas_status_success) return {LHL_ERR_HIP, "rocblas_create_handle failed"};
-        if (rocblas_set_stream(b, h->stream) != rocblas_status_success ||
-            rocblas_set_pointer_mode(b, rocblas_pointer_mode_host) != rocblas_status_success) {
-            rocblas_destroy_handle(b);
-            return {LHL_ERR_HIP, "rocblas_set_stream failed"};
-        }
-        h->blas = b;
-    }
-    *out = (rocblas_handle)h->blas;
-    return {};
-}
-
-// JX[:, q] = W X[:, q] for every plane through rocBLAS (same element type for
-// W and X): one GEMM per plane, the output columns interleaved by plane.
-template <typename T>
-Status blas_jx(lhl_handle *h, const T *W, int64_t ldw, const T *X, int64_t ld, int64_t plane, int planes, int ncol,
-               T *JX, int64_t ldq) {
-    rocblas_handle b;
-    LHL_TRY(blas_handle(h, &b));
-    const int n = (int)h->n;
-    const T one = 1, zero = 0;
-    ProfScope ps(h, "gemm_jx");
-    for (int pl = 0; pl < planes; pl++) {
-        rocblas_status st;
-        if constexpr (std::is_same<T, double>::value)
-            st = rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
-                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
-        else
-            st = rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, n, ncol, n, &one, W, (int)ldw,
-                               X + pl * plane, (int)ld, &zero, JX + pl * ldq, (int)(planes * ldq));
-        if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS GEMM of the residual failed"};
-    }
     return {};
 }
 

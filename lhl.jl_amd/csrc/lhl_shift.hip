@@ -94,26 +94,41 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         *(t0 ? rd + k : dump + tid) = rdk;
         *(t0 ? lv + k : dump + tid) = l;
         *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
-        // both eliminations with the reference's operation order (src:2550-2572)
+        // the elimination with the reference's operation order (src:2550-2572).
+        // a and b are the same in every lane (one LDS slot, one address), so the
+        // decision is read from the first lane and taken as a scalar branch.
         const T ls = l * sigma;
+        if (__builtin_amdgcn_readfirstlane((int)sw_k)) {
 #pragma unroll
-        for (int m = 0; m < NPER; m++) {
-            if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
-            const int j = tid + m * kShiftThreads;
-            const bool active = j > k && j < n;
-            const bool first = j == k + 1;
-            const T rj = r[m];
-            const T g = tau * h[m];
-            const T rs = rj - l * g;               // swap: pending row minus l * (new row)
-            const T rn = g - l * rj;               // no swap: new row minus l * pending row
-            const T us = first ? g + sigma : g;
-            const T r_s = first ? rs - ls : rs;
-            const T r_n = first ? rn + sigma : rn;
-            const T u = sw_k ? us : rj;
-            const T rnew = sw_k ? r_s : r_n;
-            r[m] = active ? rnew : rj;
-            *(active ? Urow + j : dump + tid) = u;
-            *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
+            for (int m = 0; m < NPER; m++) {
+                if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
+                const int j = tid + m * kShiftThreads;
+                const bool active = j > k && j < n;
+                const bool first = j == k + 1;
+                const T rj = r[m];
+                const T g = tau * h[m];
+                const T rs = rj - l * g;           // swap: pending row minus l * (new row)
+                const T u = first ? g + sigma : g;
+                const T rnew = first ? rs - ls : rs;
+                r[m] = active ? rnew : rj;
+                *(active ? Urow + j : dump + tid) = u;
+                *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < NPER; m++) {
+                if ((m + 1) * kShiftThreads <= k + 1) continue;
+                const int j = tid + m * kShiftThreads;
+                const bool active = j > k && j < n;
+                const bool first = j == k + 1;
+                const T rj = r[m];
+                const T g = tau * h[m];
+                const T rn = g - l * rj;           // no swap: new row minus l * pending row
+                const T rnew = first ? rn + sigma : rn;
+                r[m] = active ? rnew : rj;
+                *(active ? Urow + j : dump + tid) = rj;
+                *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
+            }
         }
         lds_barrier();   // the slot is the only data shared between steps
     };
