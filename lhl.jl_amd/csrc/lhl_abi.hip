@@ -92,6 +92,7 @@ Status do_solve_staged(lhl_handle *h, lhl_shift_state *sh, const void *xin, void
     a.rdiag = sh->rdiag.p;
     a.Uinv = sh->Uinv.p;
     a.uistride = 0;
+    a.UT = sh->UT.p;
     a.swp = sh->swp.as<uint8_t>();
     a.ustride = us;
     a.lstride = ls;
@@ -171,7 +172,7 @@ int lhl_destroy(lhl_handle *h) {
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {
-        for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo, &s->Uinv}) b->release();
+        for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo, &s->Uinv, &s->UT}) b->release();
     };
     if (h->own) { drop(h->own); delete h->own; }
     for (lhl_shift_state *s : h->held) s->h = nullptr;   // held states outlive nothing: detach
@@ -241,7 +242,7 @@ int lhl_shift_create(lhl_handle *h, lhl_shift_state **out, int is_complex) {
 
 int lhl_shift_destroy(lhl_shift_state *s) {
     if (!s) return LHL_OK;
-    for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo, &s->Uinv}) b->release();
+    for (DevBuf *b : {&s->U, &s->l, &s->rdiag, &s->swp, &s->dinfo, &s->Uinv, &s->UT}) b->release();
     if (s->h) {
         auto &v = s->h->held;
         v.erase(std::remove(v.begin(), v.end(), s), v.end());
@@ -443,6 +444,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.rdiag = pool.rdiag.p;
         a.Uinv = pool.Uinv.p;
         a.uistride = (((n + 63) / 64) * 64) * 64 * planes;
+        a.UT = pool.UT.p;
         a.swp = pool.swp.as<uint8_t>();
         const int64_t lplane = ((n + 63) / 64) * 64;
         a.ustride = pool.ustride;
@@ -520,7 +522,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
     for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
-                      &pool.params, &pool.Uinv})
+                      &pool.params, &pool.Uinv, &pool.UT})
         b->release();
     if (info) for (int64_t s = 0; s < nshift; s++) info[s] = hinfo[s];
     if (!st.ok()) return fail(h, st);

@@ -460,6 +460,199 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
     }
 }
 
+// ---- U in packed columns -----------------------------------------------------
+// UT[j(j+1)/2 + i] = U[i, j] for i <= j: each 64x64 tile of the upper triangle
+// is read along U's packed rows and written along columns through LDS (both
+// sides contiguous).  One workgroup per tile (blockIdx.x, upper-triangle
+// index) and shift-plane (blockIdx.y = shift * planes + plane).
+__host__ __device__ inline int64_t ucol_off(int64_t j) { return j * (j + 1) / 2; }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U, T *__restrict__ UT, int64_t ustride,
+                                                 int planes) {
+    // tile t of the upper triangle, column-block major: J = the largest with
+    // J(J+1)/2 <= t, I = t - J(J+1)/2 <= J
+    const int t = blockIdx.x;
+    int J = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((J + 1) * (J + 2) / 2 <= t) J++;
+    while (J * (J + 1) / 2 > t) J--;
+    const int I = t - J * (J + 1) / 2;
+    const int s = blockIdx.y / planes, pl = blockIdx.y % planes;
+    const T *Us = U + (int64_t)s * ustride + (int64_t)pl * (ustride / 2);
+    T *Ut = UT + (int64_t)s * ustride + (int64_t)pl * (ustride / 2);
+    __shared__ T tile[kT][kT + 1];
+    const int i0 = I * kT, j0 = J * kT, tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int i = i0 + ty + 4 * r, j = j0 + tx;
+        if (i < n && j < n && j >= i) tile[ty + 4 * r][tx] = Us[upack_off(n, i) - i + j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int j = j0 + ty + 4 * r, i = i0 + tx;
+        if (j < n && i <= j) Ut[ucol_off(j) + i] = tile[tx][ty + 4 * r];
+    }
+}
+
+// x <- U_s^-1 y for kNC internal columns of one shift from U in packed
+// columns, right-looking over 64-row blocks from the bottom.  Thread t keeps
+// the right-hand sides of rows t + 512 j in registers.  Per block: its owner
+// wave publishes the block's rows, the 64x64 diagonal inverse gives x_b, and
+// every thread subtracts U[i, block] x_b from its rows above the block (lanes
+// run down a column of U: contiguous loads, MB of them in flight per thread).
+// Complex shifts: columns alternate (re, im), U is planar.  n <= 512 * NPER.
+template <typename T, int UC, int NPER, int NT>
+__global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTbase, int64_t ustride,
+                                                 const T *__restrict__ Uinv, int64_t uistride,
+                                                 const UGroup *__restrict__ groups, const T *__restrict__ Yin,
+                                                 T *__restrict__ Yf, int64_t ldy) {
+    constexpr int UP = UC ? 2 : 1;
+    constexpr int NPRE = (kT * kT * UP + NT - 1) / NT;
+    constexpr int MB = 16;             // column loads per batch (two planes when UC)
+    constexpr int NB = kT / MB;        // batches per 64-column block
+    const UGroup g = groups[blockIdx.x];
+    if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
+    const int nbu = (n + kT - 1) / kT;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const T *Ut = UTbase + (int64_t)g.shift * ustride;
+    const int64_t up = ustride / 2;
+    __shared__ T Ys[kT][kNC];
+    __shared__ T Xb[kT][kNC];
+    __shared__ T UiS[UP][kT][kT + 1];   // UiS[p][k][i] = Uinv_b(i, k), plane p
+    T yv[NPER][kNC];
+#pragma unroll
+    for (int j = 0; j < NPER; j++) {
+        const int i = tid + NT * j;
+#pragma unroll
+        for (int c = 0; c < kNC; c++) {
+            const T v = Yin[min(i, n - 1) + (int64_t)(g.q0 + min(c, g.nq - 1)) * ldy];
+            yv[j][c] = (i < n && c < g.nq) ? v : (T)0;
+        }
+    }
+    const int di = tid >> 2, dc = tid & 3;
+    for (int b = nbu - 1; b >= 0; b--) {
+        const int rb = b * kT, nbk = min(kT, n - rb);
+        T pre[NPRE];
+        {
+            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
+#pragma unroll
+            for (int u = 0; u < NPRE; u++) pre[u] = Ui[min(tid + NT * u, kT * kT * UP - 1)];
+        }
+        // 1. the block's rows (slot rb / NT of wave (rb % NT) / 64) into LDS
+        const int jb = rb / NT;
+        if (w == (rb % NT) / kT) {
+            T yb[kNC];
+#pragma unroll
+            for (int c = 0; c < kNC; c++) yb[c] = 0;
+#pragma unroll
+            for (int j = 0; j < NPER; j++)
+#pragma unroll
+                for (int c = 0; c < kNC; c++) yb[c] = j == jb ? yv[j][c] : yb[c];
+#pragma unroll
+            for (int c = 0; c < kNC; c++) Ys[lane][c] = lane < nbk ? yb[c] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) {
+            const int e = tid + NT * u;   // plane p, column-major (i, k)
+            const int pl = e / (kT * kT), ik = e % (kT * kT);
+            if (e < kT * kT * UP) UiS[pl][ik / kT][ik % kT] = pre[u];
+        }
+        __syncthreads();
+        // 2. x_b = Uinv_b y_b
+        if (tid < kT * kNC) {
+            T x = 0;
+            if constexpr (UC) {
+                const int cr = dc & ~1;
+                const bool im = (dc & 1) != 0;
+#pragma unroll 8
+                for (int k = 0; k < kT; k++) {
+                    const T ar = UiS[0][k][di], ai = UiS[UP - 1][k][di];
+                    const T yr = Ys[k][cr], yi = Ys[k][cr + 1];
+                    x = im ? fma_t(ar, yi, fma_t(ai, yr, x)) : fma_t(ar, yr, fma_t(-ai, yi, x));
+                }
+            } else {
+#pragma unroll 16
+                for (int k = 0; k < kT; k++) x = fma_t(UiS[0][k][di], Ys[k][dc], x);
+            }
+            Xb[di][dc] = di < nbk ? x : (T)0;
+            if (di < nbk && dc < g.nq) Yf[(rb + di) + (int64_t)(g.q0 + dc) * ldy] = x;
+        }
+        __syncthreads();
+        // 3. rows above the block: y_i -= U[i, rb:rb+nbk] x_b.  The work is a
+        // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
+        // loads of batch t+1 are in flight while batch t's FMAs run.
+        const int nbat = NB * ((rb + NT - 1) / NT);   // slots holding rows < rb (nbat is even)
+        auto load_b = [&](int t, T(&ur)[MB], T(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
+            const int ic = min(tid + NT * (t / NB), n - 1);
+            const int col0 = rb + (t % NB) * MB;
+            // one running offset: column col+1 starts col+1 elements after column
+            // col; past the last column it stays there (those x_b rows are zero)
+            const T *pr = Ut + ucol_off(min(col0, n - 1)) + ic;
+#pragma unroll
+            for (int m = 0; m < MB; m++) {
+                ur[m] = pr[0];
+                if constexpr (UC) ui[m] = pr[up];
+                const int col = col0 + m;
+                pr += col + 1 < n ? col + 1 : 0;
+            }
+        };
+        auto fma_b = [&](int t, const T(&ur)[MB], const T(&ui)[UC ? MB : 1], T(&acc)[kNC]) __attribute__((always_inline)) {
+            const int m0 = (t % NB) * MB;
+#pragma unroll
+            for (int m = 0; m < MB; m++) {
+                // x_b is read from LDS in groups of 8 rows (hoisting all of it would spill)
+                if (m % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+                const T *xb = Xb[m0 + m];   // zero beyond the block
+                if constexpr (UC) {
+                    acc[0] = fma_t(ur[m], xb[0], fma_t(-ui[m], xb[1], acc[0]));
+                    acc[1] = fma_t(ur[m], xb[1], fma_t(ui[m], xb[0], acc[1]));
+                    acc[2] = fma_t(ur[m], xb[2], fma_t(-ui[m], xb[3], acc[2]));
+                    acc[3] = fma_t(ur[m], xb[3], fma_t(ui[m], xb[2], acc[3]));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < kNC; c++) acc[c] = fma_t(ur[m], xb[c], acc[c]);
+                }
+            }
+        };
+        // after the last batch of slot js: y -= acc for its rows above the block
+        auto apply = [&](int t, T(&acc)[kNC]) __attribute__((always_inline)) {
+            if (t % NB != NB - 1) return;
+            const int js = t / NB;
+            // a select on every slot: an `if (j == js)` chain would be folded
+            // into a dynamic index of yv, which would move yv to scratch
+#pragma unroll
+            for (int j = 0; j < NPER; j++) {
+                const bool hit = j == js && tid + NT * j < rb;
+#pragma unroll
+                for (int c = 0; c < kNC; c++) yv[j][c] = hit ? yv[j][c] - acc[c] : yv[j][c];
+            }
+#pragma unroll
+            for (int c = 0; c < kNC; c++) acc[c] = 0;
+        };
+        if (nbat > 0) {
+            T uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
+            T acc[kNC];
+#pragma unroll
+            for (int c = 0; c < kNC; c++) acc[c] = 0;
+            load_b(0, uA, iA);
+            for (int t = 0; t < nbat; t += 2) {
+                __builtin_amdgcn_sched_barrier(0);
+                load_b(t + 1, uB, iB);
+                __builtin_amdgcn_sched_barrier(0);
+                fma_b(t, uA, iA, acc);
+                apply(t, acc);
+                __builtin_amdgcn_sched_barrier(0);
+                load_b(min(t + 2, nbat - 1), uA, iA);   // a repeat of the last batch at the end
+                __builtin_amdgcn_sched_barrier(0);
+                fma_b(t + 1, uB, iB, acc);
+                apply(t + 1, acc);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+}
+
 // ---- the refinement residual: JX = J X (rocBLAS GEMM in one precision, the
 // tiled MFMA GEMM below for FP32 J with FP64 x), then
 // r = b - (sigma x + tau JX) per column with its shift, and column norms.
@@ -686,12 +879,43 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
     const dim3 gu((unsigned)ug.size());
-    if (a.shift_complex)
-        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 1>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U, a.ustride, (const T *)a.Uinv,
-                   uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
-    else
-        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 0>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U, a.ustride, (const T *)a.Uinv,
-                   uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+    if (a.UT && n <= 512 * 16) {
+        // U in packed columns, right-hand sides in registers (512 threads:
+        // measured faster than 1024 on this shape)
+        constexpr int NTR = 512;
+        auto go = [&](auto uc, auto np) -> Status {
+            constexpr int UC = decltype(uc)::value, NP = decltype(np)::value;
+            constexpr int NTK = UC ? 512 : NTR;
+            LHL_LAUNCH(h, "utrsm", (k_utrsm_t<T, UC, NP, NTK>), gu, dim3(NTK), 0, n, (const T *)a.UT, a.ustride,
+                       (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+            return {};
+        };
+        using F = std::integral_constant<int, 0>;
+        using C = std::integral_constant<int, 1>;
+        Status st;
+        int nper = (n + 511) / 512;
+        if (a.shift_complex) {
+            if (nper <= 1) st = go(C{}, std::integral_constant<int, 1>{});
+            else if (nper <= 2) st = go(C{}, std::integral_constant<int, 2>{});
+            else if (nper <= 4) st = go(C{}, std::integral_constant<int, 4>{});
+            else if (nper <= 8) st = go(C{}, std::integral_constant<int, 8>{});
+            else st = go(C{}, std::integral_constant<int, 16>{});
+        } else {
+            nper = (n + NTR - 1) / NTR;
+            if (nper <= 1) st = go(F{}, std::integral_constant<int, 1>{});
+            else if (nper <= 2) st = go(F{}, std::integral_constant<int, 2>{});
+            else if (nper <= 4) st = go(F{}, std::integral_constant<int, 4>{});
+            else if (nper <= 8) st = go(F{}, std::integral_constant<int, 8>{});
+            else st = go(F{}, std::integral_constant<int, 16>{});
+        }
+        LHL_TRY(st);
+    } else if (a.shift_complex) {
+        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 1>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U,
+                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+    } else {
+        LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 0>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U,
+                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+    }
     LHL_HIP_TRY(hipGetLastError());
     // y_new' = (I + L') y' (Yo is free again), then x = D P y_new
     LHL_TRY(blas_trmm_unit_lower<T>(h, "trmm", N1, Q, A + 1, h->lda, (const T *)Yw + 1, ldy, Yo + 1, ldy));
@@ -704,7 +928,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
 
 template <typename T>
 Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride, int64_t nshift,
-                  int cplx, DevBuf &Uinv, int64_t *uistride) {
+                  int cplx, DevBuf &Uinv, int64_t *uistride, DevBuf &UT) {
     const int n = (int)h->n;
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = (int64_t)nbu * kT * kT * (cplx ? 2 : 1);
@@ -716,6 +940,13 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     else
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
                    ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
+    // U in packed columns for the back substitution (n <= 8192)
+    if (n <= 512 * 16) {
+        LHL_TRY(UT.ensure(sizeof(T) * (size_t)ustride * (size_t)nshift));
+        const int planes = cplx ? 2 : 1;
+        LHL_LAUNCH(h, "uinv", k_ut_pack<T>, dim3((unsigned)(nbu * (nbu + 1) / 2), (unsigned)(nshift * planes)),
+                   dim3(256), 0, n, (const T *)U, UT.as<T>(), ustride, planes);
+    }
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -788,10 +1019,11 @@ Status linv_typed(lhl_handle *h) {
 }  // namespace
 
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
-                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride) {
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT) {
     if (h->n == 0 || nshift == 0) return {};
-    return h->dtype == LHL_F64 ? uinv_typed<double>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride)
-                               : uinv_typed<float>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride);
+    return h->dtype == LHL_F64
+               ? uinv_typed<double>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT)
+               : uinv_typed<float>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT);
 }
 
 Status linv_device(lhl_handle *h) {

@@ -136,7 +136,7 @@ struct lhl_shift_state {
     int is_complex = 0;
     int dtype = LHL_F64;
     int planes = 1;
-    lhl::DevBuf U, l, rdiag, swp, dinfo, Uinv;
+    lhl::DevBuf U, l, rdiag, swp, dinfo, Uinv, UT;   // UT: U in packed columns (the back substitution)
     double sigma[2] = {0, 0}, tau[2] = {0, 0};
     int64_t info = 0;
     int loaded = 0;   // a shift has been factored for the current n
@@ -200,7 +200,7 @@ struct ShiftPool {
     int64_t n = 0, count = 0;
     int planes = 1;
     int64_t ustride = 0;   // elements between consecutive U's (per plane stride = ustride/planes)
-    DevBuf U, l, rdiag, swp, dinfo, params, Uinv;
+    DevBuf U, l, rdiag, swp, dinfo, params, Uinv, UT;
 };
 Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const double *sig,
                          const double *tau, int is_complex);
@@ -211,6 +211,7 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
 struct SolveArgs {
     const void *U, *l, *rdiag;
     const void *Uinv = nullptr;   // inverses of U's 64x64 diagonal blocks (per shift, uistride apart)
+    const void *UT = nullptr;     // U in packed columns, ustride apart (nullptr: solve from the packed rows)
     int64_t uistride = 0;
     const uint8_t *swp;
     int64_t ustride = 0;          // per shift, elements (both planes)
@@ -228,10 +229,11 @@ struct SolveArgs {
     int accumulate = 0;
 };
 Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
-// inverses of the 64x64 diagonal blocks of nshift packed U's (after a shift LU);
-// returns the per-shift stride of the result in *uistride
+// after a shift LU: the inverses of the 64x64 diagonal blocks of nshift packed
+// U's (per-shift stride returned in *uistride) and U in packed columns (UT,
+// ustride apart)
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
-                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride);   // lhl_batch.hip
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT);   // lhl_batch.hip
 // the inverses of L's 64x64 diagonal blocks, formed once per reduction
 Status linv_device(lhl_handle *h);                          // lhl_batch.hip
 // the per-shift Hessenberg solves on the internal column layout Y (q = col*planes + pl)

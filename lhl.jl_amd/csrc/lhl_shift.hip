@@ -400,7 +400,7 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
                                   pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>()));
     int64_t uis = 0;
     return uinv_device(h, pool.U.p, pool.ustride, pool.rdiag.p, lplane * pool.planes, nshift, is_complex, pool.Uinv,
-                       &uis);
+                       &uis, pool.UT);
 }
 
 // One shift state (ws.shift or an LHLShift): storage is the pool layout with
@@ -435,7 +435,8 @@ Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t /*nshift*/, cons
     if (!st.ok()) { params.release(); return st; }
     if (n > 0) {
         int64_t uis = 0;
-        st = uinv_device(h, sh->U.p, uplane * planes, sh->rdiag.p, lplane * planes, 1, sh->is_complex, sh->Uinv, &uis);
+        st = uinv_device(h, sh->U.p, uplane * planes, sh->rdiag.p, lplane * planes, 1, sh->is_complex, sh->Uinv, &uis,
+                         sh->UT);
         if (!st.ok()) { params.release(); return st; }
     }
     int info = 0;
