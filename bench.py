@@ -172,6 +172,21 @@ def main():
                 times.append(t1 - t0)
         lhl_ms[name] = 1e3 * min(times)
         log(f"bench: lhl {name} {lhl_ms[name]:.2f} ms")
+    # one more lhl() per precision with every launch timed (outside the timed runs)
+    lhl_kernels = {}
+    for name, ws in (("fp64", ws64), ("fp32", ws32)):
+        lhl_hip.profile_reset(ws)
+        lhl_hip.profile_enable(ws, True)
+        lhl_hip.lhl_(ws, Jd)
+        lhl_hip.sync(ws)
+        lhl_hip.profile_enable(ws, False)
+        lhl_kernels[name] = {}
+        for k in ("balance", "reduce_copy", "reduce_pivot", "reduce_swap", "reduce_trailing", "reduce_finish"):
+            c, t = lhl_hip.profile_read(ws, k)
+            if c:
+                lhl_kernels[name][k] = {"launches": c, "total_ms": t, "avg_us": 1e3 * t / c}
+        log(f"bench: lhl {name} kernels " + ", ".join(f"{k} {v['launches']}x {v['avg_us']:.1f} us"
+                                                    for k, v in lhl_kernels[name].items()))
 
     def step(ws):
         _, info, iters = lhl_hip.shift_solve_batched(ws, sig, tau, Bd, offs, X=Xd, precision="refine64",
@@ -294,6 +309,7 @@ def main():
         "fp64_ms_per_step": fp64["ms"],
         "fp32_refine_over_fp64": fp32["sps"] / fp64["sps"],
         "lhl_ms": lhl_ms,
+        "lhl_kernel_ms": lhl_kernels,
         "max_rel_resid": {"fp32_refine": fp32["rel"], "fp64": fp64["rel"]},
         "refine_rounds": {"fp32": [min(fp32["iters"]), max(fp32["iters"])],
                           "fp64": [min(fp64["iters"]), max(fp64["iters"])]},
