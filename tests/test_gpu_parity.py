@@ -366,3 +366,65 @@ def test_precision_form_refines_to_1e12(lhl_hip):
         for c in range(offs[s], offs[s + 1]):
             r = np.linalg.norm(B[:, c] - W @ X[:, c], np.inf) / np.linalg.norm(B[:, c], np.inf)
             assert r <= 1e-12, (s, c, r, iters)
+
+
+def _rel_resid(W, X, B):
+    R = B - W @ X
+    return float(np.max(np.abs(R).max(axis=0) / np.abs(B).max(axis=0)))
+
+
+@pytest.mark.parametrize("dtype,precision", [(np.float64, "working"), (np.float32, "refine64")])
+def test_config4_complex_and_real_sweep(lhl_hip, dtype, precision):
+    """The batched shift sweep of one real reduction (scaled down from n=4096,
+    256+256 shifts): complex shifts (i*omega, -1) and real shifts (1, -gamma),
+    8 right-hand sides each, A = G/sqrt(n) - 2I."""
+    n, S, R = 512, 32, 8
+    say(f"case: config-4 sweep n={n} {np.dtype(dtype).name} {precision}")
+    rng = np.random.default_rng(404)
+    A = np.asfortranarray(rng.standard_normal((n, n)) / np.sqrt(n) - 2.0 * np.eye(n))
+    ws = lhl_hip.lhl(A, dtype=dtype, shift=np.complex128 if dtype == np.float64 else np.complex64)
+    om = np.logspace(-2, 2, S)
+    offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+    Bc = np.asfortranarray(rng.standard_normal((n, S * R)) + 1j * rng.standard_normal((n, S * R)))
+    Xc, info, iters = lhl_hip.shift_solve_batched(ws, 1j * om, -np.ones(S), Bc, offs, precision=precision,
+                                                  tol=1e-12, max_iter=30)
+    assert np.all(info == 0)
+    bound = 1e-12 if precision == "refine64" else 1e-11
+    for s in range(0, S, 5):
+        W = 1j * om[s] * np.eye(n) - A
+        cols = slice(offs[s], offs[s + 1])
+        assert _rel_resid(W, Xc[:, cols], Bc[:, cols]) <= bound, (s, iters[s])
+    gam = np.logspace(-4, -1, S)
+    Br = np.asfortranarray(rng.standard_normal((n, S * R)))
+    Xr, info, iters = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, Br, offs, precision=precision, tol=1e-12,
+                                                  max_iter=30)
+    assert np.all(info == 0)
+    for s in range(0, S, 5):
+        W = np.eye(n) - gam[s] * A
+        cols = slice(offs[s], offs[s + 1])
+        assert _rel_resid(W, Xr[:, cols], Br[:, cols]) <= bound, (s, iters[s])
+
+
+def test_config5_uneven_batches(lhl_hip):
+    """Independent stiff matrices of uneven n (scaled down from 64 matrices,
+    n in [128, 3000]), each with its own shifts and right-hand-side counts,
+    FP64 and FP32+refine to 1e-12."""
+    rng = np.random.default_rng(505)
+    for m in range(6):
+        n = int(rng.integers(128, 701))
+        S = int(rng.integers(4, 13))
+        counts = rng.integers(1, 9, size=S)
+        offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        say(f"case: config-5 matrix {m} n={n} shifts={S} rhs={int(offs[-1])}")
+        J = stiff(1000 + m, n)
+        gam = np.logspace(-4, -1, S)
+        B = np.asfortranarray(rng.standard_normal((n, int(offs[-1]))))
+        for dtype, precision, bound in ((np.float64, "working", 1e-10), (np.float32, "refine64", 1e-12)):
+            ws = lhl_hip.lhl(J, dtype=dtype)
+            X, info, iters = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs, precision=precision,
+                                                         tol=1e-12, max_iter=30)
+            assert np.all(info == 0)
+            for s in range(S):
+                W = np.eye(n) - gam[s] * J
+                cols = slice(offs[s], offs[s + 1])
+                assert _rel_resid(W, X[:, cols], B[:, cols]) <= bound, (m, n, s, precision, iters[s])
