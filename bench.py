@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU leg")
     ap.add_argument("--lhl-reps", type=int, default=2)
+    ap.add_argument("--config4", type=int, default=0,
+                    help="1 also times the n=4096 sweep of 256 complex (i*omega, -1) + 256 real shifts x 8 RHS")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of a plain device copy run at the end (the HBM-counter calibration for --pmc runs)")
     args = ap.parse_args()
@@ -219,6 +221,52 @@ def main():
                              rel=rel)
         log(f"bench: {name} step {1e3 * dt:.2f} ms, {ncol / dt:.0f} solves/s, max rel resid {rel:.2e}, "
             f"refine iters {int(iters.min())}-{int(iters.max())}")
+
+    # ---- optional: the complex + real shift sweep on one real reduction
+    sweep4 = None
+    if args.config4:
+        S4, R4 = 256, 8
+        rng4 = np.random.default_rng(404)
+        A4 = np.asfortranarray(rng4.standard_normal((n, n)) / np.sqrt(n) - 2.0 * np.eye(n))
+        A4d = torch.from_numpy(A4).to(dev)
+        om = np.logspace(-2, 2, S4)
+        gam4 = np.logspace(-4, -1, S4)
+        offs4 = np.arange(0, S4 * R4 + 1, R4, dtype=np.int64)
+        Bc = torch.from_numpy(np.asfortranarray(rng4.standard_normal((n, S4 * R4)) +
+                                                1j * rng4.standard_normal((n, S4 * R4)))).to(dev)
+        Br = torch.from_numpy(np.asfortranarray(rng4.standard_normal((n, S4 * R4)))).to(dev)
+        Xc, Xr = torch.zeros_like(Bc), torch.zeros_like(Br)
+        sweep4 = {"workload": f"n={n} A=G/sqrt(n)-2I, {S4} complex shifts (i*omega,-1) omega log-spaced [1e-2,1e2] "
+                              f"+ {S4} real shifts (1,-gamma) gamma log-spaced [1e-4,1e-1], {R4} RHS each"}
+        for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
+            ws4 = lhl_hip.LHLWorkspace(n, dt)
+            lhl_hip.lhl_(ws4, A4d)
+            prec = "refine64"   # both precisions refined to the same tolerance, as in the main step
+
+            def sweep():
+                _, ic, itc = lhl_hip.shift_solve_batched(ws4, 1j * om, -np.ones(S4), Bc, offs4, X=Xc, precision=prec,
+                                                         tol=args.tol, max_iter=30)
+                _, ir, itr = lhl_hip.shift_solve_batched(ws4, np.ones(S4), -gam4, Br, offs4, X=Xr, precision=prec,
+                                                         tol=args.tol, max_iter=30)
+                return np.concatenate([itc, itr])
+            sweep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            its = sweep()
+            torch.cuda.synchronize()
+            dt4 = time.perf_counter() - t0
+            # accuracy on the device: r = b - (sigma x + tau A x)
+            sig = torch.from_numpy(np.repeat(1j * om, R4)).to(dev)
+            Rc = Bc - (Xc * sig[None, :] - A4d.to(torch.complex128) @ Xc)
+            relc = (Rc.abs().amax(0) / Bc.abs().amax(0)).max().item()
+            g4 = torch.from_numpy(np.repeat(gam4, R4)).to(dev)
+            Rr = Br - (Xr - (A4d @ Xr) * g4[None, :])
+            relr = (Rr.abs().amax(0) / Br.abs().amax(0)).max().item()
+            sweep4[name] = {"solves_per_s": 2 * S4 * R4 / dt4, "ms": 1e3 * dt4, "max_rel_resid": max(relc, relr),
+                            "refine_rounds": [int(its.min()), int(its.max())]}
+            log(f"bench: config-4 sweep {name}: {1e3 * dt4:.1f} ms, {2 * S4 * R4 / dt4:.0f} solves/s, "
+                f"max rel resid {max(relc, relr):.2e}")
+            del ws4
 
     # ---- roofline of every kernel of the FP32+refine step; the dominant one leads
     steps = max(args.steps, 1)
@@ -319,6 +367,8 @@ def main():
                            for k, v in kernels["fp32"].items()},
         "cpu_baseline": cpu,
     }
+    if sweep4:
+        out["config4_sweep"] = sweep4
     if cpu:
         out["lhl_speedup_vs_cpu"] = cpu["lhl_s"] * 1e3 / lhl_ms["fp64"]
         out["lhl_speedup_vs_reference_published_1thread"] = (cpu["reference_published_lhl_1thread_s_scaled"] * 1e3 /
