@@ -17,7 +17,7 @@
 namespace lhl {
 namespace {
 
-constexpr int kShiftThreads = 1024;   // 16 waves: few pending-row elements per thread per step
+constexpr int kShiftThreads = 1024;  // the most threads per shift (the spare slots per U plane)
 
 template <typename T> __device__ __forceinline__ T dabs(T x) { return x < 0 ? -x : x; }
 
@@ -34,8 +34,8 @@ struct ShiftParam {
 // dump area after U.  With no branch between a load and its use, the
 // compiler waits only for the load it needs, so the rows of H prefetched
 // three steps ahead stream in while the current steps run.
-template <typename T, int NPER>
-__global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restrict__ Ht, int64_t ldh, int n,
+template <typename T, int NPER, int NT>
+__global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
                                                               T *__restrict__ Ubase, int64_t ustride,
                                                               T *__restrict__ lbase, T *__restrict__ rdbase,
@@ -45,40 +45,43 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
     const int s = blockIdx.x;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
-    T *dump = U + ustride - kShiftThreads;        // kShiftThreads spare elements per shift
+    T *dump = U + ustride - NT;        // NT spare elements per shift
     T *lv = lbase + (int64_t)s * lstride;
     T *rd = rdbase + (int64_t)s * lstride;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x;
     __shared__ T slot[2];
-    __shared__ T ldump[kShiftThreads];
+    __shared__ T ldump[NT];
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    T *subd = reinterpret_cast<T *>(lds_dyn);     // subd[k] = H[k+1, k], read by every step
+    for (int i = tid; i + 1 < n; i += NT) subd[i] = Ht[i + (int64_t)(i + 1) * ldh];
 
     T r[NPER];
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
-        const int j = tid + m * kShiftThreads;
+        const int j = tid + m * NT;
         r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
     }
     if (tid == 0) r[0] = r[0] + sigma;
     if (tid == 0) slot[0] = r[0];
     // Rows of H are prefetched three steps ahead into three register sets
-    // used in rotation (the k loop is unrolled by three so no set is copied):
-    // set X holds row `row` of H and the raw H[row, row-1] (b of step row-1).
-    T hA[NPER], hB[NPER], hC[NPER], bA, bB, bC;
-    auto load_row = [&](int row, T(&hd)[NPER], T &bd) {
+    // used in rotation (the k loop is unrolled by three so no set is copied).
+    // The subdiagonal comes from LDS: a scalar-memory prefetch of it would be
+    // waited for by every step's LDS barrier (they share one counter).
+    T hA[NPER], hB[NPER], hC[NPER];
+    auto load_row = [&](int row, T(&hd)[NPER]) {
         const int64_t col = (int64_t)min(row, n - 1) * ldh;
 #pragma unroll
-        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * kShiftThreads, n - 1) + col];
-        bd = Ht[min(row - 1, n - 1) + col];
+        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * NT, n - 1) + col];
     };
-    load_row(1, hA, bA);
-    load_row(2, hB, bB);
-    load_row(3, hC, bC);
+    load_row(1, hA);
+    load_row(2, hB);
+    load_row(3, hC);
     __syncthreads();
     int info = 0;
-    auto step = [&](int k, const T(&h)[NPER], T braw) {
+    auto step = [&](int k, const T(&h)[NPER]) {
         const T a = slot[k & 1];
-        const T b = tau * braw;
+        const T b = tau * subd[k];
         // the decision (src:2545-2565): swap iff |b| > |a|; l = a/b, b/a, or 0 for a zero pivot
         const bool sw_k = dabs(b) > dabs(a);
         const bool zero = !sw_k && a == (T)0;
@@ -87,13 +90,15 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         const T l = zero ? (T)0 : q;
         info = (zero && info == 0) ? k + 1 : info;
         const T ukk = sw_k ? b : a;
-        const T rdk = (T)1 / ukk;
         T *Urow = U + upack_off(n, k) - k;     // Urow[j] = U[k, j]
-        const bool t0 = tid == 0;
-        *(t0 ? Urow + k : dump + tid) = ukk;
-        *(t0 ? rd + k : dump + tid) = rdk;
-        *(t0 ? lv + k : dump + tid) = l;
-        *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        if (tid < kWave) {   // wave 0 alone: row k's diagonal, reciprocal pivot, multiplier, swap flag
+            const T rdk = (T)1 / ukk;
+            const bool t0 = tid == 0;
+            *(t0 ? Urow + k : dump + tid) = ukk;
+            *(t0 ? rd + k : dump + tid) = rdk;
+            *(t0 ? lv + k : dump + tid) = l;
+            *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        }
         // the elimination with the reference's operation order (src:2550-2572).
         // a and b are the same in every lane (one LDS slot, one address), so the
         // decision is read from the first lane and taken as a scalar branch.
@@ -101,8 +106,8 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         if (__builtin_amdgcn_readfirstlane((int)sw_k)) {
 #pragma unroll
             for (int m = 0; m < NPER; m++) {
-                if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
-                const int j = tid + m * kShiftThreads;
+                if ((m + 1) * NT <= k + 1) continue;   // chunk finished for every thread (uniform)
+                const int j = tid + m * NT;
                 const bool active = j > k && j < n;
                 const bool first = j == k + 1;
                 const T rj = r[m];
@@ -117,8 +122,8 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
         } else {
 #pragma unroll
             for (int m = 0; m < NPER; m++) {
-                if ((m + 1) * kShiftThreads <= k + 1) continue;
-                const int j = tid + m * kShiftThreads;
+                if ((m + 1) * NT <= k + 1) continue;
+                const int j = tid + m * NT;
                 const bool active = j > k && j < n;
                 const bool first = j == k + 1;
                 const T rj = r[m];
@@ -134,18 +139,18 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_real(const T *__restric
     };
     int k = 0;
     for (; k + 3 <= n - 1; k += 3) {
-        step(k, hA, bA);
-        load_row(k + 4, hA, bA);
-        step(k + 1, hB, bB);
-        load_row(k + 5, hB, bB);
-        step(k + 2, hC, bC);
-        load_row(k + 6, hC, bC);
+        step(k, hA);
+        load_row(k + 4, hA);
+        step(k + 1, hB);
+        load_row(k + 5, hB);
+        step(k + 2, hC);
+        load_row(k + 6, hC);
     }
-    if (k < n - 1) step(k, hA, bA);
-    if (k + 1 < n - 1) step(k + 1, hB, bB);
+    if (k < n - 1) step(k, hA);
+    if (k + 1 < n - 1) step(k + 1, hB);
     if (n >= 1) {
         // the last diagonal: U[n-1, n-1] = r[n-1]
-        const int own = (n - 1) % kShiftThreads, mm = (n - 1) / kShiftThreads;
+        const int own = (n - 1) % NT, mm = (n - 1) / NT;
         if (tid == own) {
             T rn = 0;
 #pragma unroll
@@ -203,8 +208,8 @@ __device__ __forceinline__ void cdiv_sel(T a, T b, T c, T d, T &p, T &q) {
     q = sw ? -qq : qq;
 }
 
-template <typename T, int NPER>
-__global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restrict__ Ht, int64_t ldh, int n,
+template <typename T, int NPER, int NT>
+__global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
                                                               T *__restrict__ Ubase, int64_t ustride,
                                                               T *__restrict__ lbase, T *__restrict__ rdbase,
@@ -215,38 +220,42 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     const T sr = prm[s].sr, si = prm[s].si, tr = prm[s].tr, ti = prm[s].ti;
     const int64_t up = ustride / 2, lp = lstride / 2;
     T *Ur = Ubase + (int64_t)s * ustride, *Ui = Ur + up;
-    T *dump = Ur + ustride - kShiftThreads;      // spare tail of the imaginary plane
+    T *dump = Ur + ustride - NT;      // spare tail of the imaginary plane
     T *lr_ = lbase + (int64_t)s * lstride, *li_ = lr_ + lp;
     T *rdr = rdbase + (int64_t)s * lstride, *rdi = rdr + lp;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x;
     __shared__ T slot[2][2];
-    __shared__ T ldump[2][kShiftThreads];
+    __shared__ T ldump[2][NT];
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    T *subd = reinterpret_cast<T *>(lds_dyn);     // subd[k] = H[k+1, k]
+    for (int i = tid; i + 1 < n; i += NT) subd[i] = Ht[i + (int64_t)(i + 1) * ldh];
 
     T rr[NPER], ri[NPER];
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
-        const int j = tid + m * kShiftThreads;
+        const int j = tid + m * NT;
         const T hv = (j < n) ? Ht[j] : (T)0;
         rr[m] = tr * hv;
         ri[m] = ti * hv;
     }
     if (tid == 0) { rr[0] = rr[0] + sr; ri[0] = ri[0] + si; }
     if (tid == 0) { slot[0][0] = rr[0]; slot[0][1] = ri[0]; }
-    // rows of H three steps ahead in three register sets, as in k_shift_real
-    T hA[NPER], hB[NPER], hC[NPER], bA, bB, bC;
-    auto load_row = [&](int row, T(&hd)[NPER], T &bd) {
+    // rows of H three steps ahead in three register sets, the subdiagonal in
+    // LDS, as in k_shift_real
+    T hA[NPER], hB[NPER], hC[NPER];
+    auto load_row = [&](int row, T(&hd)[NPER]) {
         const int64_t col = (int64_t)min(row, n - 1) * ldh;
 #pragma unroll
-        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * kShiftThreads, n - 1) + col];
-        bd = Ht[min(row - 1, n - 1) + col];
+        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * NT, n - 1) + col];
     };
-    load_row(1, hA, bA);
-    load_row(2, hB, bB);
-    load_row(3, hC, bC);
+    load_row(1, hA);
+    load_row(2, hB);
+    load_row(3, hC);
     __syncthreads();
     int info = 0;
-    auto step = [&](int k, const T(&h)[NPER], T hk) {
+    auto step = [&](int k, const T(&h)[NPER]) {
+        const T hk = subd[k];
         const T ar = slot[k & 1][0], ai = slot[k & 1][1];
         const T br = tr * hk, bi = ti * hk;
         const bool sw_k = (dabs(br) + dabs(bi)) > (dabs(ar) + dabs(ai));
@@ -257,22 +266,24 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
         info = (zero && info == 0) ? k + 1 : info;
         const int64_t ro = upack_off(n, k) - k;
         const T ukr = sw_k ? br : ar, uki = sw_k ? bi : ai;
-        T dr, di;
-        cdiv_sel((T)1, (T)0, ukr, uki, dr, di);
-        const bool t0 = tid == 0;
-        *(t0 ? Ur + ro + k : dump + tid) = ukr;
-        *(t0 ? Ui + ro + k : dump + tid) = uki;
-        *(t0 ? rdr + k : dump + tid) = dr;
-        *(t0 ? rdi + k : dump + tid) = di;
-        *(t0 ? lr_ + k : dump + tid) = lre;
-        *(t0 ? li_ + k : dump + tid) = lim;
-        *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        if (tid < kWave) {   // wave 0 alone: row k's diagonal, reciprocal pivot, multiplier, swap flag
+            T dr, di;
+            cdiv_sel((T)1, (T)0, ukr, uki, dr, di);
+            const bool t0 = tid == 0;
+            *(t0 ? Ur + ro + k : dump + tid) = ukr;
+            *(t0 ? Ui + ro + k : dump + tid) = uki;
+            *(t0 ? rdr + k : dump + tid) = dr;
+            *(t0 ? rdi + k : dump + tid) = di;
+            *(t0 ? lr_ + k : dump + tid) = lre;
+            *(t0 ? li_ + k : dump + tid) = lim;
+            *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        }
         // l * sigma for the swap branch's first element
         const T qsr = lre * sr - lim * si, qsi = lre * si + lim * sr;
 #pragma unroll
         for (int m = 0; m < NPER; m++) {
-            if ((m + 1) * kShiftThreads <= k + 1) continue;   // chunk finished for every thread (uniform)
-            const int j = tid + m * kShiftThreads;
+            if ((m + 1) * NT <= k + 1) continue;   // chunk finished for every thread (uniform)
+            const int j = tid + m * NT;
             const bool active = j > k && j < n;
             const bool first = j == k + 1;
             const T xr0 = rr[m], xi0 = ri[m];
@@ -300,17 +311,17 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     };
     int k = 0;
     for (; k + 3 <= n - 1; k += 3) {
-        step(k, hA, bA);
-        load_row(k + 4, hA, bA);
-        step(k + 1, hB, bB);
-        load_row(k + 5, hB, bB);
-        step(k + 2, hC, bC);
-        load_row(k + 6, hC, bC);
+        step(k, hA);
+        load_row(k + 4, hA);
+        step(k + 1, hB);
+        load_row(k + 5, hB);
+        step(k + 2, hC);
+        load_row(k + 6, hC);
     }
-    if (k < n - 1) step(k, hA, bA);
-    if (k + 1 < n - 1) step(k + 1, hB, bB);
+    if (k < n - 1) step(k, hA);
+    if (k + 1 < n - 1) step(k + 1, hB);
     if (n >= 1) {
-        const int own = (n - 1) % kShiftThreads, mm = (n - 1) / kShiftThreads;
+        const int own = (n - 1) % NT, mm = (n - 1) / NT;
         if (tid == own) {
             T xr = 0, xi = 0;
 #pragma unroll
@@ -327,28 +338,38 @@ __global__ void __launch_bounds__(kShiftThreads) k_shift_cplx(const T *__restric
     }
 }
 
-template <typename T, int NPER>
-void launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
-                  const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
-                  int64_t ss, int *info) {
-    if (cplx)
-        LHL_LAUNCH(h, "shift", (k_shift_cplx<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, Ht, ldh, n,
-                   prm, U, us, l, rd, ls, sw, ss, info);
-    else
-        LHL_LAUNCH(h, "shift", (k_shift_real<T, NPER>), dim3((unsigned)nshift), dim3(kShiftThreads), 0, Ht, ldh, n,
-                   prm, U, us, l, rd, ls, sw, ss, info);
+template <typename T, int NPER, int NT>
+Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
+                    const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss,
+                    int *info) {
+    const size_t lds = sizeof(T) * (size_t)std::max(n, 1);   // the subdiagonal of H
+    if (cplx) {
+        auto fn = k_shift_cplx<T, NPER, NT>;
+        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
+                   info);
+    } else {
+        auto fn = k_shift_real<T, NPER, NT>;
+        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
+                   info);
+    }
+    return {};
 }
 
+// 512 threads up to n = 8192 (measured best at n = 4096), 1024 beyond.
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
-    const int nper = (n + kShiftThreads - 1) / kShiftThreads;
-    if (nper <= 1) launch_shift<T, 1>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 2) launch_shift<T, 2>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 4) launch_shift<T, 4>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 8) launch_shift<T, 8>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
-    else if (nper <= 16) launch_shift<T, 16>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info);
+    if ((size_t)n * sizeof(T) + 16 * 1024 > lds_limit())
+        return {LHL_ERR_ARG, "lhl_shift: n too large for the LDS-resident subdiagonal of the shift kernel"};
+    if (n <= 512) LHL_TRY((launch_shift<T, 1, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    else if (n <= 1024) LHL_TRY((launch_shift<T, 2, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    else if (n <= 2048) LHL_TRY((launch_shift<T, 4, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    else if (n <= 4096) LHL_TRY((launch_shift<T, 8, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    else if (n <= 8192) LHL_TRY((launch_shift<T, 16, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    else if (n <= 16384) LHL_TRY((launch_shift<T, 16, 1024>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
     else return {LHL_ERR_ARG, "lhl_shift: n above 16384 is not supported by the register-resident shift kernel"};
     LHL_HIP_TRY(hipGetLastError());
     return {};
