@@ -160,9 +160,8 @@ __global__ void k_copy_scale(const TJ *__restrict__ J, int64_t ldj, T *__restric
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_t lda, int n, int k,
                                                        const T *__restrict__ part, int64_t ldp,
-                                                       RedCtl<T> *ctl, T *cand_v, int *cand_i, T *cand_x,
-                                                       T *vk1, unsigned *counter, int *ipiv, int *perm,
-                                                       int final_only) {
+                                                       const RedCtl<T> *ctl, T *cand_v, int *cand_i, T *cand_x,
+                                                       T *cand_d, T *vk1, int final_only) {
     const RedCtl<T> prev = ctl[(k + 1) & 1];
     const int add = (k >= 1) && !prev.skip;
     const int nch = prev.nchunk;
@@ -187,7 +186,7 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
             col[i] = v;
         }
         if (!final_only) {
-            if (i == k + 1) st_coherent(vk1, v);
+            if (i == k + 1) *vk1 = v;   // A[k+1, k] before k_swap touches it
             if (i >= k + 1) {
                 T a = dabs(v);
                 if (a == a && a > bv) { bv = a; bi = i; bx = v; }   // rows ascend per thread
@@ -198,7 +197,6 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
     // block argmax: larger value wins, ties to the smaller row
     __shared__ T sv[kRedThreads], sx[kRedThreads];
     __shared__ int si[kRedThreads];
-    __shared__ int last;
     sv[threadIdx.x] = bv; si[threadIdx.x] = bi; sx[threadIdx.x] = bx;
     __syncthreads();
     for (int o = kRedThreads / 2; o > 0; o >>= 1) {
@@ -212,48 +210,12 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        // the block's candidate and its row's diagonal (k_swap overwrites
+        // A[p, p] while it decides, so the decision reads this copy)
         cand_v[blockIdx.x] = sv[0]; cand_i[blockIdx.x] = si[0]; cand_x[blockIdx.x] = sx[0];
-        __threadfence();
-        last = (atomicAdd(counter, 1u) == gridDim.x - 1);
+        const int ci = si[0] == INT_MAX ? k + 1 : si[0];
+        cand_d[blockIdx.x] = A[ci + (int64_t)ci * lda];
     }
-    __syncthreads();
-    if (!last || threadIdx.x >= 64) return;
-    __threadfence();
-    // one wave: lane b reads block b's candidate (gridDim.x <= 64), then a
-    // butterfly argmax with the same rule (larger value, ties to the smaller row)
-    const int lane = threadIdx.x;
-    T gv = (T)-1, gx = 0;
-    int gi = INT_MAX;
-    if (lane < (int)gridDim.x) {
-        gv = ld_coherent(&cand_v[lane]);
-        gi = ld_coherent(&cand_i[lane]);
-        gx = ld_coherent(&cand_x[lane]);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const T v2 = __shfl_xor(gv, o, 64);
-        const int i2 = __shfl_xor(gi, o, 64);
-        const T x2 = __shfl_xor(gx, o, 64);
-        if (v2 > gv || (v2 == gv && i2 < gi)) { gv = v2; gi = i2; gx = x2; }
-    }
-    if (lane != 0) return;
-    const T a1 = ld_coherent(vk1);
-    int p = k + 1;
-    T piv = a1;
-    if (a1 == a1 && gi != INT_MAX) {   // a NaN at row k+1 keeps p = k+1 (src:470-477)
-        p = gi;
-        piv = gx;
-    }
-    RedCtl<T> c;
-    c.p = p;
-    c.piv = piv;
-    c.pk = (p != k + 1) ? A[p + (int64_t)p * lda] : A[(k + 1) + (int64_t)(k + 1) * lda];
-    c.skip = (piv == (T)0);
-    c.nchunk = (n - k - 2 + kChunk - 1) / kChunk;
-    c.pad = 0;
-    ctl[k & 1] = c;
-    ipiv[k] = p;
-    int t = perm[k + 1]; perm[k + 1] = perm[p]; perm[p] = t;
-    *counter = 0;
 }
 
 // Interchange rows and columns k+1 <-> p, scale the multipliers by 1/piv and
@@ -262,9 +224,58 @@ __global__ void __launch_bounds__(kRedThreads) k_pivot(T *__restrict__ A, int64_
 // and p; the thread of row k+1 also does row p, so no two threads touch one
 // element.
 template <typename T>
-__global__ void k_swap(T *__restrict__ A, int64_t lda, int n, int k, const RedCtl<T> *__restrict__ ctl) {
+__global__ void __launch_bounds__(256) k_swap(T *__restrict__ A, int64_t lda, int n, int k, RedCtl<T> *__restrict__ ctl,
+                                              const T *__restrict__ cand_v, const int *__restrict__ cand_i,
+                                              const T *__restrict__ cand_x, const T *__restrict__ cand_d,
+                                              const T *__restrict__ vk1, int ncand, int *__restrict__ ipiv,
+                                              int *__restrict__ perm) {
 #pragma clang fp contract(off)
-    const RedCtl<T> c = ctl[k & 1];
+    // The pivot decision, in every workgroup from k_pivot's candidates: larger
+    // |value| wins, ties to the smaller row; a NaN at row k+1 keeps p = k+1
+    // (src:470-477).  Workgroup 0 publishes it for the trailing update.
+    __shared__ RedCtl<T> sc;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        T gv = (T)-1, gx = 0, gd = 0;
+        int gi = INT_MAX;
+        if (lane < ncand) {
+            gv = cand_v[lane];
+            gi = cand_i[lane];
+            gx = cand_x[lane];
+            gd = cand_d[lane];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const T v2 = __shfl_xor(gv, o, 64);
+            const int i2 = __shfl_xor(gi, o, 64);
+            const T x2 = __shfl_xor(gx, o, 64);
+            const T d2 = __shfl_xor(gd, o, 64);
+            if (v2 > gv || (v2 == gv && i2 < gi)) { gv = v2; gi = i2; gx = x2; gd = d2; }
+        }
+        if (lane == 0) {
+            const T a1 = *vk1;
+            int p = k + 1;
+            T piv = a1;
+            if (a1 == a1 && gi != INT_MAX) {
+                p = gi;
+                piv = gx;
+            }
+            RedCtl<T> c;
+            c.p = p;
+            c.piv = piv;
+            c.pk = (p != k + 1) ? gd : A[(k + 1) + (int64_t)(k + 1) * lda];   // no swap: A[k+1, k+1] is untouched
+            c.skip = (piv == (T)0);
+            c.nchunk = (n - k - 2 + kChunk - 1) / kChunk;
+            c.pad = 0;
+            sc = c;
+            if (blockIdx.x == 0) {
+                ctl[k & 1] = c;
+                ipiv[k] = p;
+                const int t = perm[k + 1]; perm[k + 1] = perm[p]; perm[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    const RedCtl<T> c = sc;
     const int p = c.p;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -440,35 +451,32 @@ Status steps_typed(lhl_handle *h) {
     const int64_t ldp = pad_ld(n, h->dtype);
     const int maxchunk = std::max(1, (n + kChunk - 1) / kChunk);
     LHL_TRY(h->part.ensure(sizeof(T) * (size_t)ldp * maxchunk));
-    LHL_TRY(h->cand.ensure((sizeof(T) * 2 + sizeof(int)) * kRedMaxBlocks + 64));
+    LHL_TRY(h->cand.ensure((sizeof(T) * 3 + sizeof(int)) * kRedMaxBlocks + 64));
     LHL_TRY(h->ctl.ensure(sizeof(RedCtl<T>) * 2 + 64));
-    LHL_TRY(h->counter.ensure(64));
     LHL_HIP_TRY(hipMemsetAsync(h->ctl.p, 0, sizeof(RedCtl<T>) * 2 + 64, h->stream));
-    LHL_HIP_TRY(hipMemsetAsync(h->counter.p, 0, 64, h->stream));
     char *cb = h->cand.as<char>();
     T *cand_v = reinterpret_cast<T *>(cb);
     T *cand_x = cand_v + kRedMaxBlocks;
-    int *cand_i = reinterpret_cast<int *>(cand_x + kRedMaxBlocks);
+    T *cand_d = cand_x + kRedMaxBlocks;
+    int *cand_i = reinterpret_cast<int *>(cand_d + kRedMaxBlocks);
     RedCtl<T> *ctl = h->ctl.as<RedCtl<T>>();
     T *vk1 = reinterpret_cast<T *>(ctl + 2);
-    unsigned *counter = h->counter.as<unsigned>();
     int *ipiv = h->ipiv.as<int>(), *perm = h->perm.as<int>();
     const int pblocks = std::max(1, std::min(kRedMaxBlocks, (n + kRedThreads - 1) / kRedThreads));
     const int rowblocks = (n + kRedThreads - 1) / kRedThreads;
     for (int k = 0; k + 2 < n; k++) {
         LHL_LAUNCH(h, "reduce_pivot", k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, A, lda, n, k,
-                           (const T *)h->part.as<T>(), ldp, ctl, cand_v, cand_i, cand_x, vk1, counter, ipiv,
-                           perm, 0);
-        LHL_LAUNCH(h, "reduce_swap", k_swap<T>, dim3((n + 255) / 256), dim3(256), 0, A, lda, n, k,
-                           (const RedCtl<T> *)ctl);
+                   (const T *)h->part.as<T>(), ldp, (const RedCtl<T> *)ctl, cand_v, cand_i, cand_x, cand_d, vk1, 0);
+        LHL_LAUNCH(h, "reduce_swap", k_swap<T>, dim3((n + 255) / 256), dim3(256), 0, A, lda, n, k, ctl,
+                   (const T *)cand_v, (const int *)cand_i, (const T *)cand_x, (const T *)cand_d, (const T *)vk1,
+                   pblocks, ipiv, perm);
         const int nch = (n - k - 2 + kChunk - 1) / kChunk;
         LHL_LAUNCH(h, "reduce_trailing", k_trailing<T>, dim3(rowblocks, nch), dim3(kRedThreads), 0, A, lda, n,
                            k, (const RedCtl<T> *)ctl, h->part.as<T>(), ldp);
     }
     if (n >= 3)
         LHL_LAUNCH(h, "reduce_pivot", k_pivot<T>, dim3(pblocks), dim3(kRedThreads), 0, A, lda, n, n - 2,
-                           (const T *)h->part.as<T>(), ldp, ctl, cand_v, cand_i, cand_x, vk1, counter, ipiv,
-                           perm, 1);
+                   (const T *)h->part.as<T>(), ldp, (const RedCtl<T> *)ctl, cand_v, cand_i, cand_x, cand_d, vk1, 1);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
