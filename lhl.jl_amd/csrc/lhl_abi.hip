@@ -168,7 +168,7 @@ int lhl_destroy(lhl_handle *h) {
     prof_drain(h);
     blas_release(h);
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
-                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->Linv, &h->Lfull, &h->ybuf, &h->rbuf, &h->nbuf, &h->gbuf,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->Linv, &h->Lfull, &h->ybuf, &h->rbuf, &h->nbuf, &h->gbuf,
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {

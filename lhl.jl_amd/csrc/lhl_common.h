@@ -155,6 +155,7 @@ struct lhl_handle {
     int reduced = 0;
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
+    lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
     int64_t ldlf = 0;

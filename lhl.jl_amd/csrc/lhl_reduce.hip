@@ -379,6 +379,321 @@ __global__ void __launch_bounds__(kRedThreads) k_trailing(T *__restrict__ A, int
     part[i + (int64_t)chunk * ldp] = acc;
 }
 
+// ------------------------------------------------- row-resident step loop
+// One launch per step.  A is kept row-major (rowA[r * ldb + c] = A[r, c]) and
+// rows never move: physical row r sits at logical position ilog[r], so the
+// reference's row interchange k+1 <-> p (src:479-487) is a relabelling and
+// only the column interchange touches memory (two elements per row).  A
+// workgroup owns a block of physical rows across all columns, so the rank-1
+// left update and the right update's GEMV (src:583-648) finish every row of
+// column k+1 inside the workgroup: no partials, no second pass.  What crosses
+// workgroups goes through small records written by step k-1 and read by step
+// k (ping-pong by step parity):
+//   * each workgroup's pivot candidate for column k (largest |A[i, k]| over
+//     its rows with logical index >= k+1, ties to the smaller index) with the
+//     two entries of that row the interchange rewrites, A[r, k+1] and A[r, p];
+//   * the row at logical position k+1: its physical index, A[k+1, k] (the
+//     NaN rule of the pivot decision, src:470-477) and A[k+1, k+1];
+//   * column k in logical order (gcol), the multipliers' numerators.
+// Every workgroup makes the pivot decision from the candidates itself.
+template <typename T>
+struct RowCand {
+    T v, x;        // |A[r, k]| (-1: none) and A[r, k]
+    T ak1, ap;     // A[r, k+1] and A[r, i] (i = the row's logical index)
+    int i, r;      // logical index (INT_MAX: none), physical row
+};
+template <typename T>
+struct RowNext {
+    T vk1, a11;    // A[k+1, k], A[k+1, k+1] of the row at logical position k+1
+    int r1, pad;   // its physical row
+};
+
+constexpr int kRowWaves = 8;   // waves per workgroup of the step kernel
+
+template <typename T>
+__device__ __forceinline__ bool cand_better(T v2, int i2, T v, int i) {
+    return v2 > v || (v2 == v && i2 < i);
+}
+
+// rowA = (balanced) J transposed, through a 32x32 LDS tile.
+template <typename TJ, typename T>
+__global__ void k_rows_load(const TJ *__restrict__ J, int64_t ldj, T *__restrict__ B, int64_t ldb, int n,
+                            const double *__restrict__ d, const double *__restrict__ isc, int balanced) {
+    __shared__ T tile[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int r = r0 + threadIdx.x, c = c0 + q;   // read J down column c
+        T v = 0;
+        if (r < n && c < n) {
+            TJ w = J[r + (int64_t)c * ldj];
+            if (balanced) w = w * (TJ)(isc[r] * d[c]);
+            v = (T)w;
+        }
+        tile[threadIdx.x][q] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int r = r0 + q, c = c0 + threadIdx.x;   // write along row r
+        if (r < n && c < n) B[(int64_t)r * ldb + c] = tile[q][threadIdx.x];
+    }
+}
+
+// Records for step 0: identity labels, column 0, candidates over rows >= 1.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rows_init(const T *__restrict__ B, int64_t ldb, int n, int rows_per_wg,
+                                                   RowCand<T> *__restrict__ cout, RowNext<T> *__restrict__ rout,
+                                                   T *__restrict__ gout, int *__restrict__ ilog) {
+    T bv = (T)-1, bx = 0, bk1 = 0, bp = 0;
+    int bi = INT_MAX, br = 0;
+    const int r0 = blockIdx.x * rows_per_wg, r1 = min(n, r0 + rows_per_wg);
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const T *row = B + (int64_t)r * ldb;
+        const T x = row[0];
+        ilog[r] = r;
+        gout[r] = x;
+        if (r == 1) { RowNext<T> q; q.vk1 = x; q.a11 = row[1]; q.r1 = 1; q.pad = 0; *rout = q; }
+        const T a = dabs(x);
+        if (r >= 1 && a == a && a > bv) { bv = a; bi = r; bx = x; bk1 = row[1]; bp = row[r]; br = r; }
+    }
+    __shared__ RowCand<T> sc[256];
+    RowCand<T> c;
+    c.v = bv; c.x = bx; c.ak1 = bk1; c.ap = bp; c.i = bi; c.r = br;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RowCand<T> b = sc[0];
+        for (int t = 1; t < (int)blockDim.x; t++)
+            if (cand_better(sc[t].v, sc[t].i, b.v, b.i)) b = sc[t];
+        cout[blockIdx.x] = b;
+    }
+}
+
+// Step k.  Wave w owns rows blockIdx.x*R + w*RPW + q (q < RPW); lanes run
+// along the row, U column groups of 64 per row in flight, the next group
+// loaded while the current one is processed.  Dynamic LDS holds the pivot row
+// (post-interchange row k+1) and the multipliers l_j, j >= k+2.  The loads
+// that do not depend on the pivot (the rows' first column group, their labels
+// and columns k, k+1) are issued before the decision, so one step costs about
+// three memory round trips plus its streaming.
+template <typename T, int RPW, int U>
+__global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
+    T *__restrict__ B, int64_t ldb, int n, int k, const RowCand<T> *__restrict__ cin, RowCand<T> *__restrict__ cout,
+    int ncand, const RowNext<T> *__restrict__ rin, RowNext<T> *__restrict__ rout, const T *__restrict__ gin,
+    T *__restrict__ gout, int *__restrict__ ilog, int *__restrict__ ipiv) {
+#pragma clang fp contract(off)
+    constexpr int R = kRowWaves * RPW;
+    constexpr int NT = kRowWaves * 64;
+    constexpr int FILL = 4;          // pivot-row / multiplier loads in flight per thread
+    constexpr int CPL = 4;           // candidates per lane (up to 256 workgroups)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    T *prow = reinterpret_cast<T *>(lds_dyn);   // prow[j], j in [k+1, n)
+    T *lv = prow + n;                           // lv[j], j in [k+2, n)
+    __shared__ T s_piv, s_pk, s_pak1, s_a1;
+    __shared__ int s_p, s_rp, s_skip;
+    __shared__ RowCand<T> wc[R];
+
+    // the rows of this wave and what they need before the decision
+    int il[RPW], rr[RPW];
+    int64_t ro[RPW];
+    bool ok[RPW];
+    T ck[RPW], k1[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        const int r = blockIdx.x * R + wv * RPW + q;
+        ok[q] = r < n;
+        rr[q] = min(r, n - 1);
+        ro[q] = (int64_t)rr[q] * ldb;
+        il[q] = ilog[rr[q]];
+        ck[q] = B[ro[q] + k];
+        k1[q] = B[ro[q] + k + 1];
+    }
+    const int j0 = k + 2;
+    T v[RPW][U];
+    auto load_group = [&](int jb, T(&dst)[RPW][U]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jc = min(jb + 64 * u + lane, n - 1);
+#pragma unroll
+            for (int q = 0; q < RPW; q++) dst[q][u] = B[ro[q] + jc];
+        }
+    };
+    load_group(j0, v);
+
+    // the pivot decision (src:470-477): larger |value| wins, ties to the
+    // smaller logical row; a NaN at row k+1 keeps p = k+1
+    if (wv == 0) {
+        RowCand<T> e[CPL];
+#pragma unroll
+        for (int u = 0; u < CPL; u++) e[u] = cin[min(lane + 64 * u, ncand - 1)];
+        const RowNext<T> nx = *rin;
+        T gv = (T)-1, gx = 0, gk1 = 0, gp = 0;
+        int gi = INT_MAX, gr = 0;
+#pragma unroll
+        for (int u = 0; u < CPL; u++)
+            if (lane + 64 * u < ncand && cand_better(e[u].v, e[u].i, gv, gi)) {
+                gv = e[u].v; gi = e[u].i; gx = e[u].x; gk1 = e[u].ak1; gp = e[u].ap; gr = e[u].r;
+            }
+        for (int o = 32; o > 0; o >>= 1) {
+            const T v2 = __shfl_xor(gv, o, 64), x2 = __shfl_xor(gx, o, 64);
+            const T q2 = __shfl_xor(gk1, o, 64), p2 = __shfl_xor(gp, o, 64);
+            const int i2 = __shfl_xor(gi, o, 64), r2 = __shfl_xor(gr, o, 64);
+            if (cand_better(v2, i2, gv, gi)) { gv = v2; gi = i2; gx = x2; gk1 = q2; gp = p2; gr = r2; }
+        }
+        if (lane == 0) {
+            const T a1 = nx.vk1;
+            int p = k + 1, rp = nx.r1;
+            T piv = a1, pk = nx.a11, pak1 = nx.a11;
+            if (a1 == a1 && gi != INT_MAX) {
+                p = gi;
+                piv = gx;
+                if (p != k + 1) { rp = gr; pk = gp; pak1 = gk1; }
+            }
+            s_p = p; s_rp = rp; s_piv = piv; s_pk = pk; s_pak1 = pak1; s_a1 = a1;
+            s_skip = (piv == (T)0);
+            if (blockIdx.x == 0) ipiv[k] = p;
+        }
+    }
+    __syncthreads();
+    const int p = s_p, rp = s_rp, skip = s_skip;
+    const T piv = s_piv, pk = s_pk;
+    // the element moving into column k+1 (the old column p)
+    T t0[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) t0[q] = B[ro[q] + p];
+    {
+        // post-interchange row k+1 and the multipliers (src:489-491)
+        const T pak1 = s_pak1, a1 = s_a1;
+        const T *prw = B + (int64_t)rp * ldb;
+        for (int jb = k + 1; jb < n; jb += NT * FILL) {
+            T g[FILL], c[FILL];
+#pragma unroll
+            for (int u = 0; u < FILL; u++) {
+                const int jc = min(jb + NT * u + (int)threadIdx.x, n - 1);
+                g[u] = prw[jc];
+                c[u] = gin[jc];
+            }
+#pragma unroll
+            for (int u = 0; u < FILL; u++) {
+                const int j = jb + NT * u + (int)threadIdx.x;
+                if (j < n) {
+                    prow[j] = (j == k + 1) ? pk : (j == p ? pak1 : g[u]);
+                    const T num = (j == p) ? a1 : c[u];
+                    if (j >= k + 2) lv[j] = skip ? (T)0 : num / piv;
+                }
+            }
+        }
+    }
+
+    // per row: new label and multiplier
+    int nl[RPW];
+    bool act[RPW];
+    T lq[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        nl[q] = (rr[q] == rp) ? k + 1 : (il[q] == k + 1 ? p : il[q]);
+        act[q] = ok[q] && !skip && nl[q] >= k + 2;
+        lq[q] = act[q] ? ck[q] / piv : (T)0;
+        if (act[q] && lane == 0) B[ro[q] + k] = lq[q];
+    }
+    __syncthreads();
+
+    // the sweep over columns j >= k+2: a_rj = fma(l_r, -prow_j, a_rj) on the
+    // rows below the pivot, and every row accumulates sum_j fma(l_j, a_rj)
+    T acc[RPW], c2[RPW], cn[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) { acc[q] = 0; c2[q] = 0; cn[q] = 0; }
+    for (int jb = j0; jb < n; jb += 64 * U) {
+        T vn[RPW][U];
+        load_group(jb + 64 * U, vn);     // clamped: past the end it rereads column n-1
+        T pr[U], lj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int jc = min(jb + 64 * u + lane, n - 1);
+            pr[u] = prow[jc];
+            lj[u] = lv[jc];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = jb + 64 * u + lane;
+            const bool in = j < n;
+#pragma unroll
+            for (int q = 0; q < RPW; q++) {
+                const T x = (j == p) ? k1[q] : v[q][u];    // column p takes the old column k+1
+                const T y = act[q] ? fma_t(lq[q], -pr[u], x) : x;
+                if (in && ok[q] && (act[q] || j == p)) B[ro[q] + j] = y;
+                acc[q] = in ? fma_t(lj[u], y, acc[q]) : acc[q];
+                c2[q] = (j == j0) ? y : c2[q];
+                cn[q] = (j == nl[q]) ? y : cn[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < RPW; q++) v[q][u] = vn[q][u];
+    }
+
+    // column k+1: left update (src:493-497), then the GEMV sum; candidates
+    // and records for step k+1
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        const T s = wave_sum(acc[q]);
+        const int lnl = (nl[q] - j0) & 63;
+        const T a_nl = __shfl(cn[q], lnl, 64);
+        T t = t0[q];
+        if (act[q] && pk != (T)0) t = t - lq[q] * pk;
+        if (!skip) t = t + s;
+        RowCand<T> c;
+        c.v = (T)-1; c.x = t; c.ak1 = c2[q]; c.ap = a_nl; c.i = INT_MAX; c.r = rr[q];
+        if (lane == 0 && ok[q]) {
+            B[ro[q] + k + 1] = t;
+            gout[nl[q]] = t;
+            if (nl[q] != il[q]) ilog[rr[q]] = nl[q];
+            if (nl[q] == k + 2) { RowNext<T> x; x.vk1 = t; x.a11 = c2[q]; x.r1 = rr[q]; x.pad = 0; *rout = x; }
+            const T a = dabs(t);
+            if (nl[q] >= k + 2 && a == a) { c.v = a; c.i = nl[q]; }
+        }
+        if (lane == 0) wc[wv * RPW + q] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        RowCand<T> b = wc[0];
+        for (int t = 1; t < R; t++)
+            if (cand_better(wc[t].v, wc[t].i, b.v, b.i)) b = wc[t];
+        cout[blockIdx.x] = b;
+    }
+}
+
+// perm[ilog[r]] = r (the row now at each logical position), iperm = ilog.
+__global__ void k_rows_perm(const int *__restrict__ ilog, int *__restrict__ perm, int *__restrict__ iperm, int n) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) { const int i = ilog[r]; perm[i] = r; iperm[r] = i; }
+}
+
+// factors A[i, c] = rowA[perm[i], c] (column-major, through a 32x32 tile) and
+// Ht[c, i] = A[i, c] for i <= c+1 (src:2024-2039), both from the rows in
+// logical order.
+template <typename T>
+__global__ void k_rows_out(const T *__restrict__ B, int64_t ldb, const int *__restrict__ perm, T *__restrict__ A,
+                           int64_t lda, T *__restrict__ Ht, int64_t ldh, int n) {
+    __shared__ T tile[32][33];
+    const int i0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int i = i0 + q, c = c0 + threadIdx.x;
+        T v = 0;
+        if (i < n && c < n) {
+            v = B[(int64_t)perm[i] * ldb + c];
+            Ht[c + (int64_t)i * ldh] = (i <= c + 1) ? v : (T)0;
+        }
+        tile[q][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int i = i0 + threadIdx.x, c = c0 + q;
+        if (i < n && c < n) A[i + (int64_t)c * lda] = tile[threadIdx.x][q];
+    }
+}
+
 // Ht[j, i] = A[i, j] for i <= j+1, zero elsewhere (src:2024-2039).
 template <typename T>
 __global__ void k_ht_fill(const T *__restrict__ A, int64_t lda, T *__restrict__ Ht, int64_t ldh, int n) {
@@ -481,15 +796,77 @@ Status steps_typed(lhl_handle *h) {
     return {};
 }
 
+// The row-resident step loop at one (rows per wave, unroll) shape.
+template <typename T, int RPW, int U>
+Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds) {
+    const int n = (int)h->n;
+    constexpr int R = kRowWaves * RPW;
+    const int G = (n + R - 1) / R;
+    T *B = h->rowA.as<T>();
+    RowCand<T> *cand = h->cand.as<RowCand<T>>();
+    RowNext<T> *nxt = reinterpret_cast<RowNext<T> *>(cand + 2 * G);
+    T *g = h->gcol.as<T>();
+    int *ilog = h->ilog.as<int>();
+    LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt, g, ilog);
+    auto fn = k_rows_step<T, RPW, U>;
+    LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int k = 0; k + 2 < n; k++) {
+        const int a = k & 1, b = a ^ 1;
+        LHL_LAUNCH(h, "reduce_step", fn, dim3(G), dim3(kRowWaves * 64), lds, B, ldb, n, k,
+                   (const RowCand<T> *)(cand + a * G), cand + b * G, G, (const RowNext<T> *)(nxt + a), nxt + b,
+                   (const T *)(g + (int64_t)a * n), g + (int64_t)b * n, ilog, h->ipiv.as<int>());
+    }
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+// Whether the row-resident loop takes this reduction: the pivot row and the
+// multipliers (2 n elements) must fit the LDS of one workgroup.
+inline bool rows_fit(int64_t n, int dtype) {
+    // at most 256 workgroups (the decision reads 4 candidates per lane)
+    return n >= 3 && n <= 16384 && (size_t)(2 * n) * elt_size(dtype) + 8 * 1024 <= lds_limit();
+}
+
+template <typename TJ, typename T>
+Status rows_typed(lhl_handle *h, int balanced) {
+    const int n = (int)h->n;
+    const int64_t ldb = pad_ld(n, h->dtype);
+    const int maxG = (n + kRowWaves - 1) / kRowWaves;
+    LHL_TRY(h->rowA.ensure(sizeof(T) * (size_t)ldb * n));
+    LHL_TRY(h->gcol.ensure(sizeof(T) * 2 * (size_t)n));
+    LHL_TRY(h->ilog.ensure(sizeof(int) * (size_t)n));
+    LHL_TRY(h->cand.ensure(sizeof(RowCand<T>) * 2 * (size_t)maxG + sizeof(RowNext<T>) * 2 + 64));
+    T *B = h->rowA.as<T>();
+    dim3 tb(32, 8), gr((n + 31) / 32, (n + 31) / 32);
+    LHL_LAUNCH(h, "reduce_copy", (k_rows_load<TJ, T>), gr, tb, 0, (const TJ *)h->J.as<TJ>(), h->ldj, B, ldb, n,
+               (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(), balanced);
+    const size_t lds = sizeof(T) * 2 * (size_t)n;
+    // rows per wave so that about 256 workgroups cover the rows; loads in
+    // flight per lane = rows per wave x unroll
+    constexpr int W = sizeof(T) == 8 ? 1 : 2;
+    if (n <= 2048) LHL_TRY((rows_steps<T, 1, 8 * W>(h, ldb, lds)));
+    else if (n <= 4096) LHL_TRY((rows_steps<T, 2, 8 * W>(h, ldb, lds)));
+    else if (n <= 8192) LHL_TRY((rows_steps<T, 4, 4 * W>(h, ldb, lds)));
+    else LHL_TRY((rows_steps<T, 8, 1 * W>(h, ldb, lds)));
+    LHL_LAUNCH(h, "reduce_finish", k_rows_perm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->ilog.as<int>(),
+               h->perm.as<int>(), h->iperm.as<int>(), n);
+    LHL_LAUNCH(h, "reduce_finish", k_rows_out<T>, gr, tb, 0, (const T *)B, ldb, (const int *)h->perm.as<int>(),
+               h->A.as<T>(), h->lda, h->Ht.as<T>(), h->lda, n);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
 template <typename T>
-Status finish_typed(lhl_handle *h) {
+Status finish_typed(lhl_handle *h, int rows) {
     const int n = (int)h->n;
     dim3 tb(32, 8);
     dim3 gr((n + 31) / 32, (n + 31) / 32);
-    LHL_LAUNCH(h, "reduce_finish", k_ht_fill<T>, gr, tb, 0, (const T *)h->A.as<T>(), h->lda, h->Ht.as<T>(),
-                       h->lda, n);
-    LHL_LAUNCH(h, "reduce_finish", k_iperm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->perm.as<int>(),
-                       h->iperm.as<int>(), n);
+    if (!rows) {
+        LHL_LAUNCH(h, "reduce_finish", k_ht_fill<T>, gr, tb, 0, (const T *)h->A.as<T>(), h->lda, h->Ht.as<T>(),
+                   h->lda, n);
+        LHL_LAUNCH(h, "reduce_finish", k_iperm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->perm.as<int>(),
+                   h->iperm.as<int>(), n);
+    }
     LHL_LAUNCH(h, "reduce_finish", k_cast_scale<T>, dim3((n + 255) / 256), dim3(256), 0,
                        (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(),
                        h->scaleT.as<T>(), h->iscaleT.as<T>(), n);
@@ -522,20 +899,28 @@ Status reduce_device(lhl_handle *h, int balance) {
     LHL_LAUNCH(h, "reduce_copy", k_iota, dim3((n + 255) / 256), dim3(256), 0, h->perm.as<int>(), n);
     LHL_HIP_TRY(hipGetLastError());
     if (balance) LHL_TRY(balance_device(h));
-    if (h->j_dtype == LHL_F64) {
-        if (h->dtype == LHL_F64) LHL_TRY((copy_scale_typed<double, double>(h, balance)));
-        else LHL_TRY((copy_scale_typed<double, float>(h, balance)));
+    const int rows = rows_fit(n, h->dtype) ? 1 : 0;
+    if (rows) {
+        if (h->j_dtype == LHL_F64) {
+            if (h->dtype == LHL_F64) LHL_TRY((rows_typed<double, double>(h, balance)));
+            else LHL_TRY((rows_typed<double, float>(h, balance)));
+        } else {
+            if (h->dtype == LHL_F64) LHL_TRY((rows_typed<float, double>(h, balance)));
+            else LHL_TRY((rows_typed<float, float>(h, balance)));
+        }
     } else {
-        if (h->dtype == LHL_F64) LHL_TRY((copy_scale_typed<float, double>(h, balance)));
-        else LHL_TRY((copy_scale_typed<float, float>(h, balance)));
+        if (h->j_dtype == LHL_F64) {
+            if (h->dtype == LHL_F64) LHL_TRY((copy_scale_typed<double, double>(h, balance)));
+            else LHL_TRY((copy_scale_typed<double, float>(h, balance)));
+        } else {
+            if (h->dtype == LHL_F64) LHL_TRY((copy_scale_typed<float, double>(h, balance)));
+            else LHL_TRY((copy_scale_typed<float, float>(h, balance)));
+        }
+        if (h->dtype == LHL_F64) LHL_TRY(steps_typed<double>(h));
+        else LHL_TRY(steps_typed<float>(h));
     }
-    if (h->dtype == LHL_F64) {
-        LHL_TRY(steps_typed<double>(h));
-        LHL_TRY(finish_typed<double>(h));
-    } else {
-        LHL_TRY(steps_typed<float>(h));
-        LHL_TRY(finish_typed<float>(h));
-    }
+    if (h->dtype == LHL_F64) LHL_TRY(finish_typed<double>(h, rows));
+    else LHL_TRY(finish_typed<float>(h, rows));
     return linv_device(h);
 }
 
