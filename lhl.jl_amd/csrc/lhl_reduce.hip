@@ -468,22 +468,25 @@ __global__ void __launch_bounds__(256) k_rows_init(const T *__restrict__ B, int6
     }
 }
 
-// Step k.  Wave w owns rows blockIdx.x*R + w*RPW + q (q < RPW); lanes run
-// along the row, U column groups of 64 per row in flight, the next group
-// loaded while the current one is processed.  Dynamic LDS holds the pivot row
+// Step k.  Wave w owns rows blockIdx.x*R + w*RPW*PASSES + i, taken in
+// PASSES passes of RPW rows; lanes run along the rows, U column groups of 64
+// per row in flight, the next group (or the next pass's first group) loaded
+// while the current one is processed.  Dynamic LDS holds the pivot row
 // (post-interchange row k+1) and the multipliers l_j, j >= k+2.  The loads
-// that do not depend on the pivot (the rows' first column group, their labels
-// and columns k, k+1) are issued before the decision, so one step costs about
-// three memory round trips plus its streaming.
-template <typename T, int RPW, int U>
+// that do not depend on the pivot (the first column group, the labels and
+// columns k, k+1 of the rows, column k in logical order) are issued before the
+// decision, so one step costs about three memory round trips plus its
+// streaming.
+template <typename T, int RPW, int U, int PASSES>
 __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     T *__restrict__ B, int64_t ldb, int n, int k, const RowCand<T> *__restrict__ cin, RowCand<T> *__restrict__ cout,
     int ncand, const RowNext<T> *__restrict__ rin, RowNext<T> *__restrict__ rout, const T *__restrict__ gin,
     T *__restrict__ gout, int *__restrict__ ilog, int *__restrict__ ipiv) {
 #pragma clang fp contract(off)
-    constexpr int R = kRowWaves * RPW;
+    constexpr int RW = RPW * PASSES;            // rows per wave
+    constexpr int R = kRowWaves * RW;
     constexpr int NT = kRowWaves * 64;
-    constexpr int FILL = 4;          // pivot-row / multiplier loads in flight per thread
+    constexpr int FILL = 8;          // pivot-row / multiplier loads in flight per thread
     constexpr int CPL = 4;           // candidates per lane (up to 256 workgroups)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     extern __shared__ __align__(16) unsigned char lds_dyn[];
@@ -491,16 +494,16 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     T *lv = prow + n;                           // lv[j], j in [k+2, n)
     __shared__ T s_piv, s_pk, s_pak1, s_a1;
     __shared__ int s_p, s_rp, s_skip;
-    __shared__ RowCand<T> wc[R];
+    __shared__ RowCand<T> wc[kRowWaves];
 
     // the rows of this wave and what they need before the decision
-    int il[RPW], rr[RPW];
-    int64_t ro[RPW];
-    bool ok[RPW];
-    T ck[RPW], k1[RPW];
+    int il[RW], rr[RW];
+    int64_t ro[RW];
+    bool ok[RW];
+    T ck[RW], k1[RW];
 #pragma unroll
-    for (int q = 0; q < RPW; q++) {
-        const int r = blockIdx.x * R + wv * RPW + q;
+    for (int q = 0; q < RW; q++) {
+        const int r = blockIdx.x * R + wv * RW + q;
         ok[q] = r < n;
         rr[q] = min(r, n - 1);
         ro[q] = (int64_t)rr[q] * ldb;
@@ -510,15 +513,16 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     }
     const int j0 = k + 2;
     T v[RPW][U];
-    auto load_group = [&](int jb, T(&dst)[RPW][U]) __attribute__((always_inline)) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int jc = min(jb + 64 * u + lane, n - 1);
+    for (int u = 0; u < U; u++) {
+        const int jc = min(j0 + 64 * u + lane, n - 1);
 #pragma unroll
-            for (int q = 0; q < RPW; q++) dst[q][u] = B[ro[q] + jc];
-        }
-    };
-    load_group(j0, v);
+        for (int q = 0; q < RPW; q++) v[q][u] = B[ro[q] + jc];
+    }
+    // the multipliers' numerators (column k in logical order) of the first fill pass
+    T gpre[FILL];
+#pragma unroll
+    for (int u = 0; u < FILL; u++) gpre[u] = gin[min(k + 1 + NT * u + (int)threadIdx.x, n - 1)];
 
     // the pivot decision (src:470-477): larger |value| wins, ties to the
     // smaller logical row; a NaN at row k+1 keeps p = k+1
@@ -527,19 +531,26 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
 #pragma unroll
         for (int u = 0; u < CPL; u++) e[u] = cin[min(lane + 64 * u, ncand - 1)];
         const RowNext<T> nx = *rin;
-        T gv = (T)-1, gx = 0, gk1 = 0, gp = 0;
-        int gi = INT_MAX, gr = 0;
+        // each lane's best, then (value, index) alone through the butterfly
+        // with the owning lane; the winner's fields are read from that lane
+        int bu = 0;
+        T gv = (T)-1;
+        int gi = INT_MAX;
 #pragma unroll
         for (int u = 0; u < CPL; u++)
-            if (lane + 64 * u < ncand && cand_better(e[u].v, e[u].i, gv, gi)) {
-                gv = e[u].v; gi = e[u].i; gx = e[u].x; gk1 = e[u].ak1; gp = e[u].ap; gr = e[u].r;
-            }
+            if (lane + 64 * u < ncand && cand_better(e[u].v, e[u].i, gv, gi)) { gv = e[u].v; gi = e[u].i; bu = u; }
+        T gx = e[0].x, gk1 = e[0].ak1, gp = e[0].ap;
+        int gr = e[0].r;
+#pragma unroll
+        for (int u = 1; u < CPL; u++)
+            if (bu == u) { gx = e[u].x; gk1 = e[u].ak1; gp = e[u].ap; gr = e[u].r; }
+        int gl = lane;
         for (int o = 32; o > 0; o >>= 1) {
-            const T v2 = __shfl_xor(gv, o, 64), x2 = __shfl_xor(gx, o, 64);
-            const T q2 = __shfl_xor(gk1, o, 64), p2 = __shfl_xor(gp, o, 64);
-            const int i2 = __shfl_xor(gi, o, 64), r2 = __shfl_xor(gr, o, 64);
-            if (cand_better(v2, i2, gv, gi)) { gv = v2; gi = i2; gx = x2; gk1 = q2; gp = p2; gr = r2; }
+            const T v2 = __shfl_xor(gv, o, 64);
+            const int i2 = __shfl_xor(gi, o, 64), l2 = __shfl_xor(gl, o, 64);
+            if (cand_better(v2, i2, gv, gi)) { gv = v2; gi = i2; gl = l2; }
         }
+        gx = __shfl(gx, gl, 64); gk1 = __shfl(gk1, gl, 64); gp = __shfl(gp, gl, 64); gr = __shfl(gr, gl, 64);
         if (lane == 0) {
             const T a1 = nx.vk1;
             int p = k + 1, rp = nx.r1;
@@ -558,21 +569,14 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     const int p = s_p, rp = s_rp, skip = s_skip;
     const T piv = s_piv, pk = s_pk;
     // the element moving into column k+1 (the old column p)
-    T t0[RPW];
+    T t0[RW];
 #pragma unroll
-    for (int q = 0; q < RPW; q++) t0[q] = B[ro[q] + p];
+    for (int q = 0; q < RW; q++) t0[q] = B[ro[q] + p];
     {
         // post-interchange row k+1 and the multipliers (src:489-491)
         const T pak1 = s_pak1, a1 = s_a1;
         const T *prw = B + (int64_t)rp * ldb;
-        for (int jb = k + 1; jb < n; jb += NT * FILL) {
-            T g[FILL], c[FILL];
-#pragma unroll
-            for (int u = 0; u < FILL; u++) {
-                const int jc = min(jb + NT * u + (int)threadIdx.x, n - 1);
-                g[u] = prw[jc];
-                c[u] = gin[jc];
-            }
+        auto fill = [&](int jb, const T(&g)[FILL], const T(&c)[FILL]) __attribute__((always_inline)) {
 #pragma unroll
             for (int u = 0; u < FILL; u++) {
                 const int j = jb + NT * u + (int)threadIdx.x;
@@ -582,15 +586,31 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
                     if (j >= k + 2) lv[j] = skip ? (T)0 : num / piv;
                 }
             }
+        };
+        {
+            T g[FILL];
+#pragma unroll
+            for (int u = 0; u < FILL; u++) g[u] = prw[min(k + 1 + NT * u + (int)threadIdx.x, n - 1)];
+            fill(k + 1, g, gpre);
+        }
+        for (int jb = k + 1 + NT * FILL; jb < n; jb += NT * FILL) {
+            T g[FILL], c[FILL];
+#pragma unroll
+            for (int u = 0; u < FILL; u++) {
+                const int jc = min(jb + NT * u + (int)threadIdx.x, n - 1);
+                g[u] = prw[jc];
+                c[u] = gin[jc];
+            }
+            fill(jb, g, c);
         }
     }
 
     // per row: new label and multiplier
-    int nl[RPW];
-    bool act[RPW];
-    T lq[RPW];
+    int nl[RW];
+    bool act[RW];
+    T lq[RW];
 #pragma unroll
-    for (int q = 0; q < RPW; q++) {
+    for (int q = 0; q < RW; q++) {
         nl[q] = (rr[q] == rp) ? k + 1 : (il[q] == k + 1 ? p : il[q]);
         act[q] = ok[q] && !skip && nl[q] >= k + 2;
         lq[q] = act[q] ? ck[q] / piv : (T)0;
@@ -598,69 +618,95 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     }
     __syncthreads();
 
-    // the sweep over columns j >= k+2: a_rj = fma(l_r, -prow_j, a_rj) on the
-    // rows below the pivot, and every row accumulates sum_j fma(l_j, a_rj)
-    T acc[RPW], c2[RPW], cn[RPW];
+    RowCand<T> wb;
+    wb.v = (T)-1; wb.x = 0; wb.ak1 = 0; wb.ap = 0; wb.i = INT_MAX; wb.r = 0;
 #pragma unroll
-    for (int q = 0; q < RPW; q++) { acc[q] = 0; c2[q] = 0; cn[q] = 0; }
-    for (int jb = j0; jb < n; jb += 64 * U) {
-        T vn[RPW][U];
-        load_group(jb + 64 * U, vn);     // clamped: past the end it rereads column n-1
-        T pr[U], lj[U];
+    for (int ps = 0; ps < PASSES; ps++) {
+        // the sweep over columns j >= k+2: a_rj = fma(l_r, -prow_j, a_rj) on
+        // the rows below the pivot, and every row accumulates sum_j fma(l_j, a_rj)
+        T acc[RPW], c2[RPW], cn[RPW];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int jc = min(jb + 64 * u + lane, n - 1);
-            pr[u] = prow[jc];
-            lj[u] = lv[jc];
-        }
+        for (int q = 0; q < RPW; q++) { acc[q] = 0; c2[q] = 0; cn[q] = 0; }
+        for (int jb = j0; jb < n; jb += 64 * U) {
+            // next group: this pass's, or past the end the next pass's first
+            const bool last = jb + 64 * U >= n;
+            const bool jump = last && ps + 1 < PASSES;
+            T vn[RPW][U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int j = jb + 64 * u + lane;
-            const bool in = j < n;
+            for (int u = 0; u < U; u++) {
+                const int jc = min((jump ? j0 : jb + 64 * U) + 64 * u + lane, n - 1);
 #pragma unroll
-            for (int q = 0; q < RPW; q++) {
-                const T x = (j == p) ? k1[q] : v[q][u];    // column p takes the old column k+1
-                const T y = act[q] ? fma_t(lq[q], -pr[u], x) : x;
-                if (in && ok[q] && (act[q] || j == p)) B[ro[q] + j] = y;
-                acc[q] = in ? fma_t(lj[u], y, acc[q]) : acc[q];
-                c2[q] = (j == j0) ? y : c2[q];
-                cn[q] = (j == nl[q]) ? y : cn[q];
+                for (int q = 0; q < RPW; q++) {
+                    const int64_t base = (ps + 1 < PASSES && jump) ? ro[min(ps + 1, PASSES - 1) * RPW + q]
+                                                                   : ro[ps * RPW + q];
+                    vn[q][u] = B[base + jc];
+                }
             }
+            T pr[U], lj[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int jc = min(jb + 64 * u + lane, n - 1);
+                pr[u] = prow[jc];
+                lj[u] = lv[jc];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int j = jb + 64 * u + lane;
+                const bool in = j < n;
+#pragma unroll
+                for (int q = 0; q < RPW; q++) {
+                    const int w = ps * RPW + q;
+                    const T x = (j == p) ? k1[w] : v[q][u];    // column p takes the old column k+1
+                    const T y = act[w] ? fma_t(lq[w], -pr[u], x) : x;
+                    if (in && ok[w] && (act[w] || j == p)) B[ro[w] + j] = y;
+                    acc[q] = in ? fma_t(lj[u], y, acc[q]) : acc[q];
+                    c2[q] = (j == j0) ? y : c2[q];
+                    cn[q] = (j == nl[w]) ? y : cn[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int q = 0; q < RPW; q++) v[q][u] = vn[q][u];
         }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int q = 0; q < RPW; q++) v[q][u] = vn[q][u];
-    }
 
-    // column k+1: left update (src:493-497), then the GEMV sum; candidates
-    // and records for step k+1
+        // column k+1: left update (src:493-497), then the GEMV sum;
+        // candidates and records for step k+1
 #pragma unroll
-    for (int q = 0; q < RPW; q++) {
-        const T s = wave_sum(acc[q]);
-        const int lnl = (nl[q] - j0) & 63;
-        const T a_nl = __shfl(cn[q], lnl, 64);
-        T t = t0[q];
-        if (act[q] && pk != (T)0) t = t - lq[q] * pk;
-        if (!skip) t = t + s;
-        RowCand<T> c;
-        c.v = (T)-1; c.x = t; c.ak1 = c2[q]; c.ap = a_nl; c.i = INT_MAX; c.r = rr[q];
-        if (lane == 0 && ok[q]) {
-            B[ro[q] + k + 1] = t;
-            gout[nl[q]] = t;
-            if (nl[q] != il[q]) ilog[rr[q]] = nl[q];
-            if (nl[q] == k + 2) { RowNext<T> x; x.vk1 = t; x.a11 = c2[q]; x.r1 = rr[q]; x.pad = 0; *rout = x; }
-            const T a = dabs(t);
-            if (nl[q] >= k + 2 && a == a) { c.v = a; c.i = nl[q]; }
+        for (int q = 0; q < RPW; q++) {
+            const int w = ps * RPW + q;
+            const T s = wave_sum(acc[q]);
+            const int lnl = (nl[w] - j0) & 63;
+            const T a_nl = __shfl(cn[q], lnl, 64);
+            T t = t0[w];
+            if (act[w] && pk != (T)0) t = t - lq[w] * pk;
+            if (!skip) t = t + s;
+            RowCand<T> c;
+            c.v = (T)-1; c.x = t; c.ak1 = c2[q]; c.ap = a_nl; c.i = INT_MAX; c.r = rr[w];
+            if (lane == 0 && ok[w]) {
+                B[ro[w] + k + 1] = t;
+                gout[nl[w]] = t;
+                if (nl[w] != il[w]) ilog[rr[w]] = nl[w];
+                if (nl[w] == k + 2) { RowNext<T> x; x.vk1 = t; x.a11 = c2[q]; x.r1 = rr[w]; x.pad = 0; *rout = x; }
+                const T a = dabs(t);
+                if (nl[w] >= k + 2 && a == a) { c.v = a; c.i = nl[w]; }
+            }
+            if (cand_better(c.v, c.i, wb.v, wb.i)) wb = c;
         }
-        if (lane == 0) wc[wv * RPW + q] = c;
     }
+    if (lane == 0) wc[wv] = wb;
     __syncthreads();
     if (threadIdx.x == 0) {
-        RowCand<T> b = wc[0];
-        for (int t = 1; t < R; t++)
-            if (cand_better(wc[t].v, wc[t].i, b.v, b.i)) b = wc[t];
-        cout[blockIdx.x] = b;
+        int bt = 0;
+        T bv = wc[0].v;
+        int bi = wc[0].i;
+#pragma unroll
+        for (int t = 1; t < kRowWaves; t++) {
+            const T v2 = wc[t].v;
+            const int i2 = wc[t].i;
+            if (cand_better(v2, i2, bv, bi)) { bv = v2; bi = i2; bt = t; }
+        }
+        cout[blockIdx.x] = wc[bt];
     }
 }
 
@@ -797,10 +843,10 @@ Status steps_typed(lhl_handle *h) {
 }
 
 // The row-resident step loop at one (rows per wave, unroll) shape.
-template <typename T, int RPW, int U>
+template <typename T, int RPW, int U, int PASSES>
 Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds) {
     const int n = (int)h->n;
-    constexpr int R = kRowWaves * RPW;
+    constexpr int R = kRowWaves * RPW * PASSES;
     const int G = (n + R - 1) / R;
     T *B = h->rowA.as<T>();
     RowCand<T> *cand = h->cand.as<RowCand<T>>();
@@ -808,7 +854,7 @@ Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds) {
     T *g = h->gcol.as<T>();
     int *ilog = h->ilog.as<int>();
     LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt, g, ilog);
-    auto fn = k_rows_step<T, RPW, U>;
+    auto fn = k_rows_step<T, RPW, U, PASSES>;
     LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int k = 0; k + 2 < n; k++) {
         const int a = k & 1, b = a ^ 1;
@@ -844,10 +890,10 @@ Status rows_typed(lhl_handle *h, int balanced) {
     // rows per wave so that about 256 workgroups cover the rows; loads in
     // flight per lane = rows per wave x unroll
     constexpr int W = sizeof(T) == 8 ? 1 : 2;
-    if (n <= 2048) LHL_TRY((rows_steps<T, 1, 8 * W>(h, ldb, lds)));
-    else if (n <= 4096) LHL_TRY((rows_steps<T, 2, 8 * W>(h, ldb, lds)));
-    else if (n <= 8192) LHL_TRY((rows_steps<T, 4, 4 * W>(h, ldb, lds)));
-    else LHL_TRY((rows_steps<T, 8, 1 * W>(h, ldb, lds)));
+    if (n <= 2048) LHL_TRY((rows_steps<T, 1, 8 * W, 1>(h, ldb, lds)));
+    else if (n <= 4096) LHL_TRY((rows_steps<T, 2, 8 * W, 1>(h, ldb, lds)));
+    else if (n <= 8192) LHL_TRY((rows_steps<T, 2, 8 * W, 2>(h, ldb, lds)));
+    else LHL_TRY((rows_steps<T, 2, 8 * W, 4>(h, ldb, lds)));
     LHL_LAUNCH(h, "reduce_finish", k_rows_perm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->ilog.as<int>(),
                h->perm.as<int>(), h->iperm.as<int>(), n);
     LHL_LAUNCH(h, "reduce_finish", k_rows_out<T>, gr, tb, 0, (const T *)B, ldb, (const int *)h->perm.as<int>(),

@@ -261,6 +261,32 @@ def test_applyZ_roundtrip(lhl_hip, oracle):
     assert relerr(back, x0) <= 1e-12 * n
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reduction_two_pass_rows_properties(lhl_hip, dtype):
+    # n = 4500 takes the step loop's two-passes-per-wave shape, beyond the
+    # sizes the oracle finishes in seconds: check what the mathematics fixes.
+    # J Z = Z H with Z = D P L; partial pivoting bounds every multiplier by 1
+    # (src:470-491); the pivots are rows below the diagonal.
+    n = 4500
+    say(f"case: reduction properties n={n} {np.dtype(dtype).name}")
+    J = stiff(4500, n)
+    ws = lhl_hip.lhl(J, dtype=dtype)
+    F = ws.factors.astype(np.float64)
+    ipiv = ws.ipiv - 1
+    assert np.all(ipiv >= np.arange(1, n - 1)) and np.all(ipiv < n)
+    assert np.array_equal(np.sort(ws.perm - 1), np.arange(n))
+    L = np.tril(F, -2)
+    assert np.all(np.abs(L) <= 1.0)
+    H = np.triu(F, -1)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n)
+    zx = lhl_hip.applyZ_(x.copy(), ws)
+    zhx = lhl_hip.applyZ_(H @ x, ws)
+    rel = np.linalg.norm(J @ zx - zhx) / (np.linalg.norm(J, np.inf) * np.linalg.norm(zx))
+    say(f"  |J Z x - Z H x| / (|J| |Z x|) = {rel:.2e}")
+    assert rel <= (1e-13 if dtype == np.float64 else 1e-5)
+
+
 def test_workspace_reuse_and_resize(lhl_hip):
     n = 50
     J = randn(3, n)
