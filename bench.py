@@ -51,10 +51,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-KERNELS = ["shift", "uinv", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "gemm_jx", "resid_combine",
-           "colnorms", "convert"]
+KERNELS = ["shift", "uinv", "ut_pack", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "gemm_jx",
+           "resid_combine", "colnorms", "convert"]
 PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
         "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
+
+
+def lhl_roofline(n, es, ms):
+    """lhl() against two bounds.  Unblocked (the step loop run here): each of
+    the n-2 steps reads the n x (n-k) trailing columns and writes back the
+    (n-k)^2 trailing block, 5/6 s n^3 bytes.  Blocked delayed update (the
+    reference's algorithm above n = 500): a read-only GEMV stream of s n^3/3
+    bytes plus n^3 flops of panel GEMMs on MFMA."""
+    unb = es * 5.0 / 6.0 * n ** 3
+    blk_s = es * n ** 3 / 3.0 / (HBM_PEAK_GBS * 1e9) + n ** 3 / ((FP64_PEAK_TFLOPS if es == 8 else FP32_PEAK_TFLOPS)
+                                                               * 1e12)
+    return {"bound": "hbm", "algorithmic_bytes_unblocked": unb, "achieved": unb / (ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": unb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "blocked_bound_ms": 1e3 * blk_s, "frac_of_blocked_bound": 1e3 * blk_s / ms}
 
 
 def algorithmic(kernel, n, S, ncol, es):
@@ -71,6 +85,9 @@ def algorithmic(kernel, n, S, ncol, es):
         return (tri + n) * es + S * (tri + 3 * n) * es, S * n * n, "hbm"
     if kernel == "uinv":
         return S * nb * 64 * 64 * es * 2, S * nb * 2 * 64 ** 3 / 3, "hbm"
+    if kernel == "ut_pack":
+        # every shift's packed U read and written again in packed columns
+        return 2 * S * tri * es, 0, "hbm"
     if kernel == "gather":
         return col + ncol * n * 8 + n * 12, 0, "hbm"
     if kernel == "trsm":
@@ -358,6 +375,7 @@ def main():
         "fp32_refine_over_fp64": fp32["sps"] / fp64["sps"],
         "lhl_ms": lhl_ms,
         "lhl_kernel_ms": lhl_kernels,
+        "lhl_roofline": {p: lhl_roofline(n, 8 if p == "fp64" else 4, lhl_ms[p]) for p in lhl_ms},
         "max_rel_resid": {"fp32_refine": fp32["rel"], "fp64": fp64["rel"]},
         "refine_rounds": {"fp32": [min(fp32["iters"]), max(fp32["iters"])],
                           "fp64": [min(fp64["iters"]), max(fp64["iters"])]},

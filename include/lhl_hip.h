@@ -157,8 +157,10 @@ int lhl_get_stream(lhl_handle *h, void **stream);
 /* While enabled, every kernel launch is bracketed by HIP events on the
  * handle's stream.  lhl_profile_read synchronizes, then returns the launches
  * and the summed duration (ms) of one kernel since the last reset:
- * reduction "balance", "reduce_copy", "reduce_pivot", "reduce_swap",
- * "reduce_trailing", "reduce_finish"; shift "shift", "uinv"; solve "gather",
+ * reduction "balance", "reduce_copy", "reduce_step" (one launch per
+ * elimination step), "reduce_pivot", "reduce_swap", "reduce_trailing" (the
+ * three-launch step of n beyond the row-resident loop), "reduce_finish";
+ * shift "shift", "uinv", "ut_pack"; solve "gather",
  * "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "applyz", "convert"; refinement
  * "gemm_jx", "resid_combine", "colnorms". */
 int lhl_profile_enable(lhl_handle *h, int on);

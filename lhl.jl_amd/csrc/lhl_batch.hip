@@ -944,7 +944,7 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     if (n <= 512 * 16) {
         LHL_TRY(UT.ensure(sizeof(T) * (size_t)ustride * (size_t)nshift));
         const int planes = cplx ? 2 : 1;
-        LHL_LAUNCH(h, "uinv", k_ut_pack<T>, dim3((unsigned)(nbu * (nbu + 1) / 2), (unsigned)(nshift * planes)),
+        LHL_LAUNCH(h, "ut_pack", k_ut_pack<T>, dim3((unsigned)(nbu * (nbu + 1) / 2), (unsigned)(nshift * planes)),
                    dim3(256), 0, n, (const T *)U, UT.as<T>(), ustride, planes);
     }
     LHL_HIP_TRY(hipGetLastError());
