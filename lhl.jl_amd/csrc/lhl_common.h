@@ -27,6 +27,50 @@ __device__ __forceinline__ void lds_barrier() {
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Wave-wide sum by the xor butterfly v += v[lane ^ o], o = 32, 16, ..., 1 (the
+// pairing of a __shfl_xor loop, so bit-identical to it), with one LDS-crossbar
+// permute for o = 32, swizzles (bitmask mode, xor within 32 lanes) for 16, 8,
+// 4 and DPP quad permutes for 2, 1.  Call with every lane of the wave active.
+namespace wsum_detail {
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+template <int PAT> __device__ __forceinline__ float swz(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), PAT));
+}
+template <int PAT> __device__ __forceinline__ double swz(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)b, PAT);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), PAT);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+}  // namespace wsum_detail
+
+template <typename T> __device__ __forceinline__ T wave_sum_fast(T v) {
+    using namespace wsum_detail;
+    v = v + __shfl_xor(v, 32, 64);
+    v = v + swz<(16 << 10) | 0x1F>(v);
+    v = v + swz<(8 << 10) | 0x1F>(v);
+    v = v + swz<(4 << 10) | 0x1F>(v);
+    v = v + dpp<0x4E>(v);     // quad_perm [2,3,0,1]: lane ^ 2
+    v = v + dpp<0xB1>(v);     // quad_perm [1,0,3,2]: lane ^ 1
+    return v;
+}
+
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 

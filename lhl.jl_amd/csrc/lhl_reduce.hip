@@ -61,27 +61,50 @@ __global__ void __launch_bounds__(256) k_bal_block(const TJ *__restrict__ J, int
     __shared__ TJ red_r[4][kBalB];
     __shared__ int last;
 
-    // column sums: wave wv takes block columns u = wv, wv+4, ...; lanes run down j.
-    for (int u = wv; u < B; u += 4) {
-        const int i = i0 + u;
-        TJ acc = 0;
-        for (int j = ja + lane; j < jb; j += 64) {
-            if (j >= i0 && j < i0 + B) continue;
-            acc += dabs(J[j + (int64_t)i * ldj]) * (TJ)isc[j];
+    // column sums: wave wv takes block columns u = wv + 4m (m < 16); lanes run
+    // down j.  All 16 columns' loads of a 64-row chunk are issued together;
+    // in-block rows add an exact zero.
+    {
+        TJ acc[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++) acc[m] = 0;
+        for (int jc = ja; jc < jb; jc += 64) {
+            const int j = jc + lane;
+            const bool use = j < jb && !(j >= i0 && j < i0 + B);
+            const int jj = min(j, n - 1);
+            const TJ w = use ? (TJ)isc[jj] : (TJ)0;
+            TJ a[16];
+#pragma unroll
+            for (int m = 0; m < 16; m++) a[m] = J[jj + (int64_t)(i0 + min(wv + 4 * m, B - 1)) * ldj];
+#pragma unroll
+            for (int m = 0; m < 16; m++) acc[m] += use ? dabs(a[m]) * w : (TJ)0;
         }
-        acc = wave_sum(acc);
-        if (lane == 0) part[(int64_t)g * 2 * kBalB + u] = acc;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const TJ t = wave_sum_fast(acc[m]);
+            if (lane == 0 && wv + 4 * m < B) part[(int64_t)g * 2 * kBalB + wv + 4 * m] = t;
+        }
     }
-    // row sums: lane = block row, waves split the slice's columns.
+    // row sums: lane = block row, waves split the slice's columns (j = ja + wv
+    // + 4t, in order), eight loads in flight.
     {
         TJ acc = 0;
-        if (lane < B) {
-            for (int j = ja + wv; j < jb; j += 4) {
-                if (j >= i0 && j < i0 + B) continue;
-                acc += dabs(J[(i0 + lane) + (int64_t)j * ldj]) * (TJ)d[j];
+        const int r = i0 + min(lane, B - 1);
+        for (int jb8 = ja + wv; jb8 < jb; jb8 += 32) {
+            TJ a[8], w[8];
+            bool use[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int j = jb8 + 4 * t;
+                use[t] = j < jb && !(j >= i0 && j < i0 + B);
+                const int jj = min(j, n - 1);
+                a[t] = J[r + (int64_t)jj * ldj];
+                w[t] = (TJ)d[jj];
             }
+#pragma unroll
+            for (int t = 0; t < 8; t++) acc += use[t] ? dabs(a[t]) * w[t] : (TJ)0;
         }
-        red_r[wv][lane] = acc;
+        red_r[wv][lane] = lane < B ? acc : (TJ)0;
     }
     __syncthreads();
     if (threadIdx.x < kBalB)
@@ -97,20 +120,31 @@ __global__ void __launch_bounds__(256) k_bal_block(const TJ *__restrict__ J, int
     // The last workgroup: in-block terms in index order, one wave.
     __shared__ TJ blk[kBalB][kBalB + 1];   // blk[r][c] = J[i0+r, i0+c]
     __shared__ double sd[kBalB], si[kBalB];
+    __shared__ TJ sp[2 * kBalB];           // the partials summed over the workgroups
     for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
         int r = t % B, c = t / B;
         blk[r][c] = J[(i0 + r) + (int64_t)(i0 + c) * ldj];
     }
     if (threadIdx.x < B) { sd[threadIdx.x] = d[i0 + threadIdx.x]; si[threadIdx.x] = isc[i0 + threadIdx.x]; }
+    if (threadIdx.x < 2 * kBalB) {
+        // summed in workgroup order, 16 coherent loads in flight
+        const int t = threadIdx.x;
+        TJ acc = 0;
+        int q = 0;
+        for (; q + 16 <= G; q += 16) {
+            TJ v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) v[u] = ld_coherent(&part[(int64_t)(q + u) * 2 * kBalB + t]);
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc += v[u];
+        }
+        for (; q < G; q++) acc += ld_coherent(&part[(int64_t)q * 2 * kBalB + t]);
+        sp[t] = acc;
+    }
     __syncthreads();
     if (wv != 0) return;
     TJ cp = 0, rp = 0;
-    if (lane < B) {
-        for (int q = 0; q < G; q++) {
-            cp += ld_coherent(&part[(int64_t)q * 2 * kBalB + lane]);
-            rp += ld_coherent(&part[(int64_t)q * 2 * kBalB + kBalB + lane]);
-        }
-    }
+    if (lane < B) { cp = sp[lane]; rp = sp[kBalB + lane]; }
     int any = 0;
     for (int u = 0; u < B; u++) {
         TJ tc = 0, tr = 0;
@@ -118,9 +152,9 @@ __global__ void __launch_bounds__(256) k_bal_block(const TJ *__restrict__ J, int
             tc = dabs(blk[lane][u]) * (TJ)si[lane];
             tr = dabs(blk[u][lane]) * (TJ)sd[lane];
         }
-        tc = wave_sum(tc);
-        tr = wave_sum(tr);
-        const TJ cpu = __shfl(cp, u, 64), rpu = __shfl(rp, u, 64);
+        tc = wave_sum_fast(tc);
+        tr = wave_sum_fast(tr);
+        const TJ cpu = wsum_detail::lane_read(cp, u), rpu = wsum_detail::lane_read(rp, u);
         TJ c = (TJ)sd[u] * (cpu + tc);
         TJ r = (TJ)si[u] * (rpu + tr);
         if (c == 0 || r == 0) continue;
@@ -675,9 +709,9 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
 #pragma unroll
         for (int q = 0; q < RPW; q++) {
             const int w = ps * RPW + q;
-            const T s = wave_sum(acc[q]);
+            const T s = wave_sum_fast(acc[q]);
             const int lnl = (nl[w] - j0) & 63;
-            const T a_nl = __shfl(cn[q], lnl, 64);
+            const T a_nl = wsum_detail::lane_read(cn[q], lnl);
             T t = t0[w];
             if (act[w] && pk != (T)0) t = t - lq[w] * pk;
             if (!skip) t = t + s;
