@@ -434,7 +434,11 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         for (auto &v : cb) v -= c0;
         st = colbeg.ensure(sizeof(int64_t) * cb.size());
         if (!st.ok()) break;
-        hipError_t e = hipMemcpy(colbeg.p, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice);
+        // hipMemcpy runs on the null stream, which does not wait for the
+        // handle's non-blocking stream: drain the previous chunk's solves
+        // (which read colbeg) before overwriting it
+        hipError_t e = s0 > 0 ? hipStreamSynchronize(h->stream) : hipSuccess;
+        if (e == hipSuccess) e = hipMemcpy(colbeg.p, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
         const int64_t nc = c1 - c0;
         if (nc == 0) continue;
@@ -463,7 +467,9 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.accumulate = 0;
         st = solve_device(h, a);
         if (!st.ok()) break;
-        e = hipMemcpy(hinfo.data() + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost);
+        // the shift kernel wrote dinfo on the handle's stream: wait for it
+        e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(hinfo.data() + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost);
         if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
         if (precision == LHL_PREC_REFINE64) {
             st = rst.ensure(8 * (size_t)plane * planes + 64);

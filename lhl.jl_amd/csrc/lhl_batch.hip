@@ -862,7 +862,9 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         for (size_t j = 0; j < per[x].size(); j++) ug[j * kXcd + x] = per[x][j];
     LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
     UGroup *dg = h->gbuf.as<UGroup>();
-    LHL_HIP_TRY(hipMemcpy(dg, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice));
+    // ordered on the handle's stream: an earlier solve's back substitution may
+    // still be reading the table (the host vector is staged before the call returns)
+    LHL_HIP_TRY(hipMemcpyAsync(dg, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice, h->stream));
 
     const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
     LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),

@@ -350,6 +350,25 @@ def test_config2_stiff_fp64(lhl_hip, oracle, n):
         assert np.linalg.norm(b - W @ x, np.inf) / np.linalg.norm(b, np.inf) <= 1e-10
 
 
+def test_batched_reports_singular_shifts(lhl_hip):
+    # J diagonal with integer entries: H = J exactly, so sigma = J[i,i],
+    # tau = -1 makes pivot i of the shift LU exactly zero (info = i + 1, src:2450).
+    # The batch must read each shift's info after its LU has run.
+    n = 64
+    J = np.diag(np.arange(1.0, n + 1))
+    ws = lhl_hip.lhl(J, balance=False)
+    sig = np.array([1.7, 5.0, 2.0, 40.0, 0.5])
+    tau = np.array([-0.5, -1.0, -0.13, -1.0, 1.0])
+    B = np.asfortranarray(np.random.default_rng(2).standard_normal((n, 5)))
+    _, info, _ = lhl_hip.shift_solve_batched(ws, sig, tau, B, np.arange(6))
+    single = []
+    for s, t in zip(sig, tau):
+        lhl_hip.lhl_shift_(ws, s, t)
+        single.append(ws.info)
+    assert list(info) == single
+    assert single == [0, 5, 0, 40, 0]
+
+
 def test_batched_matches_single(lhl_hip):
     n = 300
     J = stiff(11, n)
