@@ -409,7 +409,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     const int planes = cplx ? 2 : 1;
     // stage B and X (FP64 planar)
     int64_t ld, plane, ldb2, planeb;
-    DevBuf bst, rst, norms, colshift, colbeg;
+    DevBuf bst, rst, norms, colshift, colbeg, colsub;
     const double *Bc = B + (on_device ? 0 : 0);
     st = stage_in(h, bst, Bc + (size_t)rhs_offsets[0] * ldb * planes, n, ncol, ldb, LHL_F64, cplx, on_device, LHL_F64,
                   ldb2, planeb);
@@ -495,25 +495,58 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
             ra.xdt = LHL_F64;
             ra.x_complex = cplx;
             ra.norms = norms.as<double>();
+            // Each round works on the shifts from the first to the last one
+            // with a column above tol; the others keep their converged x (and
+            // residual).  U, l, rdiag, swap flags, Uinv and UT sit at fixed
+            // per-shift strides, so a range of shifts is a pointer offset.
+            const size_t es = elt_size(h->dtype);
             std::vector<double> hn((size_t)2 * nc);
-            std::vector<char> conv((size_t)cnt, 0);
+            int64_t lo = 0, hi = cnt;   // active shifts [lo, hi) of the chunk
             for (int it = 0; it < max_iter; it++) {
-                st = residual_device(h, ra);
+                const int64_t clo = cb[lo], na = cb[hi] - clo;
+                ResidArgs rr = ra;
+                rr.colshift = ra.colshift + 4 * clo;
+                rr.b = (const double *)ra.b + clo * ld;
+                rr.x = (const double *)ra.x + clo * ld;
+                rr.r = (double *)ra.r + clo * ld;
+                rr.ncol = na;
+                st = residual_device(h, rr);
                 if (!st.ok()) break;
-                e = hipMemcpyAsync(hn.data(), norms.p, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, h->stream);
+                e = hipMemcpyAsync(hn.data(), norms.p, sizeof(double) * 2 * na, hipMemcpyDeviceToHost, h->stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
                 if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
-                bool all = true;
-                for (int64_t s = 0; s < cnt; s++) {
+                int64_t nlo = -1, nhi = -1;
+                for (int64_t s = lo; s < hi; s++) {
                     bool ok = true;
                     for (int64_t c = cb[s]; c < cb[s + 1]; c++)
-                        if (!(hn[c] <= tol * hn[nc + c])) ok = false;
-                    if (!ok) { all = false; if (iters) iters[s0 + s] = it + 1; }
+                        if (!(hn[c - clo] <= tol * hn[na + c - clo])) ok = false;
+                    if (!ok) {
+                        if (nlo < 0) nlo = s;
+                        nhi = s + 1;
+                        if (iters) iters[s0 + s] = it + 1;
+                    }
                 }
-                if (all) break;
+                if (nlo < 0) break;
+                lo = nlo;
+                hi = nhi;
+                std::vector<int64_t> cbr((size_t)(hi - lo + 1));
+                for (int64_t s = lo; s <= hi; s++) cbr[s - lo] = cb[s] - cb[lo];
+                st = colsub.ensure(sizeof(int64_t) * cbr.size());
+                if (!st.ok()) break;
+                e = hipMemcpy(colsub.p, cbr.data(), sizeof(int64_t) * cbr.size(), hipMemcpyHostToDevice);   // stream idle
+                if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
                 SolveArgs r = a;
-                r.xin = ra.r;
-                r.xout = h->xstage.as<double>() + c0 * ld;
+                r.U = (const char *)a.U + lo * a.ustride * es;
+                r.UT = a.UT ? (const void *)((const char *)a.UT + lo * a.ustride * es) : nullptr;
+                r.l = (const char *)a.l + lo * a.lstride * es;
+                r.rdiag = (const char *)a.rdiag + lo * a.lstride * es;
+                r.Uinv = a.Uinv ? (const void *)((const char *)a.Uinv + lo * a.uistride * es) : nullptr;
+                r.swp = a.swp + lo * a.sstride;
+                r.nshift = hi - lo;
+                r.col_begin = colsub.as<int64_t>();
+                r.ncol = cb[hi] - cb[lo];
+                r.xin = (const double *)ra.r + cb[lo] * ld;
+                r.xout = h->xstage.as<double>() + (c0 + cb[lo]) * ld;
                 r.accumulate = 1;
                 st = solve_device(h, r);
                 if (!st.ok()) break;
@@ -527,7 +560,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
-    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
+    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &colsub, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
                       &pool.params, &pool.Uinv, &pool.UT})
         b->release();
     if (info) for (int64_t s = 0; s < nshift; s++) info[s] = hinfo[s];
