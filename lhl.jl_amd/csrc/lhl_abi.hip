@@ -167,6 +167,7 @@ int lhl_destroy(lhl_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     prof_drain(h);
     blas_release(h);
+    h->cache.clear();
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
                       &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->Linv, &h->Lfull, &h->ybuf, &h->rbuf, &h->nbuf, &h->gbuf,
                       &h->xstage, &h->bstage})
@@ -410,6 +411,11 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     // stage B and X (FP64 planar)
     int64_t ld, plane, ldb2, planeb;
     DevBuf bst, rst, norms, colshift, colbeg, colsub;
+    ShiftPool pool;
+    // per-call buffers come from (and go back to) the handle's block cache
+    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &colsub, &pool.U, &pool.l, &pool.rdiag, &pool.swp,
+                      &pool.dinfo, &pool.params, &pool.Uinv, &pool.UT})
+        b->cache = &h->cache;
     const double *Bc = B + (on_device ? 0 : 0);
     st = stage_in(h, bst, Bc + (size_t)rhs_offsets[0] * ldb * planes, n, ncol, ldb, LHL_F64, cplx, on_device, LHL_F64,
                   ldb2, planeb);
@@ -421,7 +427,6 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     // per-chunk pools: bound the U storage to about 8 GiB
     const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
     int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nshift, (int64_t)((8ull << 30) / std::max<size_t>(ubytes, 1))));
-    ShiftPool pool;
     std::vector<int> hinfo((size_t)nshift, 0);
     if (iters) for (int64_t s = 0; s < nshift; s++) iters[s] = 0;
     for (int64_t s0 = 0; s0 < nshift && st.ok(); s0 += chunk) {
@@ -556,9 +561,10 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     }
     if (st.ok()) st = stage_out(h, h->xstage, ld, plane, LHL_F64, X + (size_t)rhs_offsets[0] * ldx * planes, n, ncol, ldx,
                                 LHL_F64, cplx, on_device);
-    if (st.ok()) {
+    {
+        // the blocks return to the cache only after the stream has drained
         hipError_t e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+        if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
     for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &colsub, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
                       &pool.params, &pool.Uinv, &pool.UT})

@@ -127,20 +127,74 @@ inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
 // elements at the end (the shift kernel's dump slots, one per thread), rounded to 64.
 inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 63) / 64) * 64; }
 
-// Growable device allocation.
+// Device blocks kept by a handle between calls (per-call buffers of the
+// batched solves: the shift pool is gigabytes, and hipMalloc/hipFree of it on
+// every call costs milliseconds).  A block is handed back only after the
+// handle's stream has drained, so a later call on the same stream may reuse it.
+struct BlockCache {
+    std::vector<std::pair<void *, size_t>> blocks;
+    size_t bytes = 0;
+    static constexpr size_t kCap = size_t(48) << 30;   // beyond this, give memory back to the device
+    // the smallest cached block of at least `want` bytes, or nullptr
+    void *take(size_t want, size_t *got) {
+        size_t best = blocks.size();
+        for (size_t i = 0; i < blocks.size(); i++)
+            if (blocks[i].second >= want && (best == blocks.size() || blocks[i].second < blocks[best].second)) best = i;
+        if (best == blocks.size()) return nullptr;
+        void *p = blocks[best].first;
+        *got = blocks[best].second;
+        bytes -= *got;
+        blocks.erase(blocks.begin() + (long)best);
+        return p;
+    }
+    void give(void *p, size_t n) {
+        blocks.push_back({p, n});
+        bytes += n;
+        while (bytes > kCap && !blocks.empty()) {   // oldest first
+            (void)hipFree(blocks.front().first);
+            bytes -= blocks.front().second;
+            blocks.erase(blocks.begin());
+        }
+    }
+    void clear() {
+        for (auto &b : blocks) (void)hipFree(b.first);
+        blocks.clear();
+        bytes = 0;
+    }
+};
+
+// Growable device allocation.  With `cache` set, blocks come from and return
+// to that cache instead of hipMalloc/hipFree.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    BlockCache *cache = nullptr;
     Status ensure(size_t want) {
         if (want <= bytes && p) return {};
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        release();
         if (want == 0) want = 256;
+        if (cache) {
+            size_t got = 0;
+            if ((p = cache->take(want, &got)) != nullptr) { bytes = got; return {}; }
+        }
         hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) return {LHL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+        if (e != hipSuccess && cache && !cache->blocks.empty()) {   // out of memory: return the cache and retry
+            (void)hipGetLastError();
+            cache->clear();
+            e = hipMalloc(&p, want);
+        }
+        if (e != hipSuccess) { p = nullptr; return {LHL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e)}; }
         bytes = want;
         return {};
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    void release() {
+        if (p) {
+            if (cache) cache->give(p, bytes);
+            else (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
@@ -200,6 +254,7 @@ struct lhl_handle {
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
+    lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
     int64_t ldlf = 0;
