@@ -410,11 +410,11 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     const int planes = cplx ? 2 : 1;
     // stage B and X (FP64 planar)
     int64_t ld, plane, ldb2, planeb;
-    DevBuf bst, rst, norms, colshift, colbeg, colsub;
+    DevBuf bst, rst, norms, colshift;
     ShiftPool pool;
     // per-call buffers come from (and go back to) the handle's block cache
-    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &colsub, &pool.U, &pool.l, &pool.rdiag, &pool.swp,
-                      &pool.dinfo, &pool.params, &pool.Uinv, &pool.UT})
+    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
+                      &pool.params, &pool.Uinv, &pool.UT})
         b->cache = &h->cache;
     const double *Bc = B + (on_device ? 0 : 0);
     st = stage_in(h, bst, Bc + (size_t)rhs_offsets[0] * ldb * planes, n, ncol, ldb, LHL_F64, cplx, on_device, LHL_F64,
@@ -437,14 +437,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         for (int64_t s = 0; s <= cnt; s++) cb[s] = rhs_offsets[s0 + s] - rhs_offsets[0];
         const int64_t c0 = cb[0], c1 = cb[cnt];
         for (auto &v : cb) v -= c0;
-        st = colbeg.ensure(sizeof(int64_t) * cb.size());
-        if (!st.ok()) break;
-        // hipMemcpy runs on the null stream, which does not wait for the
-        // handle's non-blocking stream: drain the previous chunk's solves
-        // (which read colbeg) before overwriting it
-        hipError_t e = s0 > 0 ? hipStreamSynchronize(h->stream) : hipSuccess;
-        if (e == hipSuccess) e = hipMemcpy(colbeg.p, cb.data(), sizeof(int64_t) * cb.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        hipError_t e = hipSuccess;
         const int64_t nc = c1 - c0;
         if (nc == 0) continue;
         SolveArgs a;
@@ -461,7 +454,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.sstride = lplane;
         a.shift_complex = cplx;
         a.nshift = cnt;
-        a.col_begin = colbeg.as<int64_t>();
+        a.col_begin = cb.data();
         a.ncol = nc;
         a.x_complex = cplx;
         a.xdt = LHL_F64;
@@ -536,10 +529,6 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                 hi = nhi;
                 std::vector<int64_t> cbr((size_t)(hi - lo + 1));
                 for (int64_t s = lo; s <= hi; s++) cbr[s - lo] = cb[s] - cb[lo];
-                st = colsub.ensure(sizeof(int64_t) * cbr.size());
-                if (!st.ok()) break;
-                e = hipMemcpy(colsub.p, cbr.data(), sizeof(int64_t) * cbr.size(), hipMemcpyHostToDevice);   // stream idle
-                if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
                 SolveArgs r = a;
                 r.U = (const char *)a.U + lo * a.ustride * es;
                 r.UT = a.UT ? (const void *)((const char *)a.UT + lo * a.ustride * es) : nullptr;
@@ -548,7 +537,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                 r.Uinv = a.Uinv ? (const void *)((const char *)a.Uinv + lo * a.uistride * es) : nullptr;
                 r.swp = a.swp + lo * a.sstride;
                 r.nshift = hi - lo;
-                r.col_begin = colsub.as<int64_t>();
+                r.col_begin = cbr.data();
                 r.ncol = cb[hi] - cb[lo];
                 r.xin = (const double *)ra.r + cb[lo] * ld;
                 r.xout = h->xstage.as<double>() + (c0 + cb[lo]) * ld;
@@ -566,7 +555,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         hipError_t e = hipStreamSynchronize(h->stream);
         if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
-    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &colbeg, &colsub, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
+    for (DevBuf *b : {&bst, &rst, &norms, &colshift, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
                       &pool.params, &pool.Uinv, &pool.UT})
         b->release();
     if (info) for (int64_t s = 0; s < nshift; s++) info[s] = hinfo[s];

@@ -22,6 +22,8 @@
 
 #include <type_traits>
 #include "lhl_common.h"
+
+#include <cstring>
 #include "lhl_mfma.h"
 
 namespace lhl {
@@ -840,7 +842,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     // the back substitution's column groups (within one shift)
     std::vector<int64_t> cb((size_t)a.nshift + 1);
     if (a.col_begin) {
-        LHL_HIP_TRY(hipMemcpy(cb.data(), a.col_begin, sizeof(int64_t) * (a.nshift + 1), hipMemcpyDeviceToHost));
+        for (int64_t s = 0; s <= a.nshift; s++) cb[s] = a.col_begin[s];
     } else {
         cb[0] = 0;
         cb[1] = a.ncol;
@@ -863,8 +865,10 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
     UGroup *dg = h->gbuf.as<UGroup>();
     // ordered on the handle's stream: an earlier solve's back substitution may
-    // still be reading the table (the host vector is staged before the call returns)
-    LHL_HIP_TRY(hipMemcpyAsync(dg, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice, h->stream));
+    // still be reading the table
+    h->host_ug.resize(sizeof(UGroup) * ug.size());
+    std::memcpy(h->host_ug.data(), ug.data(), h->host_ug.size());
+    LHL_HIP_TRY(hipMemcpyAsync(dg, h->host_ug.data(), h->host_ug.size(), hipMemcpyHostToDevice, h->stream));
 
     const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
     LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),

@@ -255,6 +255,9 @@ struct lhl_handle {
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
+    // host sources of the solves' work tables: copied to the device on the
+    // stream, kept until the next solve (every caller synchronizes between solves)
+    std::vector<unsigned char> host_ug, host_hg;
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
     int64_t ldlf = 0;
@@ -319,7 +322,7 @@ struct SolveArgs {
     int64_t sstride = 0;          // per shift, swap bytes
     int shift_complex = 0;
     int64_t nshift = 1;
-    const int64_t *col_begin = nullptr;   // device: nshift+1 column offsets (nullptr: one shift, all cols)
+    const int64_t *col_begin = nullptr;   // host: nshift+1 column offsets (nullptr: one shift, all cols)
     int64_t ncol = 0;             // total right-hand-side columns
     int x_complex = 0;            // columns hold complex values (planar in device memory)
     int xdt = LHL_F64;            // element type of x in memory

@@ -16,6 +16,8 @@
 // The multipliers and U come from the packed per-shift storage.
 #include "lhl_common.h"
 
+#include <cstring>
+
 namespace lhl {
 namespace {
 
@@ -378,7 +380,7 @@ Status hess_typed(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
     std::vector<Group> hg;
     std::vector<int64_t> cb((size_t)a.nshift + 1);
     if (a.col_begin) {
-        LHL_HIP_TRY(hipMemcpy(cb.data(), a.col_begin, sizeof(int64_t) * (a.nshift + 1), hipMemcpyDeviceToHost));
+        for (int64_t s = 0; s <= a.nshift; s++) cb[s] = a.col_begin[s];
     } else {
         cb[0] = 0;
         cb[1] = a.ncol;
@@ -388,7 +390,9 @@ Status hess_typed(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
             hg.push_back(Group{(int)s, (int)c, (int)std::min<int64_t>(nr, cb[s + 1] - c), 0});
     if (hg.empty()) return {};
     LHL_TRY(h->nbuf.ensure(sizeof(Group) * hg.size()));
-    LHL_HIP_TRY(hipMemcpyAsync(h->nbuf.p, hg.data(), sizeof(Group) * hg.size(), hipMemcpyHostToDevice, h->stream));
+    h->host_hg.resize(sizeof(Group) * hg.size());
+    std::memcpy(h->host_hg.data(), hg.data(), h->host_hg.size());
+    LHL_HIP_TRY(hipMemcpyAsync(h->nbuf.p, h->host_hg.data(), h->host_hg.size(), hipMemcpyHostToDevice, h->stream));
     const Group *dg = h->nbuf.as<Group>();
     const int ng = (int)hg.size();
     Status st;
@@ -408,10 +412,7 @@ Status hess_typed(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
         else if (nr >= 2) st = launch_hess_t<T, 2, 1, 0>(h, a, dg, ng, Y, ldy);
         else st = launch_hess_t<T, 1, 1, 0>(h, a, dg, ng, Y, ldy);
     }
-    if (!st.ok()) return st;
-    // the group table must stay valid until the kernel has read it
-    LHL_HIP_TRY(hipStreamSynchronize(h->stream));
-    return {};
+    return st;
 }
 
 template <typename T, typename TX>
