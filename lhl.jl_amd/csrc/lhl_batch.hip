@@ -467,11 +467,10 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
 // is read along U's packed rows and written along columns through LDS (both
 // sides contiguous).  One workgroup per tile (blockIdx.x, upper-triangle
 // index) and shift-plane (blockIdx.y = shift * planes + plane).
-__host__ __device__ inline int64_t ucol_off(int64_t j) { return j * (j + 1) / 2; }
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U, T *__restrict__ UT, int64_t ustride,
                                                  int planes) {
+    constexpr int V = ut_vec<T>();
     // tile t of the upper triangle, column-block major: J = the largest with
     // J(J+1)/2 <= t, I = t - J(J+1)/2 <= J
     const int t = blockIdx.x;
@@ -493,66 +492,80 @@ __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U,
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int j = j0 + ty + 4 * r, i = i0 + tx;
-        if (j < n && i <= j) Ut[ucol_off(j) + i] = tile[tx][ty + 4 * r];
+        if (j < n && i <= j) Ut[ucol_off<V>(j) + i] = tile[tx][ty + 4 * r];
     }
 }
 
 // x <- U_s^-1 y for kNC internal columns of one shift from U in packed
-// columns, right-looking over 64-row blocks from the bottom.  Thread t keeps
-// the right-hand sides of rows t + 512 j in registers.  Per block: its owner
-// wave publishes the block's rows, the 64x64 diagonal inverse gives x_b, and
-// every thread subtracts U[i, block] x_b from its rows above the block (lanes
-// run down a column of U: contiguous loads, MB of them in flight per thread).
-// Complex shifts: columns alternate (re, im), U is planar.  n <= 512 * NPER.
-template <typename T, int UC, int NPER, int NT>
+// columns, right-looking over 64-row blocks from the bottom.  Rows come in
+// groups of V consecutive rows (V = ut_vec<T>()); thread t keeps the
+// right-hand sides of groups t + NT j (j < NG) in registers.  Per block: its
+// owner threads publish the block's rows, the 64x64 diagonal inverse gives
+// x_b, and every thread subtracts U[i, block] x_b from its groups above the
+// block: one aligned V-wide load per group and column (lanes run down a
+// column of U, contiguous), MB columns per batch, the next batch in flight.
+// Complex shifts: columns alternate (re, im), U is planar.  n <= V NT NG.
+template <typename T, int UC, int NG, int NT>
 __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTbase, int64_t ustride,
                                                  const T *__restrict__ Uinv, int64_t uistride,
                                                  const UGroup *__restrict__ groups, const T *__restrict__ Yin,
                                                  T *__restrict__ Yf, int64_t ldy) {
+    constexpr int V = ut_vec<T>();
+    using VT = T __attribute__((ext_vector_type(V)));
     constexpr int UP = UC ? 2 : 1;
     constexpr int NPRE = (kT * kT * UP + NT - 1) / NT;
-    constexpr int MB = 16;             // column loads per batch (two planes when UC)
+    // columns per batch: 16 one-element loads (V = 1), or 8 (4 with two planes)
+    // V-wide loads
+    constexpr int MB = V == 1 ? 16 : (UC ? 4 : 8);
     constexpr int NB = kT / MB;        // batches per 64-column block
+    constexpr int GB = kT / V;         // groups per 64-row block
     const UGroup g = groups[blockIdx.x];
     if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
     const int nbu = (n + kT - 1) / kT;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
     const T *Ut = UTbase + (int64_t)g.shift * ustride;
     const int64_t up = ustride / 2;
     __shared__ T Ys[kT][kNC];
     __shared__ T Xb[kT][kNC];
     __shared__ T UiS[UP][kT][kT + 1];   // UiS[p][k][i] = Uinv_b(i, k), plane p
-    T yv[NPER][kNC];
+    T yv[NG][V][kNC];
 #pragma unroll
-    for (int j = 0; j < NPER; j++) {
-        const int i = tid + NT * j;
+    for (int j = 0; j < NG; j++)
 #pragma unroll
-        for (int c = 0; c < kNC; c++) {
-            const T v = Yin[min(i, n - 1) + (int64_t)(g.q0 + min(c, g.nq - 1)) * ldy];
-            yv[j][c] = (i < n && c < g.nq) ? v : (T)0;
+        for (int r = 0; r < V; r++) {
+            const int i = V * (tid + NT * j) + r;
+#pragma unroll
+            for (int c = 0; c < kNC; c++) {
+                const T v = Yin[min(i, n - 1) + (int64_t)(g.q0 + min(c, g.nq - 1)) * ldy];
+                yv[j][r][c] = (i < n && c < g.nq) ? v : (T)0;
+            }
         }
-    }
     const int di = tid >> 2, dc = tid & 3;
     for (int b = nbu - 1; b >= 0; b--) {
         const int rb = b * kT, nbk = min(kT, n - rb);
+        const int gb = rb / V;                 // the block's first group; groups < gb are above it
         T pre[NPRE];
         {
             const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
 #pragma unroll
             for (int u = 0; u < NPRE; u++) pre[u] = Ui[min(tid + NT * u, kT * kT * UP - 1)];
         }
-        // 1. the block's rows (slot rb / NT of wave (rb % NT) / 64) into LDS
-        const int jb = rb / NT;
-        if (w == (rb % NT) / kT) {
-            T yb[kNC];
+        // 1. the block's rows: groups gb .. gb+GB-1, slot gb / NT of threads
+        //    gb % NT .. (one wave: GB divides 64 and gb % NT is a multiple of GB)
+        {
+            const int jb = gb / NT, tb = gb % NT;
+            if (tid >= tb && tid < tb + GB) {
+                const int m = tid - tb;
 #pragma unroll
-            for (int c = 0; c < kNC; c++) yb[c] = 0;
+                for (int r = 0; r < V; r++)
 #pragma unroll
-            for (int j = 0; j < NPER; j++)
+                    for (int c = 0; c < kNC; c++) {
+                        T yb = 0;
 #pragma unroll
-                for (int c = 0; c < kNC; c++) yb[c] = j == jb ? yv[j][c] : yb[c];
-#pragma unroll
-            for (int c = 0; c < kNC; c++) Ys[lane][c] = lane < nbk ? yb[c] : (T)0;
+                        for (int j = 0; j < NG; j++) yb = j == jb ? yv[j][r][c] : yb;
+                        Ys[V * m + r][c] = V * m + r < nbk ? yb : (T)0;
+                    }
+            }
         }
 #pragma unroll
         for (int u = 0; u < NPRE; u++) {
@@ -581,62 +594,77 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
             if (di < nbk && dc < g.nq) Yf[(rb + di) + (int64_t)(g.q0 + dc) * ldy] = x;
         }
         __syncthreads();
-        // 3. rows above the block: y_i -= U[i, rb:rb+nbk] x_b.  The work is a
+        // 3. groups above the block: y_i -= U[i, rb:rb+nbk] x_b.  The work is a
         // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
         // loads of batch t+1 are in flight while batch t's FMAs run.
-        const int nbat = NB * ((rb + NT - 1) / NT);   // slots holding rows < rb (nbat is even)
-        auto load_b = [&](int t, T(&ur)[MB], T(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
-            const int ic = min(tid + NT * (t / NB), n - 1);
+        const int nbat = NB * ((gb + NT - 1) / NT);   // slots holding groups < gb (nbat is even)
+        auto load_b = [&](int t, VT(&ur)[MB], VT(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
+            // threads whose group is not above the block read a valid one (unused)
+            const int gc = min(tid + NT * (t / NB), gb - 1);
             const int col0 = rb + (t % NB) * MB;
-            // one running offset: column col+1 starts col+1 elements after column
-            // col; past the last column it stays there (those x_b rows are zero)
-            const T *pr = Ut + ucol_off(min(col0, n - 1)) + ic;
+            // one running offset: column col+1 starts round_up(col+1, V)
+            // elements after column col; past the last column it stays there
+            // (those x_b rows are zero)
+            const T *pr = Ut + ucol_off<V>(min(col0, n - 1)) + V * gc;
 #pragma unroll
             for (int m = 0; m < MB; m++) {
-                ur[m] = pr[0];
-                if constexpr (UC) ui[m] = pr[up];
+                ur[m] = *reinterpret_cast<const VT *>(pr);
+                if constexpr (UC) ui[m] = *reinterpret_cast<const VT *>(pr + up);
                 const int col = col0 + m;
-                pr += col + 1 < n ? col + 1 : 0;
+                // branch-free (a branch here makes the compiler wait for every load)
+                pr += ((col + V) / V) * V & -(int)(col + 1 < n);
             }
         };
-        auto fma_b = [&](int t, const T(&ur)[MB], const T(&ui)[UC ? MB : 1], T(&acc)[kNC]) __attribute__((always_inline)) {
+        auto fma_b = [&](int t, const VT(&ur)[MB], const VT(&ui)[UC ? MB : 1], T(&acc)[V][kNC])
+                         __attribute__((always_inline)) {
             const int m0 = (t % NB) * MB;
 #pragma unroll
             for (int m = 0; m < MB; m++) {
                 // x_b is read from LDS in groups of 8 rows (hoisting all of it would spill)
                 if (m % 8 == 0) __builtin_amdgcn_sched_barrier(0);
                 const T *xb = Xb[m0 + m];   // zero beyond the block
-                if constexpr (UC) {
-                    acc[0] = fma_t(ur[m], xb[0], fma_t(-ui[m], xb[1], acc[0]));
-                    acc[1] = fma_t(ur[m], xb[1], fma_t(ui[m], xb[0], acc[1]));
-                    acc[2] = fma_t(ur[m], xb[2], fma_t(-ui[m], xb[3], acc[2]));
-                    acc[3] = fma_t(ur[m], xb[3], fma_t(ui[m], xb[2], acc[3]));
-                } else {
 #pragma unroll
-                    for (int c = 0; c < kNC; c++) acc[c] = fma_t(ur[m], xb[c], acc[c]);
+                for (int r = 0; r < V; r++) {
+                    const T u = ur[m][r];
+                    if constexpr (UC) {
+                        const T v = ui[m][r];
+                        acc[r][0] = fma_t(u, xb[0], fma_t(-v, xb[1], acc[r][0]));
+                        acc[r][1] = fma_t(u, xb[1], fma_t(v, xb[0], acc[r][1]));
+                        acc[r][2] = fma_t(u, xb[2], fma_t(-v, xb[3], acc[r][2]));
+                        acc[r][3] = fma_t(u, xb[3], fma_t(v, xb[2], acc[r][3]));
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < kNC; c++) acc[r][c] = fma_t(u, xb[c], acc[r][c]);
+                    }
                 }
             }
         };
-        // after the last batch of slot js: y -= acc for its rows above the block
-        auto apply = [&](int t, T(&acc)[kNC]) __attribute__((always_inline)) {
+        // after the last batch of slot js: y -= acc for its groups above the block
+        auto apply = [&](int t, T(&acc)[V][kNC]) __attribute__((always_inline)) {
             if (t % NB != NB - 1) return;
             const int js = t / NB;
             // a select on every slot: an `if (j == js)` chain would be folded
             // into a dynamic index of yv, which would move yv to scratch
 #pragma unroll
-            for (int j = 0; j < NPER; j++) {
-                const bool hit = j == js && tid + NT * j < rb;
+            for (int j = 0; j < NG; j++) {
+                const bool hit = j == js && tid + NT * j < gb;
 #pragma unroll
-                for (int c = 0; c < kNC; c++) yv[j][c] = hit ? yv[j][c] - acc[c] : yv[j][c];
+                for (int r = 0; r < V; r++)
+#pragma unroll
+                    for (int c = 0; c < kNC; c++) yv[j][r][c] = hit ? yv[j][r][c] - acc[r][c] : yv[j][r][c];
             }
 #pragma unroll
-            for (int c = 0; c < kNC; c++) acc[c] = 0;
+            for (int r = 0; r < V; r++)
+#pragma unroll
+                for (int c = 0; c < kNC; c++) acc[r][c] = 0;
         };
         if (nbat > 0) {
-            T uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
-            T acc[kNC];
+            VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
+            T acc[V][kNC];
 #pragma unroll
-            for (int c = 0; c < kNC; c++) acc[c] = 0;
+            for (int r = 0; r < V; r++)
+#pragma unroll
+                for (int c = 0; c < kNC; c++) acc[r][c] = 0;
             load_b(0, uA, iA);
             for (int t = 0; t < nbat; t += 2) {
                 __builtin_amdgcn_sched_barrier(0);
@@ -899,7 +927,9 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         using F = std::integral_constant<int, 0>;
         using C = std::integral_constant<int, 1>;
         Status st;
-        int nper = (n + 511) / 512;
+        // groups of V rows per thread
+        constexpr int VR = ut_vec<T>();
+        int nper = (n + VR * 512 - 1) / (VR * 512);
         if (a.shift_complex) {
             if (nper <= 1) st = go(C{}, std::integral_constant<int, 1>{});
             else if (nper <= 2) st = go(C{}, std::integral_constant<int, 2>{});
@@ -907,7 +937,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
             else if (nper <= 8) st = go(C{}, std::integral_constant<int, 8>{});
             else st = go(C{}, std::integral_constant<int, 16>{});
         } else {
-            nper = (n + NTR - 1) / NTR;
+            nper = (n + VR * NTR - 1) / (VR * NTR);
             if (nper <= 1) st = go(F{}, std::integral_constant<int, 1>{});
             else if (nper <= 2) st = go(F{}, std::integral_constant<int, 2>{});
             else if (nper <= 4) st = go(F{}, std::integral_constant<int, 4>{});

@@ -124,8 +124,20 @@ inline int64_t pad_ld(int64_t n, int dtype) {
 __host__ __device__ inline int64_t upack_off(int64_t n, int64_t k) { return k * n - (k * (k - 1)) / 2; }
 inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
 // Elements per plane of one shift's U: the packed triangle plus 1024 spare
-// elements at the end (the shift kernel's dump slots, one per thread), rounded to 64.
-inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 63) / 64) * 64; }
+// elements at the end (the shift kernel's dump slots, one per thread) and 4n
+// for the column padding of the same U in packed columns (below), rounded to 64.
+inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 4 * n + 63) / 64) * 64; }
+
+// U in packed columns for the back substitution: column j holds U[0:j+1, j]
+// from ucol_off<V>(j), a multiple of V, so V consecutive rows of a column are
+// one aligned 16-byte load (V = 4 for FP32; V = 1, no padding, for FP64).
+// Column j takes round_up(j+1, V) slots; the total over n columns is at most
+// n(n+1)/2 + (V-1) n.
+template <typename T> constexpr int ut_vec() { return sizeof(T) == 4 ? 4 : 1; }
+template <int V> __host__ __device__ inline int64_t ucol_off(int64_t j) {
+    const int64_t a = j / V, b = j % V;
+    return V * ((V * a * (a + 1)) / 2 + b * (a + 1));
+}
 
 // Device blocks kept by a handle between calls (per-call buffers of the
 // batched solves: the shift pool is gigabytes, and hipMalloc/hipFree of it on
