@@ -177,6 +177,12 @@ int lhl_destroy(lhl_handle *h) {
     };
     if (h->own) { drop(h->own); delete h->own; }
     for (lhl_shift_state *s : h->held) s->h = nullptr;   // held states outlive nothing: detach
+    if (h->stream2) {
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamDestroy(h->stream2);
+    }
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LHL_OK;
@@ -386,6 +392,14 @@ int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int6
     return LHL_OK;
 }
 
+// The second stream (default priority) and the fork/join events, created on first use.
+static Status ensure_stream2(lhl_handle *h) {
+    if (!h->stream2) LHL_HIP_TRY(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    if (!h->ev_fork) LHL_HIP_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    if (!h->ev_join) LHL_HIP_TRY(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    return {};
+}
+
 int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, const double *tau,
                             int shifts_complex, const int64_t *rhs_offsets, const double *B, int64_t ldb, double *X,
                             int64_t ldx, int on_device, int precision, double tol, int max_iter, int64_t *info,
@@ -431,27 +445,13 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     if (iters) for (int64_t s = 0; s < nshift; s++) iters[s] = 0;
     for (int64_t s0 = 0; s0 < nshift && st.ok(); s0 += chunk) {
         const int64_t cnt = std::min(chunk, nshift - s0);
-        st = shift_pool_device(h, pool, cnt, sigma + 2 * s0, tau + 2 * s0, cplx);
-        if (!st.ok()) break;
         std::vector<int64_t> cb((size_t)cnt + 1);
         for (int64_t s = 0; s <= cnt; s++) cb[s] = rhs_offsets[s0 + s] - rhs_offsets[0];
         const int64_t c0 = cb[0], c1 = cb[cnt];
         for (auto &v : cb) v -= c0;
         hipError_t e = hipSuccess;
         const int64_t nc = c1 - c0;
-        if (nc == 0) continue;
         SolveArgs a;
-        a.U = pool.U.p;
-        a.l = pool.l.p;
-        a.rdiag = pool.rdiag.p;
-        a.Uinv = pool.Uinv.p;
-        a.uistride = (((n + 63) / 64) * 64) * 64 * planes;
-        a.UT = pool.UT.p;
-        a.swp = pool.swp.as<uint8_t>();
-        const int64_t lplane = ((n + 63) / 64) * 64;
-        a.ustride = pool.ustride;
-        a.lstride = lplane * planes;
-        a.sstride = lplane;
         a.shift_complex = cplx;
         a.nshift = cnt;
         a.col_begin = cb.data();
@@ -463,12 +463,50 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.ldx = ld;
         a.xplane = plane;
         a.accumulate = 0;
-        st = solve_device(h, a);
+        if (nc > 0) {
+            // the first pass's gather and L solve read no shift state: they run
+            // on the second stream while the shift LUs run on the handle's
+            st = ensure_stream2(h);
+            if (!st.ok()) break;
+            e = hipEventRecord(h->ev_fork, h->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+            SolveArgs p1 = a;
+            p1.phase = 1;
+            hipStream_t keep = h->stream;
+            h->stream = h->stream2;
+            st = solve_device(h, p1);
+            h->stream = keep;
+            if (!st.ok()) break;
+            e = hipEventRecord(h->ev_join, h->stream2);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        }
+        st = shift_pool_device(h, pool, cnt, sigma + 2 * s0, tau + 2 * s0, cplx);
         if (!st.ok()) break;
+        a.U = pool.U.p;
+        a.l = pool.l.p;
+        a.rdiag = pool.rdiag.p;
+        a.Uinv = pool.Uinv.p;
+        a.uistride = (((n + 63) / 64) * 64) * 64 * planes;
+        a.UT = pool.UT.p;
+        a.swp = pool.swp.as<uint8_t>();
+        const int64_t lplane = ((n + 63) / 64) * 64;
+        a.ustride = pool.ustride;
+        a.lstride = lplane * planes;
+        a.sstride = lplane;
+        if (nc > 0) {
+            e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+            a.phase = 2;
+            st = solve_device(h, a);
+            if (!st.ok()) break;
+            a.phase = 0;
+        }
         // the shift kernel wrote dinfo on the handle's stream: wait for it
         e = hipStreamSynchronize(h->stream);
         if (e == hipSuccess) e = hipMemcpy(hinfo.data() + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost);
         if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        if (nc == 0) continue;
         if (precision == LHL_PREC_REFINE64) {
             st = rst.ensure(8 * (size_t)plane * planes + 64);
             if (st.ok()) st = norms.ensure(sizeof(double) * 2 * (size_t)nc);
@@ -551,8 +589,12 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     if (st.ok()) st = stage_out(h, h->xstage, ld, plane, LHL_F64, X + (size_t)rhs_offsets[0] * ldx * planes, n, ncol, ldx,
                                 LHL_F64, cplx, on_device);
     {
-        // the blocks return to the cache only after the stream has drained
+        // the blocks return to the cache only after both streams have drained
         hipError_t e = hipStreamSynchronize(h->stream);
+        if (h->stream2) {
+            const hipError_t e2 = hipStreamSynchronize(h->stream2);
+            if (e == hipSuccess) e = e2;
+        }
         if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
     for (DevBuf *b : {&bst, &rst, &norms, &colshift, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,

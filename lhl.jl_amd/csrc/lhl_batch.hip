@@ -808,6 +808,12 @@ Status blas_handle(lhl_handle *h, rocblas_handle *out) {
             return {LHL_ERR_HIP, "rocblas_set_stream failed"};
         }
         h->blas = b;
+        h->blas_stream = h->stream;
+    }
+    if (h->blas_stream != (void *)h->stream) {   // follows the handle's current stream
+        if (rocblas_set_stream((rocblas_handle)h->blas, h->stream) != rocblas_status_success)
+            return {LHL_ERR_HIP, "rocblas_set_stream failed"};
+        h->blas_stream = h->stream;
     }
     *out = (rocblas_handle)h->blas;
     return {};
@@ -867,6 +873,20 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
     T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
     const T *A = h->A.as<T>();
+    if (a.phase != 2) {
+        // y = P^-1 D^-1 x, then y' <- L'^-1 y' as a product with the explicit
+        // inverse formed by lhl() (final rows in Yo)
+        const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
+        LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
+                   (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
+                   ldy);
+        LHL_TRY(blas_trmm_unit_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf,
+                                        (const T *)Yw + 1, ldy, Yo + 1, ldy));
+        if (a.phase == 1) {
+            LHL_HIP_TRY(hipGetLastError());
+            return {};
+        }
+    }
     // the back substitution's column groups (within one shift)
     std::vector<int64_t> cb((size_t)a.nshift + 1);
     if (a.col_begin) {
@@ -898,15 +918,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     std::memcpy(h->host_ug.data(), ug.data(), h->host_ug.size());
     LHL_HIP_TRY(hipMemcpyAsync(dg, h->host_ug.data(), h->host_ug.size(), hipMemcpyHostToDevice, h->stream));
 
-    const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
-    LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
-               (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo, ldy);
-    // y' <- L'^-1 y'  (final rows in Yo)
     const int N1 = n - 1;
-    const int nbt = (N1 + kT - 1) / kT;
-    // y' <- L'^-1 y' as a product with the explicit inverse formed by lhl()
-    LHL_TRY(blas_trmm_unit_lower<T>(h, "trsm", N1, Q, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
-                                    Yo + 1, ldy));
     // forward interchange chain in place on Yo (columns resident in LDS)
     LHL_TRY(hess_device(h, a, Yo, ldy));
     // x <- U^-1 y per shift (solution in Yw)

@@ -259,6 +259,11 @@ struct lhl_handle {
     int shift_complex = 0;      // element type of ws.shift
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) and two events: the first solve pass's
+    // shift-independent part runs there while the shift LUs run on `stream`
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void *blas_stream = nullptr;    // the stream the rocBLAS handle is set to
     int64_t lda = 0;            // leading dimension of A and Ht
     int64_t ldj = 0;            // leading dimension of the resident J
     int j_dtype = LHL_F64;      // element type of the resident J
@@ -342,6 +347,9 @@ struct SolveArgs {
     void *xout = nullptr;
     int64_t ldx = 0, xplane = 0;
     int accumulate = 0;
+    // 0: the whole solve; 1: only the gather and the L solve (no shift state
+    // read); 2: the rest, after a phase-1 call on the same columns
+    int phase = 0;
 };
 Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
 // after a shift LU: the inverses of the 64x64 diagonal blocks of nshift packed
