@@ -683,6 +683,121 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
     }
 }
 
+// Forward interchange chain of the Hessenberg solve (src:2806-2816) as a scan.
+// Per column, with c the carried value (c = y[0] at the start), step k reads
+// b = y[k+1] and writes y[k] = (s_k ? b : c), then c <- (s_k ? c : b) - l_k y[k];
+// y[n-1] = c at the end.  Step k is the affine map c -> alpha c + beta with
+// (alpha, beta) = (1, -l_k b) when s_k, else (-l_k, b).  One wave per chain:
+// lane L takes steps [L bs, (L+1) bs), composes its block's map, the 64 maps
+// are scanned across the wave (Hillis-Steele with lane shuffles), and each
+// lane replays its block from its start value with the step formula above.
+// Partial pivoting keeps |l_k| <= 1, so the composed maps stay bounded.  The
+// column is updated in place: the only element one lane reads that another
+// writes is the first of the next block, read once before any write.  Real
+// shifts: one chain per internal column.  Complex shifts: columns (re, im)
+// pair up into one complex chain.
+template <typename T, int UC>
+__global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ lbase, int64_t lstride,
+                                                  const uint8_t *__restrict__ swbase, int64_t sstride,
+                                                  const UGroup *__restrict__ groups, T *__restrict__ Y, int64_t ldy) {
+#pragma clang fp contract(off)
+    constexpr int PL = UC ? 2 : 1;               // internal columns per chain
+    const UGroup g = groups[blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (PL * wv >= g.nq) return;                 // whole wave: no chain here
+    const int steps = n - 1;
+    const int bs = (steps + 63) / 64;
+    const T *lr = lbase + (int64_t)g.shift * lstride;
+    const T *li = lr + lstride / 2;              // imaginary parts (UC)
+    const uint8_t *sw = swbase + (int64_t)g.shift * sstride;
+    T *yr = Y + (int64_t)(g.q0 + PL * wv) * ldy;
+    T *yi = yr + ldy;                            // imaginary column (UC)
+    const int k0 = lane * bs, k1 = min(k0 + bs, steps);   // this lane's steps [k0, k1)
+    // 1. the block's composed map (A, B), and the next block's first element
+    T Ar = 1, Ai = 0, Br = 0, Bi = 0;
+#pragma unroll 4
+    for (int k = k0; k < k1; k++) {
+        const bool s = sw[k] != 0;
+        const T l = lr[k], b = yr[k + 1];
+        if constexpr (UC) {
+            const T lm = li[k], bim = yi[k + 1];
+            if (s) {   // A unchanged, B -= l b
+                Br = Br - (l * b - lm * bim);
+                Bi = Bi - (l * bim + lm * b);
+            } else {   // (A, B) <- (-l A, b - l B)
+                const T ar = -(l * Ar - lm * Ai), ai = -(l * Ai + lm * Ar);
+                const T nr = b - (l * Br - lm * Bi), ni = bim - (l * Bi + lm * Br);
+                Ar = ar; Ai = ai; Br = nr; Bi = ni;
+            }
+        } else {
+            if (s) {
+                Br = Br - l * b;
+            } else {
+                Ar = -l * Ar;
+                Br = b - l * Br;
+            }
+        }
+    }
+    const int kb = min(k0 + bs, steps);          // y[kb]: read by this lane's last step
+    const T bndr = yr[kb], bndi = UC ? yi[kb] : (T)0;
+    // 2. inclusive scan of the maps in lane order: (A, B) <- (A A', A B' + B)
+    //    with (A', B') the composition of the lanes before
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T pAr = __shfl_up(Ar, o, 64), pBr = __shfl_up(Br, o, 64);
+        const T pAi = UC ? __shfl_up(Ai, o, 64) : (T)0, pBi = UC ? __shfl_up(Bi, o, 64) : (T)0;
+        if (lane >= o) {
+            if constexpr (UC) {
+                const T nAr = Ar * pAr - Ai * pAi, nAi = Ar * pAi + Ai * pAr;
+                const T nBr = (Ar * pBr - Ai * pBi) + Br, nBi = (Ar * pBi + Ai * pBr) + Bi;
+                Ar = nAr; Ai = nAi; Br = nBr; Bi = nBi;
+            } else {
+                Br = Ar * pBr + Br;
+                Ar = Ar * pAr;
+            }
+        }
+    }
+    // the start value of this lane's block: the lanes before applied to y[0]
+    const T c0r = yr[0], c0i = UC ? yi[0] : (T)0;
+    T eAr = __shfl_up(Ar, 1, 64), eBr = __shfl_up(Br, 1, 64);
+    T eAi = UC ? __shfl_up(Ai, 1, 64) : (T)0, eBi = UC ? __shfl_up(Bi, 1, 64) : (T)0;
+    if (lane == 0) { eAr = 1; eAi = 0; eBr = 0; eBi = 0; }
+    T cr, ci = 0;
+    if constexpr (UC) {
+        cr = (eAr * c0r - eAi * c0i) + eBr;
+        ci = (eAr * c0i + eAi * c0r) + eBi;
+    } else {
+        cr = eAr * c0r + eBr;
+    }
+    __builtin_amdgcn_wave_barrier();   // every lane's reads above precede the writes below
+    // 3. replay the block with the step formula, writing y[k]
+    for (int k = k0; k < k1; k++) {
+        const bool s = sw[k] != 0;
+        const T l = lr[k];
+        const bool last = k + 1 == kb;
+        const T b = last ? bndr : yr[k + 1];
+        if constexpr (UC) {
+            const T lm = li[k];
+            const T bim = last ? bndi : yi[k + 1];
+            const T xr = s ? b : cr, xi = s ? bim : ci;
+            yr[k] = xr;
+            yi[k] = xi;
+            const T orr = s ? cr : b, oi = s ? ci : bim;
+            cr = orr - (l * xr - lm * xi);
+            ci = oi - (l * xi + lm * xr);
+        } else {
+            const T xk = s ? b : cr;
+            yr[k] = xk;
+            cr = (s ? cr : b) - l * xk;
+        }
+    }
+    // y[n-1]: the carried value after the last step (the lane whose block holds step n-2)
+    if (k0 < steps && k1 == steps) {
+        yr[n - 1] = cr;
+        if constexpr (UC) yi[n - 1] = ci;
+    }
+}
+
 // ---- the refinement residual: JX = J X (rocBLAS GEMM in one precision, the
 // tiled MFMA GEMM below for FP32 J with FP64 x), then
 // r = b - (sigma x + tau JX) per column with its shift, and column norms.
@@ -919,8 +1034,16 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_HIP_TRY(hipMemcpyAsync(dg, h->host_ug.data(), h->host_ug.size(), hipMemcpyHostToDevice, h->stream));
 
     const int N1 = n - 1;
-    // forward interchange chain in place on Yo (columns resident in LDS)
-    LHL_TRY(hess_device(h, a, Yo, ldy));
+    // forward interchange chain in place on Yo: a scan per column
+    if (n >= 2) {
+        if (a.shift_complex)
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 1>), dim3((unsigned)ug.size()), dim3(256), 0, n,
+                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy);
+        else
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0>), dim3((unsigned)ug.size()), dim3(256), 0, n,
+                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy);
+        LHL_HIP_TRY(hipGetLastError());
+    }
     // x <- U^-1 y per shift (solution in Yw)
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
