@@ -274,11 +274,11 @@ struct lhl_handle {
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     // host sources of the solves' work tables: copied to the device on the
     // stream, kept until the next solve (every caller synchronizes between solves)
-    std::vector<unsigned char> host_ug, host_hg;
+    std::vector<unsigned char> host_ug;
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
     int64_t ldlf = 0;
-    lhl::DevBuf ybuf, rbuf, nbuf, gbuf, xstage, bstage;  // solve / refine scratch
+    lhl::DevBuf ybuf, rbuf, gbuf, xstage, bstage;  // solve / refine scratch
     lhl::Profiler prof;
     void *blas = nullptr;       // rocBLAS handle on `stream` (the residual's plain GEMM), created on first use
     lhl_shift_state *own = nullptr;   // ws.shift
@@ -359,8 +359,6 @@ Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rd
                    int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT);   // lhl_batch.hip
 // the inverses of L's 64x64 diagonal blocks, formed once per reduction
 Status linv_device(lhl_handle *h);                          // lhl_batch.hip
-// the per-shift Hessenberg solves on the internal column layout Y (q = col*planes + pl)
-Status hess_device(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy);   // lhl_solve.hip
 Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane,
                      int xdt, int x_complex);              // lhl_solve.hip
 

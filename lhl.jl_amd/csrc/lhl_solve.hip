@@ -1,19 +1,6 @@
-// The per-shift Hessenberg solves of the batched solve (SciML/LHL.jl
-// _hessenberg_solve!, src:2746-2816, and its planar complex form
-// _hessenberg_solve_planar!, src:2823-2884), applyZ!/applyZinv! on a few
-// vectors (src:2069-2126), and the planar layout conversions.
-//
-// k_hess: one workgroup owns NR right-hand sides (PL planes each: 2 for
-// complex values) of one shift with the columns resident in LDS (the batched
-// solve runs it for the forward chain only; the back substitution of a batch
-// runs as blocked MFMA steps over all shifts at once):
-//   forward      the interchange/elimination chain, one lane per column, with
-//                the flags and multipliers staged through LDS;
-//   back subst.  right-looking over 32-row blocks from the bottom: the
-//                block's triangle of U is staged in LDS and solved by one wave
-//                with lane broadcasts, then every row above subtracts its
-//                contiguous 32-element U segment times the finished values.
-// The multipliers and U come from the packed per-shift storage.
+// applyZ!/applyZinv! on a few vectors (SciML/LHL.jl src:2069-2126) and the
+// planar layout conversions of the staged right-hand sides.  (The Hessenberg
+// solves of the batch are in lhl_batch.hip.)
 #include "lhl_common.h"
 
 #include <cstring>
@@ -23,8 +10,6 @@ namespace {
 
 constexpr int kSolveThreads = 256;
 constexpr int kBlk = 64;
-constexpr int kHB = 32;    // back-substitution block rows
-constexpr int kCH = 512;   // forward-chain staging chunk
 
 template <typename T> __device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, 64); }
 template <typename T> __device__ __forceinline__ T wave_sum(T v) {
@@ -32,216 +17,7 @@ template <typename T> __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-struct Group {
-    int shift;     // shift index
-    int col0;      // first column
-    int ncol;      // columns in this group (<= NR)
-    int pad;
-};
 
-template <typename T, int NR, int PL, int UC, bool BACK>
-__global__ void __launch_bounds__(kSolveThreads) k_hess(
-    int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restrict__ lbase, const T *__restrict__ rdbase,
-    int64_t lstride, const uint8_t *__restrict__ swbase, int64_t sstride, const Group *__restrict__ groups,
-    T *__restrict__ Yg, int64_t ldy) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *Y = reinterpret_cast<T *>(smem_raw);   // Y[v*n + i], v = r*PL + pl
-    constexpr int NV = NR * PL;
-    constexpr int UP = UC ? 2 : 1;            // planes of U, l, rdiag
-    __shared__ T tri[UP][kHB][kHB + 1];        // tri[.][r][c] = U[j0+r, j0+c], c >= r
-    __shared__ T trd[UP][kHB];
-    __shared__ T stl[UP][kCH];                 // staged multipliers of the forward chain
-    __shared__ uint8_t sts[kCH];               // staged interchange flags
-    const Group g = groups[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nc = g.ncol;
-    const T *U = Ubase + (int64_t)g.shift * ustride;
-    const T *lv = lbase + (int64_t)g.shift * lstride;
-    const T *rd = rdbase + (int64_t)g.shift * lstride;
-    const uint8_t *sw = swbase + (int64_t)g.shift * sstride;
-    const int64_t up = ustride / 2, lp = lstride / 2;   // imaginary planes (UC)
-    auto y = [&](int v, int i) -> T & { return Y[(int64_t)v * n + i]; };
-    auto gcol = [&](int v) -> T * { return Yg + (int64_t)(g.col0 * PL + v) * ldy; };
-
-    for (int i = tid; i < n; i += kSolveThreads)
-#pragma unroll
-        for (int v = 0; v < NV; v++) y(v, i) = (v / PL < nc) ? gcol(v)[i] : (T)0;
-    __syncthreads();
-
-    // ---- forward chain (src:2806-2816): one lane per right-hand side walks the
-    // interchange flags and multipliers, staged through LDS kCH at a time.
-    const bool chain = wv == 0 && lane < NR;
-    const int r = chain ? lane : 0;
-    T ca[PL];
-#pragma unroll
-    for (int pl = 0; pl < PL; pl++) ca[pl] = chain && n > 0 ? y(r * PL + pl, 0) : (T)0;
-    for (int c0 = 0; c0 < n - 1; c0 += kCH) {
-        const int c1 = min(c0 + kCH, n - 1);
-        for (int t = tid; t < c1 - c0; t += kSolveThreads) {
-            sts[t] = sw[c0 + t];
-            stl[0][t] = lv[c0 + t];
-            if constexpr (UC) stl[UP - 1][t] = lv[c0 + t + lp];
-        }
-        __syncthreads();
-        if (chain) {
-            for (int k = c0; k < c1; k++) {
-                const bool s = sts[k - c0] != 0;
-                if constexpr (UC) {
-                    const T br = y(r * PL, k + 1), bi = y(r * PL + 1, k + 1);
-                    const T xr = s ? br : ca[0], xi = s ? bi : ca[1];
-                    y(r * PL, k) = xr;
-                    y(r * PL + 1, k) = xi;
-                    const T lr = stl[0][k - c0], li = stl[UP - 1][k - c0];
-                    const T orr = s ? ca[0] : br, oi = s ? ca[1] : bi;
-                    ca[0] = orr - (lr * xr - li * xi);
-                    ca[1] = oi - (lr * xi + li * xr);
-                } else {
-                    const T l = stl[0][k - c0];
-#pragma unroll
-                    for (int pl = 0; pl < PL; pl++) {
-                        const T b = y(r * PL + pl, k + 1);
-                        const T xk = s ? b : ca[pl];
-                        y(r * PL + pl, k) = xk;
-                        ca[pl] = (s ? ca[pl] : b) - l * xk;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (chain && n > 0)
-#pragma unroll
-        for (int pl = 0; pl < PL; pl++) y(r * PL + pl, n - 1) = ca[pl];
-    __syncthreads();
-
-    if constexpr (!BACK) {
-        for (int i = tid; i < n; i += kSolveThreads)
-#pragma unroll
-            for (int v = 0; v < NV; v++)
-                if (v / PL < nc) gcol(v)[i] = y(v, i);
-        return;
-    }
-    // ---- back substitution, right-looking over kHB-row blocks from the bottom:
-    // solve the block's triangle from LDS, then every row above subtracts its
-    // contiguous U segment times the block's finished values.
-    for (int j1 = n; j1 > 0; j1 -= kHB) {
-        const int j0 = max(0, j1 - kHB);
-        const int nb = j1 - j0;
-        for (int t = tid; t < kHB * kHB; t += kSolveThreads) {
-            const int c = t % kHB, rr = t / kHB;
-            T v0 = 0, v1 = 0;
-            if (rr < nb && c < nb && c >= rr) {
-                const T *Ur = U + upack_off(n, j0 + rr) - (j0 + rr);
-                v0 = Ur[j0 + c];
-                if constexpr (UC) v1 = Ur[j0 + c + up];
-            }
-            tri[0][rr][c] = v0;
-            if constexpr (UC) tri[UP - 1][rr][c] = v1;
-        }
-        if (tid < nb) {
-            trd[0][tid] = rd[j0 + tid];
-            if constexpr (UC) trd[UP - 1][tid] = rd[j0 + tid + lp];
-        }
-        __syncthreads();
-        if (wv == 0) {
-            const bool own = lane < nb;
-            T v[NV];
-#pragma unroll
-            for (int q = 0; q < NV; q++) v[q] = own ? y(q, j0 + lane) : (T)0;
-            const T dr = own ? trd[0][lane] : (T)0;
-            T di = 0;
-            if constexpr (UC) di = own ? trd[UP - 1][lane] : (T)0;
-            for (int c = nb - 1; c >= 0; c--) {
-                if (lane == c) {
-                    if constexpr (UC) {
-#pragma unroll
-                        for (int rr = 0; rr < NR; rr++) {
-                            const T a = v[rr * PL], b = v[rr * PL + 1];
-                            v[rr * PL] = a * dr - b * di;
-                            v[rr * PL + 1] = a * di + b * dr;
-                        }
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < NV; q++) v[q] = v[q] * dr;
-                    }
-                }
-                const bool take = own && lane < c;
-                const T u = take ? tri[0][lane][c] : (T)0;
-                if constexpr (UC) {
-                    const T ui = take ? tri[UP - 1][lane][c] : (T)0;
-#pragma unroll
-                    for (int rr = 0; rr < NR; rr++) {
-                        const T xr = shfl(v[rr * PL], c), xi = shfl(v[rr * PL + 1], c);
-                        v[rr * PL] = v[rr * PL] - (u * xr - ui * xi);
-                        v[rr * PL + 1] = v[rr * PL + 1] - (u * xi + ui * xr);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < NV; q++) {
-                        const T xc = shfl(v[q], c);
-                        v[q] = fma_t(-u, xc, v[q]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NV; q++)
-                if (own) y(q, j0 + lane) = v[q];
-        }
-        __syncthreads();
-        if (j0 > 0) {
-            // here nb == kHB: only the top block is partial, and it has no rows above.
-            // RU rows per thread at a time, all their U segments loaded before use.
-            constexpr int RU = (UC ? 1 : 2) * (sizeof(T) == 4 ? 2 : 1);
-            for (int i0 = tid; i0 < j0; i0 += RU * kSolveThreads) {
-                T uu[RU][kHB], ui[RU][kHB];
-#pragma unroll
-                for (int ru = 0; ru < RU; ru++) {
-                    const int i = i0 + ru * kSolveThreads;
-                    const int ii = i < j0 ? i : 0;
-                    const T *Ur = U + upack_off(n, ii) - ii + j0;   // Ur[c] = U[ii, j0+c]
-#pragma unroll
-                    for (int c = 0; c < kHB; c++) {
-                        uu[ru][c] = Ur[c];
-                        if constexpr (UC) ui[ru][c] = Ur[c + up];
-                    }
-                }
-#pragma unroll
-                for (int ru = 0; ru < RU; ru++) {
-                    const int i = i0 + ru * kSolveThreads;
-                    T acc[NV];
-#pragma unroll
-                    for (int q = 0; q < NV; q++) acc[q] = 0;
-#pragma unroll
-                    for (int c = 0; c < kHB; c++) {
-                        if constexpr (UC) {
-#pragma unroll
-                            for (int rr = 0; rr < NR; rr++) {
-                                const T xr = y(rr * PL, j0 + c), xi = y(rr * PL + 1, j0 + c);
-                                acc[rr * PL] = fma_t(uu[ru][c], xr, acc[rr * PL]);
-                                acc[rr * PL] = fma_t(-ui[ru][c], xi, acc[rr * PL]);
-                                acc[rr * PL + 1] = fma_t(uu[ru][c], xi, acc[rr * PL + 1]);
-                                acc[rr * PL + 1] = fma_t(ui[ru][c], xr, acc[rr * PL + 1]);
-                            }
-                        } else {
-#pragma unroll
-                            for (int q = 0; q < NV; q++) acc[q] = fma_t(uu[ru][c], y(q, j0 + c), acc[q]);
-                        }
-                    }
-                    if (i < j0)
-#pragma unroll
-                        for (int q = 0; q < NV; q++) y(q, i) -= acc[q];
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int i = tid; i < n; i += kSolveThreads)
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-            if (v / PL < nc) gcol(v)[i] = y(v, i);
-}
-
-// applyZ / applyZinv on NR columns (planes independent).
 template <typename T, typename TX, int NV>
 __global__ void __launch_bounds__(kSolveThreads) k_applyz(int n, int inverse, const T *__restrict__ A, int64_t lda,
                                                           const int *__restrict__ perm, const int *__restrict__ iperm,
@@ -353,68 +129,6 @@ __global__ void k_from_planar(const TS *__restrict__ src, int64_t lds, int64_t p
 
 
 // Right-hand sides per workgroup: as many as fit in LDS, at most 4.
-inline int pick_nr(int64_t n, int planes, size_t es) {
-    const size_t budget = lds_limit() - 28 * 1024;   // static LDS of k_hess
-    const size_t per = (size_t)planes * (size_t)n * es;
-    int nr = per ? (int)(budget / per) : 4;
-    return nr >= 4 ? 4 : nr >= 2 ? 2 : nr;
-}
-
-template <typename T, int NR, int PL, int UC>
-Status launch_hess_t(lhl_handle *h, const SolveArgs &a, const Group *groups, int ngroups, void *Y, int64_t ldy) {
-    const int n = (int)h->n;
-    const size_t lds = sizeof(T) * (size_t)NR * PL * n;
-    auto fn = k_hess<T, NR, PL, UC, false>;   // the forward chain only; U is solved by the blocked kernels
-    LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    LHL_LAUNCH(h, "fwd_chain", fn, dim3(ngroups), dim3(kSolveThreads), lds, n, (const T *)a.U, a.ustride, (const T *)a.l,
-               (const T *)a.rdiag, a.lstride, a.swp, a.sstride, groups, (T *)Y, ldy);
-    LHL_HIP_TRY(hipGetLastError());
-    return {};
-}
-
-template <typename T>
-Status hess_typed(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
-    const int PLn = a.x_complex ? 2 : 1;
-    const int nr = pick_nr(h->n, PLn, sizeof(T));
-    if (nr < 1) return {LHL_ERR_ARG, "lhl_ldiv: n too large for the LDS-resident Hessenberg solve"};
-    std::vector<Group> hg;
-    std::vector<int64_t> cb((size_t)a.nshift + 1);
-    if (a.col_begin) {
-        for (int64_t s = 0; s <= a.nshift; s++) cb[s] = a.col_begin[s];
-    } else {
-        cb[0] = 0;
-        cb[1] = a.ncol;
-    }
-    for (int64_t s = 0; s < a.nshift; s++)
-        for (int64_t c = cb[s]; c < cb[s + 1]; c += nr)
-            hg.push_back(Group{(int)s, (int)c, (int)std::min<int64_t>(nr, cb[s + 1] - c), 0});
-    if (hg.empty()) return {};
-    LHL_TRY(h->nbuf.ensure(sizeof(Group) * hg.size()));
-    h->host_hg.resize(sizeof(Group) * hg.size());
-    std::memcpy(h->host_hg.data(), hg.data(), h->host_hg.size());
-    LHL_HIP_TRY(hipMemcpyAsync(h->nbuf.p, h->host_hg.data(), h->host_hg.size(), hipMemcpyHostToDevice, h->stream));
-    const Group *dg = h->nbuf.as<Group>();
-    const int ng = (int)hg.size();
-    Status st;
-    if (a.shift_complex) {
-        if (nr >= 8) st = launch_hess_t<T, 8, 2, 1>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 4) st = launch_hess_t<T, 4, 2, 1>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 2) st = launch_hess_t<T, 2, 2, 1>(h, a, dg, ng, Y, ldy);
-        else st = launch_hess_t<T, 1, 2, 1>(h, a, dg, ng, Y, ldy);
-    } else if (a.x_complex) {
-        if (nr >= 8) st = launch_hess_t<T, 8, 2, 0>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 4) st = launch_hess_t<T, 4, 2, 0>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 2) st = launch_hess_t<T, 2, 2, 0>(h, a, dg, ng, Y, ldy);
-        else st = launch_hess_t<T, 1, 2, 0>(h, a, dg, ng, Y, ldy);
-    } else {
-        if (nr >= 8) st = launch_hess_t<T, 8, 1, 0>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 4) st = launch_hess_t<T, 4, 1, 0>(h, a, dg, ng, Y, ldy);
-        else if (nr >= 2) st = launch_hess_t<T, 2, 1, 0>(h, a, dg, ng, Y, ldy);
-        else st = launch_hess_t<T, 1, 1, 0>(h, a, dg, ng, Y, ldy);
-    }
-    return st;
-}
-
 template <typename T, typename TX>
 Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane, int planes) {
     const int n = (int)h->n;
@@ -445,11 +159,6 @@ Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t l
 }
 
 }  // namespace
-
-Status hess_device(lhl_handle *h, const SolveArgs &a, void *Y, int64_t ldy) {
-    if (h->n == 0 || a.ncol == 0) return {};
-    return h->dtype == LHL_F64 ? hess_typed<double>(h, a, Y, ldy) : hess_typed<float>(h, a, Y, ldy);
-}
 
 Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane, int xdt,
                      int x_complex) {
