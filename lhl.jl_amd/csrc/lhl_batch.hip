@@ -957,25 +957,40 @@ Status blas_jx(lhl_handle *h, const T *W, int64_t ldw, const T *X, int64_t ld, i
     return {};
 }
 
-// C = (unit lower part of L) * B for m rows and ncol columns through rocBLAS:
-// the two triangular products of the solve are plain library operations.
+// C = L B for the unit lower-triangular L stored densely (zeros above the
+// diagonal, ones on it): one GEMM, faster on this chip than the triangular
+// product with half the flops.
 template <typename T>
-Status blas_trmm_unit_lower(lhl_handle *h, const char *name, int m, int ncol, const T *L, int64_t ldl, const T *B,
-                            int64_t ldb, T *C, int64_t ldc) {
+Status blas_gemm_lower(lhl_handle *h, const char *name, int m, int ncol, const T *L, int64_t ldl, const T *B,
+                       int64_t ldb, T *C, int64_t ldc) {
     if (m <= 0 || ncol <= 0) return {};
     rocblas_handle b;
     LHL_TRY(blas_handle(h, &b));
-    const T one = 1;
+    const T one = 1, zero = 0;
     ProfScope ps(h, name);
     rocblas_status st;
     if constexpr (std::is_same<T, double>::value)
-        st = rocblas_dtrmm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, m,
-                           ncol, &one, L, (int)ldl, B, (int)ldb, C, (int)ldc);
+        st = rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, m, ncol, m, &one, L, (int)ldl, B,
+                           (int)ldb, &zero, C, (int)ldc);
     else
-        st = rocblas_strmm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, m,
-                           ncol, &one, L, (int)ldl, B, (int)ldb, C, (int)ldc);
-    if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS TRMM of the solve failed"};
+        st = rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, m, ncol, m, &one, L, (int)ldl, B,
+                           (int)ldb, &zero, C, (int)ldc);
+    if (st != rocblas_status_success) return {LHL_ERR_HIP, "rocBLAS GEMM of the solve failed"};
     return {};
+}
+
+// Dense unit lower-triangular m x m matrix in dst: dst(i, c) = src(i, c) below
+// the diagonal, 1 on it, 0 above.  src == dst fixes the diagonal and upper part
+// in place.
+template <typename T>
+__global__ void k_unit_lower(int m, const T *__restrict__ src, int64_t lds, T *__restrict__ dst, int64_t ldd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (i >= m) return;
+    if (i > c) {
+        if (src != dst) dst[i + (int64_t)c * ldd] = src[i + (int64_t)c * lds];
+    } else {
+        dst[i + (int64_t)c * ldd] = i == c ? (T)1 : (T)0;
+    }
 }
 
 template <typename T, typename TX>
@@ -995,8 +1010,8 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
                    (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
                    ldy);
-        LHL_TRY(blas_trmm_unit_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf,
-                                        (const T *)Yw + 1, ldy, Yo + 1, ldy));
+        LHL_TRY(blas_gemm_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1,
+                                   ldy, Yo + 1, ldy));
         if (a.phase == 1) {
             LHL_HIP_TRY(hipGetLastError());
             return {};
@@ -1089,7 +1104,8 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     }
     LHL_HIP_TRY(hipGetLastError());
     // y_new' = (I + L') y' (Yo is free again), then x = D P y_new
-    LHL_TRY(blas_trmm_unit_lower<T>(h, "trmm", N1, Q, A + 1, h->lda, (const T *)Yw + 1, ldy, Yo + 1, ldy));
+    LHL_TRY(blas_gemm_lower<T>(h, "trmm", N1, Q, (const T *)h->Lp1.as<T>(), h->ldlf, (const T *)Yw + 1, ldy, Yo + 1,
+                               ldy));
     LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), dim3((unsigned)((n + 255) / 256), (unsigned)Q), dim3(256), 0, n,
                (const T *)Yo, (const T *)Yw, ldy, planes, (const int *)h->perm.as<int>(),
                (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
@@ -1183,6 +1199,16 @@ Status linv_typed(lhl_handle *h) {
     if (nbt > 0)
         LHL_LAUNCH(h, "reduce_finish", k_linv_full<T>, dim3((unsigned)((n - 1 + 15) / 16)), dim3(512), 0, n,
                    (const T *)h->A.as<T>(), h->lda, (const T *)h->Linv.as<T>(), h->Lfull.as<T>(), ldf);
+    // both sweeps of the solves as dense GEMMs: L'^-1 with its diagonal and
+    // upper part set, and I + L' (L'(i, m) = A(i+1, m)) as a dense copy
+    LHL_TRY(h->Lp1.ensure(sizeof(T) * (size_t)ldf * (size_t)std::max(n - 1, 1)));
+    if (n >= 2) {
+        const dim3 gl((unsigned)((n - 1 + 255) / 256), (unsigned)(n - 1));
+        LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->Lfull.as<T>(), ldf,
+                   h->Lfull.as<T>(), ldf);
+        LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->A.as<T>() + 1, h->lda,
+                   h->Lp1.as<T>(), ldf);
+    }
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }

@@ -276,7 +276,8 @@ struct lhl_handle {
     // stream, kept until the next solve (every caller synchronizes between solves)
     std::vector<unsigned char> host_ug;
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
-    lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), leading dimension ldlf
+    lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), dense unit lower, leading dimension ldlf
+    lhl::DevBuf Lp1;            // I + L' as a dense unit lower matrix (solve), leading dimension ldlf
     int64_t ldlf = 0;
     lhl::DevBuf ybuf, rbuf, gbuf, xstage, bstage;  // solve / refine scratch
     lhl::Profiler prof;
