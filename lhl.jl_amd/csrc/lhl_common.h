@@ -71,6 +71,40 @@ template <typename T> __device__ __forceinline__ T wave_sum_fast(T v) {
     return v;
 }
 
+// Raw buffer access.  A resource covers `bytes` bytes from `base`; the
+// hardware's range check compares the per-lane offset with that size, so a
+// load at or beyond it returns 0 and a store there is dropped.  Callers pass
+// offsets in [0, bytes) or at/above it, never negative: kBufNone (2^31) marks
+// a lane with nothing to load or store.  The offset goes through an empty asm
+// so the compiler cannot move a constant part of it into the instruction's
+// immediate offset: the range check sees the per-lane register as given, and a
+// split that leaves it negative turns an in-range store into a dropped one.
+// Word 3 = 0x00020000: 32-bit data format, no swizzle, no thread-id
+// addressing.  The scalar offset is 0.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kBufNone = (int)0x80000000u;
+__device__ __forceinline__ BufRsrc buf_rsrc(const void *base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ int buf_opaque(int off) {
+    asm volatile("" : "+v"(off));
+    return off;
+}
+template <typename T> __device__ __forceinline__ T buf_ld(BufRsrc r, int off);
+template <> __device__ __forceinline__ float buf_ld<float>(BufRsrc r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, buf_opaque(off), 0, 0));
+}
+template <> __device__ __forceinline__ double buf_ld<double>(BufRsrc r, int off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, buf_opaque(off), 0, 0));
+}
+__device__ __forceinline__ void buf_st(float v, BufRsrc r, int off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, buf_opaque(off), 0, 0);
+}
+__device__ __forceinline__ void buf_st(double v, BufRsrc r, int off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 0);
+}
+
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 

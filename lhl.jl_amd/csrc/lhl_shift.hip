@@ -14,6 +14,8 @@
 // (U, multipliers, swap, rdiag, info) equals the reference's bit for bit.
 #include "lhl_common.h"
 
+#include <type_traits>
+
 namespace lhl {
 namespace {
 
@@ -28,12 +30,19 @@ struct ShiftParam {
 };
 
 // ---------------------------------------------------------------- real shifts
-// The step body is straight-line code: every load is unconditional (clamped,
-// in-bounds addresses), both eliminations are formed and a select keeps the
-// decided one, and stores of lanes with nothing to store go to a per-shift
-// dump area after U.  With no branch between a load and its use, the
-// compiler waits only for the load it needs, so the rows of H prefetched
-// three steps ahead stream in while the current steps run.
+// Per step, each owned element costs two products, a difference and one
+// store: rows of H come in through buffer loads at per-lane offsets fixed for
+// the whole kernel, and U row k is written through a buffer resource that
+// starts at U[k,k] and ends with the row, so lanes past the end (j >= n) fall
+// outside it and store nothing, with no select on the address.  Only the
+// chunk holding column k has lanes of finished columns (j < k); they get the
+// out-of-range offset.  The lane of column k writes the diagonal itself: its
+// element is a (no swap) or tau*H[k+1,k] = b (swap).
+// Only the chunk holding column k+1 (a uniform branch) adds sigma and
+// publishes r[k+1].  The reciprocal pivots are formed after the loop from the
+// diagonals, kept in the subdiagonal's LDS slots once those have been read.
+// There is no branch between a prefetched row's load and its use, so the rows
+// of H loaded three steps ahead stream in while the current steps run.
 template <typename T, int NPER, int NT>
 __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
@@ -42,43 +51,45 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
                                                               int64_t lstride, uint8_t *__restrict__ swbase,
                                                               int64_t sstride, int *__restrict__ infobase) {
 #pragma clang fp contract(off)
+    constexpr int ES = (int)sizeof(T);
     const int s = blockIdx.x;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
-    T *dump = U + ustride - NT;        // NT spare elements per shift
     T *lv = lbase + (int64_t)s * lstride;
     T *rd = rdbase + (int64_t)s * lstride;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x;
     __shared__ T slot[2];
-    __shared__ T ldump[NT];
     extern __shared__ __align__(16) unsigned char lds_dyn[];
-    T *subd = reinterpret_cast<T *>(lds_dyn);     // subd[k] = H[k+1, k], read by every step
+    // subd[k] = H[k+1, k], read by every lane at step k; afterwards U[k, k]
+    T *subd = reinterpret_cast<T *>(lds_dyn);
     for (int i = tid; i + 1 < n; i += NT) subd[i] = Ht[i + (int64_t)(i + 1) * ldh];
 
     T r[NPER];
+    int jb[NPER];   // byte offset of owned column j = tid + m NT within a row
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
         const int j = tid + m * NT;
+        jb[m] = j * ES;
         r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
     }
     if (tid == 0) r[0] = r[0] + sigma;
     if (tid == 0) slot[0] = r[0];
     // Rows of H are prefetched three steps ahead into three register sets
-    // used in rotation (the k loop is unrolled by three so no set is copied).
-    // The subdiagonal comes from LDS: a scalar-memory prefetch of it would be
-    // waited for by every step's LDS barrier (they share one counter).
+    // used in rotation (the k loop is unrolled by three so no set is copied);
+    // columns j >= n read 0 and are never used.
     T hA[NPER], hB[NPER], hC[NPER];
     auto load_row = [&](int row, T(&hd)[NPER]) {
-        const int64_t col = (int64_t)min(row, n - 1) * ldh;
+        const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
 #pragma unroll
-        for (int m = 0; m < NPER; m++) hd[m] = Ht[min(tid + m * NT, n - 1) + col];
+        for (int m = 0; m < NPER; m++) hd[m] = buf_ld<T>(rs, jb[m]);
     };
     load_row(1, hA);
     load_row(2, hB);
     load_row(3, hC);
     __syncthreads();
     int info = 0;
+    T ukp = 0;   // U[k-1, k-1], stored into subd[k-1] at step k (lane 0)
     auto step = [&](int k, const T(&h)[NPER]) {
         const T a = slot[k & 1];
         const T b = tau * subd[k];
@@ -89,16 +100,22 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
         const T q = num / den;
         const T l = zero ? (T)0 : q;
         info = (zero && info == 0) ? k + 1 : info;
-        const T ukk = sw_k ? b : a;
-        T *Urow = U + upack_off(n, k) - k;     // Urow[j] = U[k, j]
-        if (tid < kWave) {   // wave 0 alone: row k's diagonal, reciprocal pivot, multiplier, swap flag
-            const T rdk = (T)1 / ukk;
-            const bool t0 = tid == 0;
-            *(t0 ? Urow + k : dump + tid) = ukk;
-            *(t0 ? rd + k : dump + tid) = rdk;
-            *(t0 ? lv + k : dump + tid) = l;
-            *(t0 ? sw + k : reinterpret_cast<uint8_t *>(dump + tid)) = (uint8_t)sw_k;
+        if (tid == 0) {
+            lv[k] = l;
+            sw[k] = (uint8_t)sw_k;
+            if (k > 0) subd[k - 1] = ukp;   // every lane read subd[k-1] before the last barrier
         }
+        ukp = sw_k ? b : a;
+        // row k of U from its diagonal: column j at byte (j - k) ES
+        const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
+        const int kb = k * ES;
+        const int dcm = k / NT;         // the chunk holding column k
+        const int fcm = (k + 1) / NT;   // the chunk holding column k + 1
+        // U[k, j] at byte (j - k) ES; j < k (chunk dcm only) stores nothing
+        auto uoff = [&](int m) {
+            const int x = jb[m] - kb;
+            return m == dcm ? (x >= 0 ? x : kBufNone) : x;
+        };
         // the elimination with the reference's operation order (src:2550-2572).
         // a and b are the same in every lane (one LDS slot, one address), so the
         // decision is read from the first lane and taken as a scalar branch.
@@ -106,33 +123,38 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
         if (__builtin_amdgcn_readfirstlane((int)sw_k)) {
 #pragma unroll
             for (int m = 0; m < NPER; m++) {
-                if ((m + 1) * NT <= k + 1) continue;   // chunk finished for every thread (uniform)
-                const int j = tid + m * NT;
-                const bool active = j > k && j < n;
-                const bool first = j == k + 1;
+                if ((m + 1) * NT <= k) continue;   // every column of the chunk is finished (uniform)
                 const T rj = r[m];
                 const T g = tau * h[m];
                 const T rs = rj - l * g;           // swap: pending row minus l * (new row)
-                const T u = first ? g + sigma : g;
-                const T rnew = first ? rs - ls : rs;
-                r[m] = active ? rnew : rj;
-                *(active ? Urow + j : dump + tid) = u;
-                *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
+                if (m == fcm) {
+                    const bool first = tid + m * NT == k + 1;
+                    const T rnew = first ? rs - ls : rs;
+                    r[m] = rnew;
+                    buf_st(first ? g + sigma : g, ur, uoff(m));
+                    if (first) slot[(k + 1) & 1] = rnew;
+                } else {
+                    r[m] = rs;
+                    buf_st(g, ur, uoff(m));
+                }
             }
         } else {
 #pragma unroll
             for (int m = 0; m < NPER; m++) {
-                if ((m + 1) * NT <= k + 1) continue;
-                const int j = tid + m * NT;
-                const bool active = j > k && j < n;
-                const bool first = j == k + 1;
+                if ((m + 1) * NT <= k) continue;
                 const T rj = r[m];
                 const T g = tau * h[m];
                 const T rn = g - l * rj;           // no swap: new row minus l * pending row
-                const T rnew = first ? rn + sigma : rn;
-                r[m] = active ? rnew : rj;
-                *(active ? Urow + j : dump + tid) = rj;
-                *(first ? &slot[(k + 1) & 1] : &ldump[tid]) = rnew;
+                if (m == fcm) {
+                    const bool first = tid + m * NT == k + 1;
+                    const T rnew = first ? rn + sigma : rn;
+                    r[m] = rnew;
+                    buf_st(rj, ur, uoff(m));
+                    if (first) slot[(k + 1) & 1] = rnew;
+                } else {
+                    r[m] = rn;
+                    buf_st(rj, ur, uoff(m));
+                }
             }
         }
         lds_barrier();   // the slot is the only data shared between steps
@@ -148,19 +170,262 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
     }
     if (k < n - 1) step(k, hA);
     if (k + 1 < n - 1) step(k + 1, hB);
-    if (n >= 1) {
-        // the last diagonal: U[n-1, n-1] = r[n-1]
-        const int own = (n - 1) % NT, mm = (n - 1) / NT;
-        if (tid == own) {
-            T rn = 0;
+    // the last diagonal: U[n-1, n-1] = r[n-1]
+    const int own = (n - 1) % NT, mm = (n - 1) / NT;
+    if (tid == 0 && n >= 2) subd[n - 2] = ukp;
+    if (tid == own) {
+        T rn = 0;
 #pragma unroll
-            for (int m = 0; m < NPER; m++) if (m == mm) rn = r[m];
-            U[upack_off(n, n - 1)] = rn;
-            rd[n - 1] = (T)1 / rn;
-            sw[n - 1] = 0;
-            if (rn == (T)0 && info == 0) info = n;
-            infobase[s] = info;
+        for (int m = 0; m < NPER; m++) if (m == mm) rn = r[m];
+        U[upack_off(n, n - 1)] = rn;
+        subd[n - 1] = rn;
+        sw[n - 1] = 0;
+        if (rn == (T)0 && info == 0) info = n;
+        infobase[s] = info;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) rd[i] = (T)1 / subd[i];
+}
+
+// Panel + bulk form of the same elimination, for n <= 8192 (512 threads).
+// The decisions of steps k0 .. k0+62 depend only on the pending row's columns
+// k0 .. k0+62 (the window), so one wave runs those 63 steps alone with no
+// barrier: lane L holds column k0+L, the step's pivot candidates are read from
+// lanes i (a) and the row of H (b), the whole window is updated, and the
+// window's part of U rows k0 .. k0+62 is stored.  The block's multipliers and
+// swap flags go to LDS; after one barrier every thread applies the 63 known
+// steps to its columns beyond the window (each column's recurrence needs only
+// those and its rows of H, prefetched three steps ahead across blocks), and
+// the owners of the next window hand its pending values over through LDS.
+// Each element sees the reference's operations in the reference's order, as
+// in the one-step kernel above, so the results are bit-identical to it.
+constexpr int kPanel = 63;   // steps per block = window columns (a multiple of 3 and of the prefetch depth)
+
+// f(integral_constant<int, v>) for the runtime value v in [M, N)
+template <int M, int N, typename F> __device__ __forceinline__ void dispatch_const(int v, F &f) {
+    if constexpr (M < N) {
+        if (v == M) f(std::integral_constant<int, M>{});
+        else dispatch_const<M + 1, N>(v, f);
+    }
+}
+
+template <typename T> __device__ __forceinline__ T rdlane(T v, int l);
+template <> __device__ __forceinline__ float rdlane<float>(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+template <> __device__ __forceinline__ double rdlane<double>(double v, int l) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+template <typename T, int NPER, int NT, int PD>
+__global__ void __launch_bounds__(NT) k_shift_panel(const T *__restrict__ Ht, int64_t ldh, int n,
+                                                    const ShiftParam<T> *__restrict__ prm,
+                                                    T *__restrict__ Ubase, int64_t ustride,
+                                                    T *__restrict__ lbase, T *__restrict__ rdbase,
+                                                    int64_t lstride, uint8_t *__restrict__ swbase,
+                                                    int64_t sstride, int *__restrict__ infobase) {
+#pragma clang fp contract(off)
+    constexpr int ES = (int)sizeof(T);
+    const int s = blockIdx.x;
+    const T sigma = prm[s].sr, tau = prm[s].tr;
+    T *U = Ubase + (int64_t)s * ustride;
+    T *lv = lbase + (int64_t)s * lstride;
+    T *rd = rdbase + (int64_t)s * lstride;
+    uint8_t *sw = swbase + (int64_t)s * sstride;
+    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ T win[64];     // pending values of the next window
+    __shared__ T lsh[64];     // the block's multipliers
+    __shared__ int ssh[64];   // the block's swap flags
+    __shared__ T plast;       // the pending value of column n-1 after the last step
+
+    T r[NPER];
+    int jb[NPER];   // byte offset of owned column j = tid + m NT within a row
+#pragma unroll
+    for (int m = 0; m < NPER; m++) {
+        const int j = tid + m * NT;
+        jb[m] = j * ES;
+        r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
+    }
+    if (tid == 0) r[0] = r[0] + sigma;
+    if (tid < kPanel) win[tid] = r[0];
+    if (tid == 0 && n == 1) plast = r[0];
+    // rows of H for the bulk steps, three steps ahead in rotation
+    T hA[NPER], hB[NPER], hC[NPER];
+    auto load_row = [&](int row, T(&hd)[NPER]) {
+        const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
+#pragma unroll
+        for (int m = 0; m < NPER; m++) hd[m] = buf_ld<T>(rs, jb[m]);
+    };
+    load_row(1, hA);
+    load_row(2, hB);
+    load_row(3, hC);
+    __syncthreads();
+
+    int info = 0;   // wave 0 (the same in every lane)
+    const int nsteps = n - 1;
+    for (int k0 = 0; k0 < nsteps; k0 += kPanel) {
+        const int nb = min(kPanel, nsteps - k0);
+        const int c1 = k0 + kPanel;   // the first column after the window
+        // ---- panel: steps k0 .. k0+nb-1 on the window, wave 0 alone.  A full
+        // block (every block but the last) is straight-line code, so the waits
+        // for the prefetched rows count the stores between exactly.
+        auto panel = [&](auto fullc) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(fullc)::value;
+            const int c = k0 + lane;
+            T p = win[lane];
+            T lvec = 0, uvec = 1;
+            int svec = 0;
+            T hw[PD];
+            auto load_w = [&](int i, T &hd) {   // H[k0+i+1, c] (0 for c >= n)
+                const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(k0 + i + 1, n - 1) * ldh, n * ES);
+                hd = buf_ld<T>(rs, c * ES);
+            };
+#pragma unroll
+            for (int i = 0; i < PD; i++) load_w(i, hw[i]);
+            // groups of PD steps (PD divides 63): the prefetch slot of step i is i % PD
+#pragma nounroll
+            for (int ib = 0; ib < kPanel; ib += PD)
+#pragma unroll
+            for (int ii = 0; ii < PD; ii++) {
+                const int i = ib + ii;
+                if (FULL || i < nb) {
+                    const int k = k0 + i;
+                    const T h = hw[ii];
+                    const T a = rdlane(p, i);
+                    const T b = tau * rdlane(h, i);
+                    const bool swk = dabs(b) > dabs(a);
+                    const bool zero = !swk && a == (T)0;
+                    const T num = swk ? a : b, den = swk ? b : a;
+                    const T q = num / den;
+                    const T l = zero ? (T)0 : q;
+                    info = (zero && info == 0) ? k + 1 : info;
+                    const bool first = lane == i + 1;
+                    const T g = tau * h;
+                    const T ls = l * sigma;
+                    const T rs = p - l * g;       // swap: pending row minus l * (new row)
+                    const T rn = g - l * p;       // no swap: new row minus l * pending row
+                    const T u = swk ? (first ? g + sigma : g) : p;
+                    const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
+                    // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
+                    const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
+                    buf_st(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
+                    lvec = lane == i ? l : lvec;
+                    svec = lane == i ? (int)swk : svec;
+                    uvec = lane == i ? (swk ? b : a) : uvec;
+                    p = pn;   // lanes <= i are finished: their value is no longer read
+                }
+                load_w(i + PD, hw[ii]);   // past the block: a clamped row, unused
+            }
+            if (lane < nb) {
+                lsh[lane] = lvec;
+                ssh[lane] = svec;
+                lv[k0 + lane] = lvec;
+                sw[k0 + lane] = (uint8_t)svec;
+                rd[k0 + lane] = (T)1 / uvec;
+            }
+            if (c == n - 1 && lane < kPanel) plast = p;   // the last block's window holds column n-1
+        };
+        if (tid < 64) {
+            if (nb == kPanel) panel(std::true_type{});
+            else panel(std::false_type{});
         }
+        __syncthreads();
+        // ---- bulk: the block's steps on columns c1 .. n-1 (a full block: nb = 63).
+        // Instantiated per first live chunk MS = c1 / NT (chunks below it hold
+        // only finished columns), so the step body is straight-line code.
+        auto bulk = [&](auto msc) __attribute__((always_inline)) {
+            constexpr int MS = decltype(msc)::value;
+            int boff[NPER];   // U[k, j] at byte (j - c1) ES from U[k, c1]; j < c1 stores nothing
+#pragma unroll
+            for (int m = MS; m < NPER; m++) {
+                const int j = tid + m * NT;
+                boff[m] = j >= c1 ? (j - c1) * ES : kBufNone;
+            }
+            const T lvv = lsh[lane];
+            const int svv = ssh[lane];
+            const int ubytes = (n - c1) * ES;
+            auto load_live = [&](int row, T(&hd)[NPER]) {
+                const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
+#pragma unroll
+                for (int m = MS; m < NPER; m++) hd[m] = buf_ld<T>(rs, jb[m]);
+            };
+            int64_t uo = upack_off(n, k0);   // U[k, k] of the current step k
+            // one step: the decided elimination as values only (the two branches
+            // issue no memory operation), then the stores, so every path has the
+            // same memory operations and the waits for prefetched rows stay exact
+            auto bstep = [&](int i, const T(&h)[NPER], bool last) __attribute__((always_inline)) {
+                const int k = k0 + i;
+                const T l = rdlane(lvv, i);
+                const int sk = __builtin_amdgcn_readlane(svv, i);
+                const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
+                uo += n - k;
+                T u[NPER];
+                if (sk) {
+#pragma unroll
+                    for (int m = MS; m < NPER; m++) {
+                        const T g = tau * h[m];
+                        const T rs = r[m] - l * g;
+                        if (last) {   // step k0+62: column c1 = k+1 is the new row's diagonal
+                            const bool first = tid + m * NT == c1;
+                            u[m] = first ? g + sigma : g;
+                            r[m] = first ? rs - l * sigma : rs;
+                        } else {
+                            u[m] = g;
+                            r[m] = rs;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = MS; m < NPER; m++) {
+                        const T g = tau * h[m];
+                        const T rn = g - l * r[m];
+                        u[m] = r[m];
+                        if (last) {
+                            const bool first = tid + m * NT == c1;
+                            r[m] = first ? rn + sigma : rn;
+                        } else {
+                            r[m] = rn;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = MS; m < NPER; m++) buf_st(u[m], ur, boff[m]);
+            };
+            for (int i = 0; i < kPanel - 3; i += 3) {
+                bstep(i, hA, false);
+                load_live(k0 + i + 4, hA);
+                bstep(i + 1, hB, false);
+                load_live(k0 + i + 5, hB);
+                bstep(i + 2, hC, false);
+                load_live(k0 + i + 6, hC);
+            }
+            bstep(kPanel - 3, hA, false);
+            load_live(k0 + kPanel + 1, hA);
+            bstep(kPanel - 2, hB, false);
+            load_live(k0 + kPanel + 2, hB);
+            bstep(kPanel - 1, hC, true);
+            load_live(k0 + kPanel + 3, hC);
+            // the next window's pending values (and column n-1 when it is c1)
+#pragma unroll
+            for (int m = MS; m < NPER; m++) {
+                const int j = tid + m * NT;
+                if (j >= c1 && j < c1 + kPanel) win[j - c1] = r[m];
+                if (j == n - 1 && j == c1) plast = r[m];
+            }
+        };
+        if (c1 < n) dispatch_const<0, NPER>(c1 / NT, bulk);
+        __syncthreads();
+    }
+    // the last diagonal: U[n-1, n-1] = p[n-1]
+    if (tid == 0) {
+        const T pl = plast;
+        U[upack_off(n, n - 1)] = pl;
+        rd[n - 1] = (T)1 / pl;
+        sw[n - 1] = 0;
+        if (pl == (T)0 && info == 0) info = n;
+        infobase[s] = info;
     }
 }
 
@@ -348,6 +613,10 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
                    info);
+    } else if constexpr (NT == 512 && (sizeof(T) == 4 || NPER <= 8)) {   // FP64 at 16 columns per thread: registers
+        auto fn = k_shift_panel<T, NPER, NT, 21>;
+        LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), 0, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
+                   info);
     } else {
         auto fn = k_shift_real<T, NPER, NT>;
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -357,7 +626,7 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
     return {};
 }
 
-// 512 threads up to n = 8192 (measured best at n = 4096), 1024 beyond.
+// 512 threads up to n = 8192 (the panel form for real shifts), 1024 beyond.
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
