@@ -187,19 +187,22 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
     for (int i = tid; i < n; i += NT) rd[i] = (T)1 / subd[i];
 }
 
-// Panel + bulk form of the same elimination, for n <= 8192 (512 threads).
-// The decisions of steps k0 .. k0+62 depend only on the pending row's columns
-// k0 .. k0+62 (the window), so one wave runs those 63 steps alone with no
-// barrier: lane L holds column k0+L, the step's pivot candidates are read from
-// lanes i (a) and the row of H (b), the whole window is updated, and the
-// window's part of U rows k0 .. k0+62 is stored.  The block's multipliers and
-// swap flags go to LDS; after one barrier every thread applies the 63 known
-// steps to its columns beyond the window (each column's recurrence needs only
-// those and its rows of H, prefetched three steps ahead across blocks), and
-// the owners of the next window hand its pending values over through LDS.
-// Each element sees the reference's operations in the reference's order, as
-// in the one-step kernel above, so the results are bit-identical to it.
-constexpr int kPanel = 63;   // steps per block = window columns (a multiple of 3 and of the prefetch depth)
+// Pipelined panel + bulk form of the same elimination (real shifts).  The
+// decisions of steps k0 .. k0+62 (block b) depend only on the pending row's
+// columns k0 .. k0+62 (window b), so wave 0 (the panel wave) runs those 63
+// steps alone with no barrier: lane L holds column k0+L, the step's pivot
+// candidates are read from lane i (a) and from the row of H (b), the window is
+// updated and its part of U rows k0 .. k0+62 stored.  It then applies the
+// block, with the multipliers still in its registers, to window b+1 (the next
+// 63 columns, handed over through LDS) and goes straight on to block b+1's
+// decisions.  Meanwhile the other NTB threads apply block b to every column
+// beyond window b+1 (each column's recurrence needs only the block's
+// multipliers and swap flags, published in LDS, and its rows of H, prefetched
+// R steps ahead across blocks) and hand window b+2 over.  One barrier per
+// block separates the two; LDS buffers alternate by block parity.  Each
+// element sees the reference's operations in the reference's order, as in the
+// one-step kernel above, so the results are bit-identical to it.
+constexpr int kPanel = 63;   // steps per block = window columns (a multiple of R and of the prefetch depth)
 
 // f(integral_constant<int, v>) for the runtime value v in [M, N)
 template <int M, int N, typename F> __device__ __forceinline__ void dispatch_const(int v, F &f) {
@@ -219,14 +222,15 @@ template <> __device__ __forceinline__ double rdlane<double>(double v, int l) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename T, int NPER, int NT, int PD>
-__global__ void __launch_bounds__(NT) k_shift_panel(const T *__restrict__ Ht, int64_t ldh, int n,
-                                                    const ShiftParam<T> *__restrict__ prm,
-                                                    T *__restrict__ Ubase, int64_t ustride,
-                                                    T *__restrict__ lbase, T *__restrict__ rdbase,
-                                                    int64_t lstride, uint8_t *__restrict__ swbase,
-                                                    int64_t sstride, int *__restrict__ infobase) {
+template <typename T, int NPER, int NTB, int PD, int R>
+__global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ Ht, int64_t ldh, int n,
+                                                          const ShiftParam<T> *__restrict__ prm,
+                                                          T *__restrict__ Ubase, int64_t ustride,
+                                                          T *__restrict__ lbase, T *__restrict__ rdbase,
+                                                          int64_t lstride, uint8_t *__restrict__ swbase,
+                                                          int64_t sstride, int *__restrict__ infobase) {
 #pragma clang fp contract(off)
+    static_assert(kPanel % R == 0 && kPanel % PD == 0, "block of 63 steps in whole rotations");
     constexpr int ES = (int)sizeof(T);
     const int s = blockIdx.x;
     const T sigma = prm[s].sr, tau = prm[s].tr;
@@ -235,187 +239,219 @@ __global__ void __launch_bounds__(NT) k_shift_panel(const T *__restrict__ Ht, in
     T *rd = rdbase + (int64_t)s * lstride;
     uint8_t *sw = swbase + (int64_t)s * sstride;
     const int tid = threadIdx.x, lane = tid & 63;
-    __shared__ T win[64];     // pending values of the next window
-    __shared__ T lsh[64];     // the block's multipliers
-    __shared__ int ssh[64];   // the block's swap flags
-    __shared__ T plast;       // the pending value of column n-1 after the last step
+    const bool is_panel = tid < 64;
+    const int tb = tid - 64;   // bulk thread index (tid >= 64)
+    __shared__ T win[2][64];   // pending values of a window, by block parity
+    __shared__ T lsh[2][64];   // a block's multipliers
+    __shared__ int ssh[2][64]; // a block's swap flags
+    __shared__ T plast;        // the pending value of column n-1 after the last step
+    const int nsteps = n - 1;
 
+    // bulk threads: columns j = tb + m NTB of the pending row r = tau H[0, :] + sigma e_0
     T r[NPER];
-    int jb[NPER];   // byte offset of owned column j = tid + m NT within a row
+    int jb[NPER];   // byte offset of column j within a row
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
-        const int j = tid + m * NT;
+        const int j = tb + m * NTB;
         jb[m] = j * ES;
-        r[m] = (j < n) ? tau * Ht[j] : (T)0;          // r = tau * H[0, :]
+        r[m] = (!is_panel && j < n) ? tau * Ht[j] : (T)0;
+        if (j == 0) r[m] = r[m] + sigma;
+        if (!is_panel && j < 2 * kPanel) win[j / kPanel][j % kPanel] = r[m];   // windows 0 and 1
+        if (!is_panel && j == 0 && n == 1) plast = r[m];
     }
-    if (tid == 0) r[0] = r[0] + sigma;
-    if (tid < kPanel) win[tid] = r[0];
-    if (tid == 0 && n == 1) plast = r[0];
-    // rows of H for the bulk steps, three steps ahead in rotation
-    T hA[NPER], hB[NPER], hC[NPER];
-    auto load_row = [&](int row, T(&hd)[NPER]) {
+    // rows of H for the bulk steps, R steps ahead in rotation
+    T hs[R][NPER];
+    auto load_rows = [&](int row, T(&hd)[NPER], int ms) __attribute__((always_inline)) {
         const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
 #pragma unroll
-        for (int m = 0; m < NPER; m++) hd[m] = buf_ld<T>(rs, jb[m]);
+        for (int m = 0; m < NPER; m++)
+            if (m >= ms) hd[m] = buf_ld<T>(rs, jb[m]);
     };
-    load_row(1, hA);
-    load_row(2, hB);
-    load_row(3, hC);
+    if (!is_panel) {
+#pragma unroll
+        for (int q = 0; q < R; q++) load_rows(1 + q, hs[q], 0);
+    }
     __syncthreads();
 
-    int info = 0;   // wave 0 (the same in every lane)
-    const int nsteps = n - 1;
-    for (int k0 = 0; k0 < nsteps; k0 += kPanel) {
-        const int nb = min(kPanel, nsteps - k0);
-        const int c1 = k0 + kPanel;   // the first column after the window
-        // ---- panel: steps k0 .. k0+nb-1 on the window, wave 0 alone.  A full
-        // block (every block but the last) is straight-line code, so the waits
-        // for the prefetched rows count the stores between exactly.
-        auto panel = [&](auto fullc) __attribute__((always_inline)) {
-            constexpr bool FULL = decltype(fullc)::value;
-            const int c = k0 + lane;
-            T p = win[lane];
-            T lvec = 0, uvec = 1;
-            int svec = 0;
-            T hw[PD];
-            auto load_w = [&](int i, T &hd) {   // H[k0+i+1, c] (0 for c >= n)
-                const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(k0 + i + 1, n - 1) * ldh, n * ES);
-                hd = buf_ld<T>(rs, c * ES);
-            };
+    // ---- panel wave state: the current window (lane L = column c0 + L)
+    int info = 0;
+    T p = 0;                       // pending value of the lane's window column
+    T lvec = 0;                    // the last panel block's multipliers (lane i: step i)
+    int svec = 0;                  // and swap flags
+    T hw[PD];
+    // H[k0+i+1, cb+lane] for the steps i of a block, PD ahead
+    auto load_w = [&](int k0, int cb, int i, T &hd) __attribute__((always_inline)) {
+        const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(k0 + i + 1, n - 1) * ldh, n * ES);
+        hd = buf_ld<T>(rs, (cb + lane) * ES);
+    };
+    // decisions of block k0 on window k0 (p holds it): steps k0 .. k0+nb-1.  A
+    // full block (nb = 63) is straight-line code, so the waits for the
+    // prefetched rows count the stores between them exactly.
+    auto panel_nb = [&](int k0, int nb, auto fullc) __attribute__((always_inline)) {
+        constexpr bool FULL = decltype(fullc)::value;
+        T uvec = 1;
 #pragma unroll
-            for (int i = 0; i < PD; i++) load_w(i, hw[i]);
-            // groups of PD steps (PD divides 63): the prefetch slot of step i is i % PD
+        for (int i = 0; i < PD; i++) load_w(k0, k0, i, hw[i]);
 #pragma nounroll
-            for (int ib = 0; ib < kPanel; ib += PD)
+        for (int ib = 0; ib < kPanel; ib += PD)
 #pragma unroll
-            for (int ii = 0; ii < PD; ii++) {
-                const int i = ib + ii;
-                if (FULL || i < nb) {
-                    const int k = k0 + i;
-                    const T h = hw[ii];
-                    const T a = rdlane(p, i);
-                    const T b = tau * rdlane(h, i);
-                    const bool swk = dabs(b) > dabs(a);
-                    const bool zero = !swk && a == (T)0;
-                    const T num = swk ? a : b, den = swk ? b : a;
-                    const T q = num / den;
-                    const T l = zero ? (T)0 : q;
-                    info = (zero && info == 0) ? k + 1 : info;
-                    const bool first = lane == i + 1;
-                    const T g = tau * h;
-                    const T ls = l * sigma;
-                    const T rs = p - l * g;       // swap: pending row minus l * (new row)
-                    const T rn = g - l * p;       // no swap: new row minus l * pending row
-                    const T u = swk ? (first ? g + sigma : g) : p;
-                    const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
-                    // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
-                    const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
-                    buf_st(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
-                    lvec = lane == i ? l : lvec;
-                    svec = lane == i ? (int)swk : svec;
-                    uvec = lane == i ? (swk ? b : a) : uvec;
-                    p = pn;   // lanes <= i are finished: their value is no longer read
-                }
-                load_w(i + PD, hw[ii]);   // past the block: a clamped row, unused
-            }
-            if (lane < nb) {
-                lsh[lane] = lvec;
-                ssh[lane] = svec;
-                lv[k0 + lane] = lvec;
-                sw[k0 + lane] = (uint8_t)svec;
-                rd[k0 + lane] = (T)1 / uvec;
-            }
-            if (c == n - 1 && lane < kPanel) plast = p;   // the last block's window holds column n-1
-        };
-        if (tid < 64) {
-            if (nb == kPanel) panel(std::true_type{});
-            else panel(std::false_type{});
-        }
-        __syncthreads();
-        // ---- bulk: the block's steps on columns c1 .. n-1 (a full block: nb = 63).
-        // Instantiated per first live chunk MS = c1 / NT (chunks below it hold
-        // only finished columns), so the step body is straight-line code.
-        auto bulk = [&](auto msc) __attribute__((always_inline)) {
-            constexpr int MS = decltype(msc)::value;
-            int boff[NPER];   // U[k, j] at byte (j - c1) ES from U[k, c1]; j < c1 stores nothing
-#pragma unroll
-            for (int m = MS; m < NPER; m++) {
-                const int j = tid + m * NT;
-                boff[m] = j >= c1 ? (j - c1) * ES : kBufNone;
-            }
-            const T lvv = lsh[lane];
-            const int svv = ssh[lane];
-            const int ubytes = (n - c1) * ES;
-            auto load_live = [&](int row, T(&hd)[NPER]) {
-                const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
-#pragma unroll
-                for (int m = MS; m < NPER; m++) hd[m] = buf_ld<T>(rs, jb[m]);
-            };
-            int64_t uo = upack_off(n, k0);   // U[k, k] of the current step k
-            // one step: the decided elimination as values only (the two branches
-            // issue no memory operation), then the stores, so every path has the
-            // same memory operations and the waits for prefetched rows stay exact
-            auto bstep = [&](int i, const T(&h)[NPER], bool last) __attribute__((always_inline)) {
+        for (int ii = 0; ii < PD; ii++) {
+            const int i = ib + ii;
+            if (FULL || i < nb) {
                 const int k = k0 + i;
-                const T l = rdlane(lvv, i);
-                const int sk = __builtin_amdgcn_readlane(svv, i);
-                const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
-                uo += n - k;
-                T u[NPER];
-                if (sk) {
+                const T h = hw[ii];
+                const T a = rdlane(p, i);
+                const T b = tau * rdlane(h, i);
+                // the decision (src:2545-2565): swap iff |b| > |a|; l = a/b, b/a, or 0 for a zero pivot
+                const bool swk = dabs(b) > dabs(a);
+                const bool zero = !swk && a == (T)0;
+                const T num = swk ? a : b, den = swk ? b : a;
+                const T q = num / den;
+                const T l = zero ? (T)0 : q;
+                info = (zero && info == 0) ? k + 1 : info;
+                // the elimination with the reference's operation order (src:2550-2572)
+                const bool first = lane == i + 1;
+                const T g = tau * h;
+                const T ls = l * sigma;
+                const T rs = p - l * g;       // swap: pending row minus l * (new row)
+                const T rn = g - l * p;       // no swap: new row minus l * pending row
+                const T u = swk ? (first ? g + sigma : g) : p;
+                const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
+                // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
+                const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
+                buf_st(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
+                lvec = lane == i ? l : lvec;
+                svec = lane == i ? (int)swk : svec;
+                uvec = lane == i ? (swk ? b : a) : uvec;
+                p = pn;   // lanes <= i are finished: their value is no longer read
+            }
+            load_w(k0, k0, i + PD, hw[ii]);   // past the block: a clamped row, unused
+        }
+        const int bp = (k0 / kPanel) & 1;
+        if (lane < nb) {
+            lsh[bp][lane] = lvec;
+            ssh[bp][lane] = svec;
+            lv[k0 + lane] = lvec;
+            sw[k0 + lane] = (uint8_t)svec;
+            rd[k0 + lane] = (T)1 / uvec;
+        }
+    };
+    auto panel = [&](int k0, int nb) __attribute__((always_inline)) {
+        if (nb == kPanel) panel_nb(k0, nb, std::true_type{});
+        else panel_nb(k0, nb, std::false_type{});
+    };
+    // block k0 (full, multipliers in lvec/svec) applied to window c1 = k0+63,
+    // whose pending values are in p; the last step's new row has its
+    // diagonal (sigma) in column c1
+    auto window_apply = [&](int k0) __attribute__((always_inline)) {
+        const int c1 = k0 + kPanel;
+        const int off = lane < kPanel ? lane * ES : kBufNone;
+        const int ubytes = (n - c1) * ES;
+        int64_t uo = upack_off(n, k0);
 #pragma unroll
-                    for (int m = MS; m < NPER; m++) {
-                        const T g = tau * h[m];
-                        const T rs = r[m] - l * g;
-                        if (last) {   // step k0+62: column c1 = k+1 is the new row's diagonal
-                            const bool first = tid + m * NT == c1;
-                            u[m] = first ? g + sigma : g;
-                            r[m] = first ? rs - l * sigma : rs;
-                        } else {
-                            u[m] = g;
-                            r[m] = rs;
-                        }
-                    }
-                } else {
+        for (int i = 0; i < PD; i++) load_w(k0, c1, i, hw[i]);
+#pragma nounroll
+        for (int ib = 0; ib < kPanel; ib += PD)
 #pragma unroll
-                    for (int m = MS; m < NPER; m++) {
-                        const T g = tau * h[m];
-                        const T rn = g - l * r[m];
-                        u[m] = r[m];
-                        if (last) {
-                            const bool first = tid + m * NT == c1;
-                            r[m] = first ? rn + sigma : rn;
+        for (int ii = 0; ii < PD; ii++) {
+            const int i = ib + ii;
+            const int k = k0 + i;
+            const T l = rdlane(lvec, i);
+            const bool swk = __builtin_amdgcn_readlane(svec, i) != 0;
+            const bool first = lane == 0 && i == kPanel - 1;
+            const T g = tau * hw[ii];
+            const T rs = p - l * g;
+            const T rn = g - l * p;
+            const T u = swk ? (first ? g + sigma : g) : p;
+            p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
+            const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
+            uo += n - k;
+            buf_st(u, ur, off);
+            load_w(k0, c1, i + PD, hw[ii]);
+        }
+    };
+
+    if (is_panel && nsteps > 0) {
+        p = win[0][lane];
+        panel(0, min(kPanel, nsteps));
+        if (nsteps < kPanel && lane == nsteps) plast = p;   // one partial block: column n-1 is in window 0
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < nsteps; k0 += kPanel) {
+        const int bp = (k0 / kPanel) & 1;
+        const int c1 = k0 + kPanel;     // window b+1
+        const int c2 = c1 + kPanel;     // the bulk's first column
+        if (is_panel) {
+            if (c1 < n) {   // block b is full
+                p = win[bp ^ 1][lane];
+                window_apply(k0);
+                if (c1 < nsteps) {
+                    const int nb1 = min(kPanel, nsteps - c1);
+                    panel(c1, nb1);
+                    // a partial last block: column n-1 is in window b+1 (a full one
+                    // leaves it to the next window's pass)
+                    if (c1 + nb1 == nsteps && nb1 < kPanel && lane == nb1) plast = p;
+                } else if (lane == 0) {
+                    plast = p;   // c1 = n-1: no step after block b
+                }
+            }
+        } else if (c2 < n) {
+            // block b on columns c2 .. n-1, instantiated per first live chunk
+            // MS = c2 / NTB so that every step is straight-line code (exact
+            // waits for the prefetched rows)
+            auto bulk = [&](auto msc) __attribute__((always_inline)) {
+                constexpr int MS = decltype(msc)::value;
+                int boff[NPER];   // U[k, j] at byte (j - c2) ES from U[k, c2]; j < c2 stores nothing
+#pragma unroll
+                for (int m = MS; m < NPER; m++) {
+                    const int j = tb + m * NTB;
+                    boff[m] = j >= c2 ? (j - c2) * ES : kBufNone;
+                }
+                const T lvv = lsh[bp][lane];
+                const int svv = ssh[bp][lane];
+                const int ubytes = (n - c2) * ES;
+                int64_t uo = upack_off(n, k0);
+                for (int i0 = 0; i0 < kPanel; i0 += R) {
+#pragma unroll
+                    for (int q = 0; q < R; q++) {
+                        const int i = i0 + q;
+                        const int k = k0 + i;
+                        const T l = rdlane(lvv, i);
+                        const int sk = __builtin_amdgcn_readlane(svv, i);
+                        const BufRsrc ur = buf_rsrc(U + uo + (c2 - k), ubytes);
+                        uo += n - k;
+                        // values only in the two branches, then the stores: every
+                        // path issues the same memory operations
+                        T u[NPER];
+                        if (sk) {
+#pragma unroll
+                            for (int m = MS; m < NPER; m++) {
+                                const T g = tau * hs[q][m];
+                                u[m] = g;
+                                r[m] = r[m] - l * g;
+                            }
                         } else {
-                            r[m] = rn;
+#pragma unroll
+                            for (int m = MS; m < NPER; m++) {
+                                const T g = tau * hs[q][m];
+                                u[m] = r[m];
+                                r[m] = g - l * r[m];
+                            }
                         }
+#pragma unroll
+                        for (int m = MS; m < NPER; m++) buf_st(u[m], ur, boff[m]);
+                        load_rows(k + R + 1, hs[q], MS);
                     }
                 }
+                // window b+2 = columns c2 .. c2+62 for the panel wave
 #pragma unroll
-                for (int m = MS; m < NPER; m++) buf_st(u[m], ur, boff[m]);
+                for (int m = MS; m < NPER; m++) {
+                    const int j = tb + m * NTB;
+                    if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = r[m];
+                }
             };
-            for (int i = 0; i < kPanel - 3; i += 3) {
-                bstep(i, hA, false);
-                load_live(k0 + i + 4, hA);
-                bstep(i + 1, hB, false);
-                load_live(k0 + i + 5, hB);
-                bstep(i + 2, hC, false);
-                load_live(k0 + i + 6, hC);
-            }
-            bstep(kPanel - 3, hA, false);
-            load_live(k0 + kPanel + 1, hA);
-            bstep(kPanel - 2, hB, false);
-            load_live(k0 + kPanel + 2, hB);
-            bstep(kPanel - 1, hC, true);
-            load_live(k0 + kPanel + 3, hC);
-            // the next window's pending values (and column n-1 when it is c1)
-#pragma unroll
-            for (int m = MS; m < NPER; m++) {
-                const int j = tid + m * NT;
-                if (j >= c1 && j < c1 + kPanel) win[j - c1] = r[m];
-                if (j == n - 1 && j == c1) plast = r[m];
-            }
-        };
-        if (c1 < n) dispatch_const<0, NPER>(c1 / NT, bulk);
+            dispatch_const<0, NPER>(c2 / NTB, bulk);
+        }
         __syncthreads();
     }
     // the last diagonal: U[n-1, n-1] = p[n-1]
@@ -613,10 +649,6 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
                    info);
-    } else if constexpr (NT == 512 && (sizeof(T) == 4 || NPER <= 8)) {   // FP64 at 16 columns per thread: registers
-        auto fn = k_shift_panel<T, NPER, NT, 21>;
-        LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), 0, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
-                   info);
     } else {
         auto fn = k_shift_real<T, NPER, NT>;
         LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -626,11 +658,39 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
     return {};
 }
 
-// 512 threads up to n = 8192 (the panel form for real shifts), 1024 beyond.
+// the pipelined panel kernel: one panel wave + kPanelBulk bulk threads
+constexpr int kPanelBulk = 448;
+template <typename T, int NPER>
+Status launch_panel(lhl_handle *h, int64_t nshift, const T *Ht, int64_t ldh, int n, const ShiftParam<T> *prm,
+                    T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
+    auto fn = k_shift_panel<T, NPER, kPanelBulk, 21, 3>;
+    LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(64 + kPanelBulk), 0, Ht, ldh, n, prm, U, us, l, rd, ls,
+               sw, ss, info);
+    return {};
+}
+
+// Real shifts: the panel kernel up to n = 4480 (FP64) / 8512 (FP32), its
+// columns per bulk thread chosen from n.  Otherwise (and complex shifts) the
+// one-step kernels: 512 threads up to n = 8192, 1024 beyond.
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
+    constexpr int B = kPanelBulk;
+    constexpr bool F32 = sizeof(T) == 4;
+    if (!cplx && n <= (F32 ? 19 : 10) * B) {
+        if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        else if (n <= 4 * B) LHL_TRY((launch_panel<T, 4>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        else if (n <= 6 * B) LHL_TRY((launch_panel<T, 6>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        else if (n <= 10 * B) LHL_TRY((launch_panel<T, 10>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        else if constexpr (F32) {
+            if (n <= 14 * B) LHL_TRY((launch_panel<T, 14>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 19>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        }
+        LHL_HIP_TRY(hipGetLastError());
+        return {};
+    }
     if ((size_t)n * sizeof(T) + 16 * 1024 > lds_limit())
         return {LHL_ERR_ARG, "lhl_shift: n too large for the LDS-resident subdiagonal of the shift kernel"};
     if (n <= 512) LHL_TRY((launch_shift<T, 1, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));

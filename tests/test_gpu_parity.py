@@ -111,7 +111,9 @@ def shift_state_equal(lhl_hip, oracle, ws, s, t, sh=None):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("tie", [False, True])
 def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
-    for n in [1, 2, 3, 5, 17, 64, 255, 256, 257, 511, 512, 520, 1000, 3000]:
+    # 64, 127, 190: the last 63-step block of the pipelined kernel is full;
+    # 126, 128: a partial last block (62 steps, 1 step)
+    for n in [1, 2, 3, 5, 17, 64, 126, 127, 128, 190, 255, 256, 257, 511, 512, 520, 1000, 3000]:
         say(f"case: shift bitwise n={n} {np.dtype(dtype).name} tie={tie}")
         J = tie_matrix(n, n, dtype=dtype) if tie else randn(n, n, dtype=dtype)
         ws = lhl_hip.lhl(J)
