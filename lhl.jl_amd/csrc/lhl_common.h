@@ -104,6 +104,15 @@ __device__ __forceinline__ void buf_st(double v, BufRsrc r, int off) {
     using U2 = unsigned __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 0);
 }
+// The same with the non-temporal cache policy (aux bit 1, "nt" on gfx950), for
+// data written once that no cache holds until it is read back.
+__device__ __forceinline__ void buf_st_nt(float v, BufRsrc r, int off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, buf_opaque(off), 0, 2);
+}
+__device__ __forceinline__ void buf_st_nt(double v, BufRsrc r, int off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 2);
+}
 
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950

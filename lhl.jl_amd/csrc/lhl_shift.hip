@@ -199,7 +199,9 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 // beyond window b+1 (each column's recurrence needs only the block's
 // multipliers and swap flags, published in LDS, and its rows of H, prefetched
 // R steps ahead across blocks) and hand window b+2 over.  One barrier per
-// block separates the two; LDS buffers alternate by block parity.  Each
+// block separates the two; LDS buffers alternate by block parity.  U is
+// written with non-temporal stores (it is read back only after the kernel,
+// and a batch of shifts' U far exceeds the caches).  Each
 // element sees the reference's operations in the reference's order, as in the
 // one-step kernel above, so the results are bit-identical to it.
 constexpr int kPanel = 63;   // steps per block = window columns (a multiple of R and of the prefetch depth)
@@ -319,7 +321,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
                 // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
                 const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
-                buf_st(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
+                buf_st_nt(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
                 lvec = lane == i ? l : lvec;
                 svec = lane == i ? (int)swk : svec;
                 uvec = lane == i ? (swk ? b : a) : uvec;
@@ -366,7 +368,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
             const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
             uo += n - k;
-            buf_st(u, ur, off);
+            buf_st_nt(u, ur, off);
             load_w(k0, c1, i + PD, hw[ii]);
         }
     };
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                             }
                         }
 #pragma unroll
-                        for (int m = MS; m < NPER; m++) buf_st(u[m], ur, boff[m]);
+                        for (int m = MS; m < NPER; m++) buf_st_nt(u[m], ur, boff[m]);
                         load_rows(k + R + 1, hs[q], MS);
                     }
                 }
