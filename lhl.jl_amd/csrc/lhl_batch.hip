@@ -463,10 +463,15 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
 }
 
 // ---- U in packed columns -----------------------------------------------------
-// UT[j(j+1)/2 + i] = U[i, j] for i <= j: each 64x64 tile of the upper triangle
-// is read along U's packed rows and written along columns through LDS (both
-// sides contiguous).  One workgroup per tile (blockIdx.x, upper-triangle
-// index) and shift-plane (blockIdx.y = shift * planes + plane).
+// UT[ucol_off(j) + i] = U[i, j] for i <= j: each 64x64 tile of the upper
+// triangle is read along U's packed rows and written along columns through
+// LDS (both sides contiguous).  One workgroup per tile (blockIdx.x,
+// upper-triangle index) and shift-plane (blockIdx.y = shift * planes + plane).
+// FP32 (V = 4): each lane writes four consecutive rows of a column as one
+// aligned 16-byte store, 16 lanes per 64-row column segment; the rows past
+// the diagonal in a diagonal tile land in the column's padding (its slots up
+// to round_up(j+1, 4)), which the back substitution never reads.  Offsets are
+// 32-bit: UT is built for n <= 8192, where every packed offset is below 2^26.
 template <typename T>
 __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U, T *__restrict__ UT, int64_t ustride,
                                                  int planes) {
@@ -486,13 +491,27 @@ __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U,
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int i = i0 + ty + 4 * r, j = j0 + tx;
-        if (i < n && j < n && j >= i) tile[ty + 4 * r][tx] = Us[upack_off(n, i) - i + j];
+        // U[i, j] at i n - i (i + 1) / 2 + j (row i of the packed rows starts at U[i, i])
+        if (i < n && j < n && j >= i) tile[ty + 4 * r][tx] = Us[i * n - (i * (i + 1)) / 2 + j];
     }
     __syncthreads();
+    if constexpr (V == 4) {
+        const int l = threadIdx.x & 15, cb = threadIdx.x >> 4;
+        using F4 = T __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int j = j0 + ty + 4 * r, i = i0 + tx;
-        if (j < n && i <= j) Ut[ucol_off<V>(j) + i] = tile[tx][ty + 4 * r];
+        for (int r = 0; r < 4; r++) {
+            const int c = cb + 16 * r, j = j0 + c, ir = 4 * l, i = i0 + ir;
+            if (j < n && i <= j) {
+                const F4 v = {tile[ir][c], tile[ir + 1][c], tile[ir + 2][c], tile[ir + 3][c]};
+                *reinterpret_cast<F4 *>(Ut + (int)ucol_off<4>(j) + i) = v;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int j = j0 + ty + 4 * r, i = i0 + tx;
+            if (j < n && i <= j) Ut[(int)ucol_off<V>(j) + i] = tile[tx][ty + 4 * r];
+        }
     }
 }
 
