@@ -117,6 +117,18 @@ def algorithmic(kernel, n, S, ncol, es):
     return 0, 0, "hbm"
 
 
+# HBM bytes per full-width launch at n = 4096, 64 shifts x 16 RHS, FP32
+# factors: two rocprofv3 counter passes over `bench.py --pmc-workload 3`
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the launches),
+# each scaled by what the same counter reads in the same run for k_rows_load,
+# which reads the FP64 J (134.2 MB) once and writes its row-major copy once:
+# FETCH_SIZE counts 0.501 and WRITE_SIZE 1.000 of those bytes.
+MEASURED_TRAFFIC_N4096_S64X16_F32 = {
+    "shift": 2.8259e9, "uinv": 0.1337e9, "ut_pack": 5.2995e9, "gather": 0.1923e9,
+    "fwd_chain": 0.1183e9, "utrsm": 2.2289e9, "scatter": 0.1157e9,
+}
+
+
 def roofline_of(kernel, launches, total_ms, n, S, ncol, es):
     """Roofline figures of one kernel from its live per-launch average."""
     byts, flops, bound = algorithmic(kernel, n, S, ncol, es)
@@ -127,7 +139,10 @@ def roofline_of(kernel, launches, total_ms, n, S, ncol, es):
     else:
         achieved = flops / avg_s / 1e12
     return {"bound": "hbm" if bound == "hbm" else "mfma", "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None, "kernel": kernel, "launches": launches,
+            "frac": achieved / peak,
+            "traffic": (MEASURED_TRAFFIC_N4096_S64X16_F32.get(kernel)
+                        if (n, S, ncol, es) == (4096, 64, 1024, 4) else None),
+            "kernel": kernel, "launches": launches,
             "avg_ms": total_ms / launches, "algorithmic_bytes": byts, "algorithmic_flops": flops}
 
 
@@ -146,6 +161,9 @@ def main():
                     help="1 also times the n=4096 sweep of 256 complex (i*omega, -1) + 256 real shifts x 8 RHS")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of a plain device copy run at the end (the HBM-counter calibration for --pmc runs)")
+    ap.add_argument("--pmc-workload", type=int, default=0,
+                    help="K > 0: only lhl() once and K + 1 plain batched solves per precision, then exit (the "
+                         "short program the rocprofv3 counter passes run)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -178,6 +196,18 @@ def main():
     # ---- lhl(): FP64 and FP32 factors from the resident FP64 J
     ws64 = lhl_hip.LHLWorkspace(n, np.float64)
     ws32 = lhl_hip.LHLWorkspace(n, np.float32)
+    if args.pmc_workload:
+        # every kernel of a full-width solve pass, a few times, and nothing else
+        # (k_rows_load in it reads the FP64 J once and writes the row-major
+        # copy once: the counters' calibration in the same run)
+        for name, ws in (("fp64", ws64), ("fp32", ws32)):
+            lhl_hip.lhl_(ws, Jd)
+            Xd = torch.zeros_like(Bd)
+            for _ in range(args.pmc_workload + 1):
+                lhl_hip.shift_solve_batched(ws, sig, tau, Bd, offs, X=Xd)
+            lhl_hip.sync(ws)
+            log(f"bench: pmc workload {name} done ({args.pmc_workload + 1} solve calls)")
+        return
     lhl_ms = {}
     for name, ws in (("fp64", ws64), ("fp32", ws32)):
         times = []
