@@ -121,6 +121,17 @@ def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
             shift_state_equal(lhl_hip, oracle, ws, s, t)
 
 
+@pytest.mark.parametrize("dtype,sizes", [(np.float32, [4096, 6000, 8192]), (np.float64, [4096, 4480, 4481, 6000])])
+def test_shift_bitwise_large(lhl_hip, oracle, dtype, sizes):
+    # the pipelined shift kernel at 10, 14 and 19 columns per bulk thread, and
+    # (FP64) both sides of its size limit, where the one-step kernel takes over
+    for n in sizes:
+        say(f"case: shift bitwise large n={n} {np.dtype(dtype).name}")
+        ws = lhl_hip.lhl(randn(n, n, dtype=dtype))
+        for s, t in ((1, -0.1), (0.5, -3)):
+            shift_state_equal(lhl_hip, oracle, ws, s, t)
+
+
 @pytest.mark.parametrize("tie", [False, True])
 def test_complex_shift_bitwise(lhl_hip, oracle, tie):
     for n in [1, 2, 5, 40, 100, 256, 511]:
