@@ -411,7 +411,7 @@ def main():
                           "fp64": [min(fp64["iters"]), max(fp64["iters"])]},
         "kernel_ms_per_step": {p: {k: v["ms_per_step"] for k, v in kernels[p].items()} for p in kernels},
         "roofline": roof,
-        "rooflines_fp32": {k: {f: v[f] for f in ("bound", "achieved", "unit", "frac", "avg_ms")}
+        "rooflines_fp32": {k: {f: v[f] for f in ("bound", "achieved", "unit", "frac", "avg_ms", "traffic")}
                            for k, v in kernels["fp32"].items()},
         "cpu_baseline": cpu,
     }
