@@ -167,8 +167,9 @@ inline int64_t pad_ld(int64_t n, int dtype) {
 __host__ __device__ inline int64_t upack_off(int64_t n, int64_t k) { return k * n - (k * (k - 1)) / 2; }
 inline int64_t upack_len(int64_t n) { return n * (n + 1) / 2; }
 // Elements per plane of one shift's U: the packed triangle plus 1024 spare
-// elements at the end (the shift kernel's dump slots, one per thread) and 4n
-// for the column padding of the same U in packed columns (below), rounded to 64.
+// elements at the end (the complex one-step shift kernel's dump slots, one per
+// thread) and 4n for the column padding of the same U in packed columns
+// (below), rounded to 64.
 inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 4 * n + 63) / 64) * 64; }
 
 // U in packed columns for the back substitution: column j holds U[0:j+1, j]

@@ -2,12 +2,16 @@
 // src:2456-2594; bit-for-bit spec test/core_tests.jl:119-171).
 //
 // One workgroup per shift.  The pending row r (the row of G not yet chosen as
-// a pivot row) lives in registers: thread t owns elements j = t + m*1024.  Step
-// k compares r[k] with tau*H[k+1,k], writes U row k (coalesced, packed rows
-// of U) and updates the owned r[j], j > k, against row k+1 of H (a
-// contiguous column of Ht).  The owner of r[k+1] publishes it through a
-// double-buffered LDS slot, so one barrier separates consecutive steps; the
-// next step's row of H is prefetched into registers before the barrier.
+// a pivot row) lives in registers, a strided share of its columns per thread.
+// Step k compares r[k] with tau*H[k+1,k], writes U row k (coalesced, packed
+// rows of U) and updates the owned r[j], j > k, against row k+1 of H (a
+// contiguous column of Ht).  Two forms:
+//  - k_shift_panel (real shifts up to n = 4480 in FP64, 8512 in FP32): one
+//    wave decides 63 steps at a time on a 63-column window with no barrier
+//    while the other threads apply the previous block to the rest of the row;
+//  - k_shift_real / k_shift_cplx (larger n, complex shifts): one step per
+//    barrier; the owner of r[k+1] publishes it through a double-buffered LDS
+//    slot and the next rows of H are prefetched into registers.
 //
 // Every element sees the reference's operations in the reference's order with
 // no contraction (mul, mul, then sub), and IEEE division, so the real state
