@@ -114,6 +114,26 @@ __device__ __forceinline__ void buf_st_nt(double v, BufRsrc r, int off) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 2);
 }
 
+// Two consecutive elements per lane (one 8- or 16-byte access; the range check
+// applies per dword, and the callers' offsets are never negative).  The store
+// is non-temporal, as buf_st_nt.
+template <typename T> using Vec2 = T __attribute__((ext_vector_type(2)));
+template <typename T> __device__ __forceinline__ Vec2<T> buf_ld2(BufRsrc r, int off);
+template <> __device__ __forceinline__ Vec2<float> buf_ld2<float>(BufRsrc r, int off) {
+    return __builtin_bit_cast(Vec2<float>, __builtin_amdgcn_raw_buffer_load_b64(r, buf_opaque(off), 0, 0));
+}
+template <> __device__ __forceinline__ Vec2<double> buf_ld2<double>(BufRsrc r, int off) {
+    return __builtin_bit_cast(Vec2<double>, __builtin_amdgcn_raw_buffer_load_b128(r, buf_opaque(off), 0, 0));
+}
+__device__ __forceinline__ void buf_st2_nt(Vec2<float> v, BufRsrc r, int off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 2);
+}
+__device__ __forceinline__ void buf_st2_nt(Vec2<double> v, BufRsrc r, int off) {
+    using U4 = unsigned __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, v), r, buf_opaque(off), 0, 2);
+}
+
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 

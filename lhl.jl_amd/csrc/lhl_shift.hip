@@ -7,7 +7,8 @@
 // rows of U) and updates the owned r[j], j > k, against row k+1 of H (a
 // contiguous column of Ht).  Two forms:
 //  - k_shift_panel (real shifts up to n = 4480 in FP64, 8512 in FP32): one
-//    wave decides 63 steps at a time on a 63-column window with no barrier
+//    wave decides 63 or 64 steps at a time on a window of as many columns with
+//    no barrier
 //    while the other threads apply the previous block to the rest of the row;
 //  - k_shift_real / k_shift_cplx (larger n, complex shifts): one step per
 //    barrier; the owner of r[k+1] publishes it through a double-buffered LDS
@@ -192,23 +193,44 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 }
 
 // Pipelined panel + bulk form of the same elimination (real shifts).  The
-// decisions of steps k0 .. k0+62 (block b) depend only on the pending row's
-// columns k0 .. k0+62 (window b), so wave 0 (the panel wave) runs those 63
-// steps alone with no barrier: lane L holds column k0+L, the step's pivot
+// decisions of a block of KP steps k0 .. k0+KP-1 (block b; KP = 63 or 64)
+// depend only on the pending row's columns k0 .. k0+KP-1 (window b), so wave 0
+// (the panel wave) runs those steps alone with no barrier: lane L holds column
+// k0+L, the step's pivot
 // candidates are read from lane i (a) and from the row of H (b), the window is
-// updated and its part of U rows k0 .. k0+62 stored.  It then applies the
+// updated and its part of the block's U rows stored.  It then applies the
 // block, with the multipliers still in its registers, to window b+1 (the next
-// 63 columns, handed over through LDS) and goes straight on to block b+1's
+// KP columns, handed over through LDS) and goes straight on to block b+1's
 // decisions.  Meanwhile the other NTB threads apply block b to every column
 // beyond window b+1 (each column's recurrence needs only the block's
 // multipliers and swap flags, published in LDS, and its rows of H, prefetched
-// R steps ahead across blocks) and hand window b+2 over.  One barrier per
+// R steps ahead across blocks) and hand window b+2 over.  A bulk thread owns
+// groups of W adjacent columns (W = 2 for FP64), so each of its loads and
+// stores moves W elements: the bulk is bound by its memory operations (loads
+// and stores share one in-order counter, so the stores' completion limits the
+// prefetch), and with KP even the first bulk column, a multiple of KP, never
+// splits a pair.  One barrier per
 // block separates the two; LDS buffers alternate by block parity.  U is
 // written with non-temporal stores (it is read back only after the kernel,
 // and a batch of shifts' U far exceeds the caches).  Each
 // element sees the reference's operations in the reference's order, as in the
 // one-step kernel above, so the results are bit-identical to it.
-constexpr int kPanel = 63;   // steps per block = window columns (a multiple of R and of the prefetch depth)
+// W-column groups: the scalar (W = 1) or a two-element vector (W = 2)
+template <typename T, int W> struct ColGroup;
+template <typename T> struct ColGroup<T, 1> {
+    using V = T;
+    static __device__ __forceinline__ V ld(BufRsrc r, int off) { return buf_ld<T>(r, off); }
+    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st_nt(v, r, off); }
+    static __device__ __forceinline__ T get(V v, int) { return v; }
+    static __device__ __forceinline__ void set(V &v, int, T x) { v = x; }
+};
+template <typename T> struct ColGroup<T, 2> {
+    using V = Vec2<T>;
+    static __device__ __forceinline__ V ld(BufRsrc r, int off) { return buf_ld2<T>(r, off); }
+    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st2_nt(v, r, off); }
+    static __device__ __forceinline__ T get(V v, int e) { return v[e]; }
+    static __device__ __forceinline__ void set(V &v, int e, T x) { v[e] = x; }
+};
 
 // f(integral_constant<int, v>) for the runtime value v in [M, N)
 template <int M, int N, typename F> __device__ __forceinline__ void dispatch_const(int v, F &f) {
@@ -228,7 +250,7 @@ template <> __device__ __forceinline__ double rdlane<double>(double v, int l) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 
-template <typename T, int NPER, int NTB, int PD, int R>
+template <typename T, int NP, int NTB, int PD, int R, int W, int KP>
 __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ Ht, int64_t ldh, int n,
                                                           const ShiftParam<T> *__restrict__ prm,
                                                           T *__restrict__ Ubase, int64_t ustride,
@@ -236,7 +258,11 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                                                           int64_t lstride, uint8_t *__restrict__ swbase,
                                                           int64_t sstride, int *__restrict__ infobase) {
 #pragma clang fp contract(off)
-    static_assert(kPanel % R == 0 && kPanel % PD == 0, "block of 63 steps in whole rotations");
+    constexpr int kPanel = KP;   // steps per block = window columns
+    static_assert(KP <= 64 && kPanel % R == 0 && kPanel % PD == 0, "a block of steps in whole rotations");
+    static_assert(W == 1 || KP % W == 0, "the bulk's first column starts a group");
+    using G = ColGroup<T, W>;
+    using T2 = typename G::V;
     constexpr int ES = (int)sizeof(T);
     const int s = blockIdx.x;
     const T sigma = prm[s].sr, tau = prm[s].tr;
@@ -253,25 +279,32 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     __shared__ T plast;        // the pending value of column n-1 after the last step
     const int nsteps = n - 1;
 
-    // bulk threads: columns j = tb + m NTB of the pending row r = tau H[0, :] + sigma e_0
-    T r[NPER];
-    int jb[NPER];   // byte offset of column j within a row
+    // bulk threads: the column groups j .. j+W-1 with j = W (tb + m NTB) of
+    // the pending row r = tau H[0, :] + sigma e_0
+    T2 r[NP];
+    int jb[NP];   // byte offset of column j within a row
 #pragma unroll
-    for (int m = 0; m < NPER; m++) {
-        const int j = tb + m * NTB;
-        jb[m] = j * ES;
-        r[m] = (!is_panel && j < n) ? tau * Ht[j] : (T)0;
-        if (j == 0) r[m] = r[m] + sigma;
-        if (!is_panel && j < 2 * kPanel) win[j / kPanel][j % kPanel] = r[m];   // windows 0 and 1
-        if (!is_panel && j == 0 && n == 1) plast = r[m];
+    for (int m = 0; m < NP; m++) {
+        const int j0 = W * (tb + m * NTB);
+        jb[m] = j0 * ES;
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            const int j = j0 + e;
+            T v = (!is_panel && j < n) ? tau * Ht[j] : (T)0;
+            if (j == 0) v = v + sigma;
+            G::set(r[m], e, v);
+            if (!is_panel && j < 2 * kPanel) win[j / kPanel][j % kPanel] = v;   // windows 0 and 1
+            if (!is_panel && j == 0 && n == 1) plast = v;
+        }
     }
-    // rows of H for the bulk steps, R steps ahead in rotation
-    T hs[R][NPER];
-    auto load_rows = [&](int row, T(&hd)[NPER], int ms) __attribute__((always_inline)) {
+    // rows of H for the bulk steps, R steps ahead in rotation (a group's
+    // elements at columns >= n read 0)
+    T2 hs[R][NP];
+    auto load_rows = [&](int row, T2(&hd)[NP], int ms) __attribute__((always_inline)) {
         const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(row, n - 1) * ldh, n * ES);
 #pragma unroll
-        for (int m = 0; m < NPER; m++)
-            if (m >= ms) hd[m] = buf_ld<T>(rs, jb[m]);
+        for (int m = 0; m < NP; m++)
+            if (m >= ms) hd[m] = G::ld(rs, jb[m]);
     };
     if (!is_panel) {
 #pragma unroll
@@ -291,7 +324,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         hd = buf_ld<T>(rs, (cb + lane) * ES);
     };
     // decisions of block k0 on window k0 (p holds it): steps k0 .. k0+nb-1.  A
-    // full block (nb = 63) is straight-line code, so the waits for the
+    // full block (nb = kPanel) is straight-line code, so the waits for the
     // prefetched rows count the stores between them exactly.
     auto panel_nb = [&](int k0, int nb, auto fullc) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(fullc)::value;
@@ -346,7 +379,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         if (nb == kPanel) panel_nb(k0, nb, std::true_type{});
         else panel_nb(k0, nb, std::false_type{});
     };
-    // block k0 (full, multipliers in lvec/svec) applied to window c1 = k0+63,
+    // block k0 (full, multipliers in lvec/svec) applied to window c1 = k0+kPanel,
     // whose pending values are in p; the last step's new row has its
     // diagonal (sigma) in column c1
     auto window_apply = [&](int k0) __attribute__((always_inline)) {
@@ -402,16 +435,18 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 }
             }
         } else if (c2 < n) {
-            // block b on columns c2 .. n-1, instantiated per first live chunk
-            // MS = c2 / NTB so that every step is straight-line code (exact
-            // waits for the prefetched rows)
+            // block b on columns c2 .. n-1, instantiated per first live group
+            // chunk MS = c2 / (W NTB) so that every step is straight-line code
+            // (exact waits for the prefetched rows)
             auto bulk = [&](auto msc) __attribute__((always_inline)) {
                 constexpr int MS = decltype(msc)::value;
-                int boff[NPER];   // U[k, j] at byte (j - c2) ES from U[k, c2]; j < c2 stores nothing
+                // U[k, j] at byte (j - c2) ES from U[k, c2]; groups below c2 store
+                // nothing, and a group's elements at columns >= n fall past the row
+                int boff[NP];
 #pragma unroll
-                for (int m = MS; m < NPER; m++) {
-                    const int j = tb + m * NTB;
-                    boff[m] = j >= c2 ? (j - c2) * ES : kBufNone;
+                for (int m = MS; m < NP; m++) {
+                    const int j0 = W * (tb + m * NTB);
+                    boff[m] = j0 >= c2 ? (j0 - c2) * ES : kBufNone;
                 }
                 const T lvv = lsh[bp][lane];
                 const int svv = ssh[bp][lane];
@@ -428,35 +463,37 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                         uo += n - k;
                         // values only in the two branches, then the stores: every
                         // path issues the same memory operations
-                        T u[NPER];
+                        T2 u[NP];
                         if (sk) {
 #pragma unroll
-                            for (int m = MS; m < NPER; m++) {
-                                const T g = tau * hs[q][m];
+                            for (int m = MS; m < NP; m++) {
+                                const T2 g = tau * hs[q][m];
                                 u[m] = g;
                                 r[m] = r[m] - l * g;
                             }
                         } else {
 #pragma unroll
-                            for (int m = MS; m < NPER; m++) {
-                                const T g = tau * hs[q][m];
+                            for (int m = MS; m < NP; m++) {
+                                const T2 g = tau * hs[q][m];
                                 u[m] = r[m];
                                 r[m] = g - l * r[m];
                             }
                         }
 #pragma unroll
-                        for (int m = MS; m < NPER; m++) buf_st_nt(u[m], ur, boff[m]);
+                        for (int m = MS; m < NP; m++) G::st(u[m], ur, boff[m]);
                         load_rows(k + R + 1, hs[q], MS);
                     }
                 }
-                // window b+2 = columns c2 .. c2+62 for the panel wave
+                // window b+2 = columns c2 .. c2+63 for the panel wave
 #pragma unroll
-                for (int m = MS; m < NPER; m++) {
-                    const int j = tb + m * NTB;
-                    if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = r[m];
-                }
+                for (int m = MS; m < NP; m++)
+#pragma unroll
+                    for (int e = 0; e < W; e++) {
+                        const int j = W * (tb + m * NTB) + e;
+                        if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = G::get(r[m], e);
+                    }
             };
-            dispatch_const<0, NPER>(c2 / NTB, bulk);
+            dispatch_const<0, NP>(c2 / (W * NTB), bulk);
         }
         __syncthreads();
     }
@@ -664,35 +701,47 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
     return {};
 }
 
-// the pipelined panel kernel: one panel wave + kPanelBulk bulk threads
+// The pipelined panel kernel: one panel wave + kPanelBulk bulk threads, NP
+// column groups per bulk thread.  FP32: single columns, 63-step blocks, the
+// panel's rows 21 steps ahead and the bulk's 3; FP64: column pairs, 64-step
+// blocks (even, so no pair straddles the bulk's first column), 16 and 2 steps
+// ahead (measured at n = 4096: pairs take the FP64 shift LUs 3.74 -> 3.20 ms
+// and would cost FP32 10 %).
 constexpr int kPanelBulk = 448;
-template <typename T, int NPER>
+template <typename T> constexpr int panel_w() { return sizeof(T) == 8 ? 2 : 1; }
+template <typename T, int NP>
 Status launch_panel(lhl_handle *h, int64_t nshift, const T *Ht, int64_t ldh, int n, const ShiftParam<T> *prm,
                     T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
-    auto fn = k_shift_panel<T, NPER, kPanelBulk, 21, 3>;
+    constexpr bool F64 = sizeof(T) == 8;
+    auto fn = k_shift_panel<T, NP, kPanelBulk, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
     LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(64 + kPanelBulk), 0, Ht, ldh, n, prm, U, us, l, rd, ls,
                sw, ss, info);
     return {};
 }
 
 // Real shifts: the panel kernel up to n = 4480 (FP64) / 8512 (FP32), its
-// columns per bulk thread chosen from n.  Otherwise (and complex shifts) the
-// one-step kernels: 512 threads up to n = 8192, 1024 beyond.
+// column groups per bulk thread chosen from n.  Otherwise (and complex
+// shifts) the one-step kernels: 512 threads up to n = 8192, 1024 beyond.
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
-    constexpr int B = kPanelBulk;
+    constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
     constexpr bool F32 = sizeof(T) == 4;
-    if (!cplx && n <= (F32 ? 19 : 10) * B) {
-        if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        else if (n <= 4 * B) LHL_TRY((launch_panel<T, 4>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        else if (n <= 6 * B) LHL_TRY((launch_panel<T, 6>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        else if (n <= 10 * B) LHL_TRY((launch_panel<T, 10>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        else if constexpr (F32) {
-            if (n <= 14 * B) LHL_TRY((launch_panel<T, 14>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    if (!cplx && n <= (F32 ? 19 : 5) * B) {
+        if constexpr (F32) {
+            if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 4 * B) LHL_TRY((launch_panel<T, 4>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 6 * B) LHL_TRY((launch_panel<T, 6>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 10 * B) LHL_TRY((launch_panel<T, 10>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 14 * B) LHL_TRY((launch_panel<T, 14>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else LHL_TRY((launch_panel<T, 19>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        } else {
+            if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 3 * B) LHL_TRY((launch_panel<T, 3>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 5>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
         }
         LHL_HIP_TRY(hipGetLastError());
         return {};

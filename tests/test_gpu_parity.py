@@ -111,9 +111,9 @@ def shift_state_equal(lhl_hip, oracle, ws, s, t, sh=None):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("tie", [False, True])
 def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
-    # 64, 127, 190: the last 63-step block of the pipelined kernel is full;
-    # 126, 128: a partial last block (62 steps, 1 step)
-    for n in [1, 2, 3, 5, 17, 64, 126, 127, 128, 190, 255, 256, 257, 511, 512, 520, 1000, 3000]:
+    # The pipelined kernel's last block is full at 64, 127, 190 (FP32, 63-step
+    # blocks) and 65, 129, 193 (FP64, 64-step blocks), partial at the others
+    for n in [1, 2, 3, 5, 17, 64, 65, 126, 127, 128, 129, 130, 190, 193, 255, 256, 257, 511, 512, 520, 1000, 3000]:
         say(f"case: shift bitwise n={n} {np.dtype(dtype).name} tie={tie}")
         J = tie_matrix(n, n, dtype=dtype) if tie else randn(n, n, dtype=dtype)
         ws = lhl_hip.lhl(J)
@@ -121,10 +121,12 @@ def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
             shift_state_equal(lhl_hip, oracle, ws, s, t)
 
 
-@pytest.mark.parametrize("dtype,sizes", [(np.float32, [4096, 6000, 8192]), (np.float64, [4096, 4480, 4481, 6000])])
+@pytest.mark.parametrize("dtype,sizes", [(np.float32, [2500, 4096, 6000, 8192]),
+                                         (np.float64, [2500, 4096, 4480, 4481, 6000])])
 def test_shift_bitwise_large(lhl_hip, oracle, dtype, sizes):
-    # the pipelined shift kernel at 10, 14 and 19 columns per bulk thread, and
-    # (FP64) both sides of its size limit, where the one-step kernel takes over
+    # the pipelined shift kernel at 6, 10, 14 and 19 columns (FP32) and 3 and 5
+    # column pairs (FP64) per bulk thread, and (FP64) both sides of its size
+    # limit, where the one-step kernel takes over
     for n in sizes:
         say(f"case: shift bitwise large n={n} {np.dtype(dtype).name}")
         ws = lhl_hip.lhl(randn(n, n, dtype=dtype))
