@@ -569,6 +569,32 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
 #pragma unroll
             for (int u = 0; u < NPRE; u++) pre[u] = Ui[min(tid + NT * u, kT * kT * UP - 1)];
         }
+        // Step 3 below (groups above the block: y_i -= U[i, rb:rb+nbk] x_b) is a
+        // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
+        // loads of batch t+1 are in flight while batch t's FMAs run.  Batch 0
+        // depends only on b, so its loads are issued here, before steps 1-2,
+        // and stay in flight across their LDS-only barriers.
+        const int nbat = NB * ((gb + NT - 1) / NT);   // slots holding groups < gb (nbat is even)
+        auto load_b = [&](int t, VT(&ur)[MB], VT(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
+            // threads whose group is not above the block read a valid one
+            // (unused; group 0 for block 0, which has none above it)
+            const int gc = max(min(tid + NT * (t / NB), gb - 1), 0);
+            const int col0 = rb + (t % NB) * MB;
+            // one running offset: column col+1 starts round_up(col+1, V)
+            // elements after column col; past the last column it stays there
+            // (those x_b rows are zero)
+            const T *pr = Ut + ucol_off<V>(min(col0, n - 1)) + V * gc;
+#pragma unroll
+            for (int m = 0; m < MB; m++) {
+                ur[m] = *reinterpret_cast<const VT *>(pr);
+                if constexpr (UC) ui[m] = *reinterpret_cast<const VT *>(pr + up);
+                const int col = col0 + m;
+                // branch-free (a branch here makes the compiler wait for every load)
+                pr += ((col + V) / V) * V & -(int)(col + 1 < n);
+            }
+        };
+        VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
+        load_b(0, uA, iA);
         // 1. the block's rows: groups gb .. gb+GB-1, slot gb / NT of threads
         //    gb % NT .. (one wave: GB divides 64 and gb % NT is a multiple of GB)
         {
@@ -592,7 +618,7 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
             const int pl = e / (kT * kT), ik = e % (kT * kT);
             if (e < kT * kT * UP) UiS[pl][ik / kT][ik % kT] = pre[u];
         }
-        __syncthreads();
+        lds_barrier();   // LDS only: the early U loads stay in flight
         // 2. x_b = Uinv_b y_b
         if (tid < kT * kNC) {
             T x = 0;
@@ -612,28 +638,7 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
             Xb[di][dc] = di < nbk ? x : (T)0;
             if (di < nbk && dc < g.nq) Yf[(rb + di) + (int64_t)(g.q0 + dc) * ldy] = x;
         }
-        __syncthreads();
-        // 3. groups above the block: y_i -= U[i, rb:rb+nbk] x_b.  The work is a
-        // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
-        // loads of batch t+1 are in flight while batch t's FMAs run.
-        const int nbat = NB * ((gb + NT - 1) / NT);   // slots holding groups < gb (nbat is even)
-        auto load_b = [&](int t, VT(&ur)[MB], VT(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
-            // threads whose group is not above the block read a valid one (unused)
-            const int gc = min(tid + NT * (t / NB), gb - 1);
-            const int col0 = rb + (t % NB) * MB;
-            // one running offset: column col+1 starts round_up(col+1, V)
-            // elements after column col; past the last column it stays there
-            // (those x_b rows are zero)
-            const T *pr = Ut + ucol_off<V>(min(col0, n - 1)) + V * gc;
-#pragma unroll
-            for (int m = 0; m < MB; m++) {
-                ur[m] = *reinterpret_cast<const VT *>(pr);
-                if constexpr (UC) ui[m] = *reinterpret_cast<const VT *>(pr + up);
-                const int col = col0 + m;
-                // branch-free (a branch here makes the compiler wait for every load)
-                pr += ((col + V) / V) * V & -(int)(col + 1 < n);
-            }
-        };
+        lds_barrier();   // LDS only: the early U loads stay in flight
         auto fma_b = [&](int t, const VT(&ur)[MB], const VT(&ui)[UC ? MB : 1], T(&acc)[V][kNC])
                          __attribute__((always_inline)) {
             const int m0 = (t % NB) * MB;
@@ -678,13 +683,11 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
                 for (int c = 0; c < kNC; c++) acc[r][c] = 0;
         };
         if (nbat > 0) {
-            VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
             T acc[V][kNC];
 #pragma unroll
             for (int r = 0; r < V; r++)
 #pragma unroll
                 for (int c = 0; c < kNC; c++) acc[r][c] = 0;
-            load_b(0, uA, iA);
             for (int t = 0; t < nbat; t += 2) {
                 __builtin_amdgcn_sched_barrier(0);
                 load_b(t + 1, uB, iB);
