@@ -560,6 +560,28 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
             }
         }
     const int di = tid >> 2, dc = tid & 3;
+    // batch t of block bb's update: MB columns of U for the thread's group
+    // slot t / NB (threads whose group is not above the block read a valid
+    // one, unused; group 0 for block 0, which has none above it)
+    auto load_at = [&](int bb, int t, VT(&ur)[MB], VT(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
+        const int rbb = bb * kT, gbb = rbb / V;
+        const int gc = max(min(tid + NT * (t / NB), gbb - 1), 0);
+        const int col0 = rbb + (t % NB) * MB;
+        // one running offset: column col+1 starts round_up(col+1, V)
+        // elements after column col; past the last column it stays there
+        // (those x_b rows are zero)
+        const T *pr = Ut + ucol_off<V>(min(col0, n - 1)) + V * gc;
+#pragma unroll
+        for (int m = 0; m < MB; m++) {
+            ur[m] = *reinterpret_cast<const VT *>(pr);
+            if constexpr (UC) ui[m] = *reinterpret_cast<const VT *>(pr + up);
+            const int col = col0 + m;
+            // branch-free (a branch here makes the compiler wait for every load)
+            pr += ((col + V) / V) * V & -(int)(col + 1 < n);
+        }
+    };
+    VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
+    load_at(nbu - 1, 0, uA, iA);
     for (int b = nbu - 1; b >= 0; b--) {
         const int rb = b * kT, nbk = min(kT, n - rb);
         const int gb = rb / V;                 // the block's first group; groups < gb are above it
@@ -571,30 +593,10 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
         }
         // Step 3 below (groups above the block: y_i -= U[i, rb:rb+nbk] x_b) is a
         // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
-        // loads of batch t+1 are in flight while batch t's FMAs run.  Batch 0
-        // depends only on b, so its loads are issued here, before steps 1-2,
-        // and stay in flight across their LDS-only barriers.
+        // loads of batch t+1 are in flight while batch t's FMAs run, and the
+        // last batch's successor is the next block's batch 0, so U loads stay
+        // in flight across steps 1-2 and their LDS-only barriers.
         const int nbat = NB * ((gb + NT - 1) / NT);   // slots holding groups < gb (nbat is even)
-        auto load_b = [&](int t, VT(&ur)[MB], VT(&ui)[UC ? MB : 1]) __attribute__((always_inline)) {
-            // threads whose group is not above the block read a valid one
-            // (unused; group 0 for block 0, which has none above it)
-            const int gc = max(min(tid + NT * (t / NB), gb - 1), 0);
-            const int col0 = rb + (t % NB) * MB;
-            // one running offset: column col+1 starts round_up(col+1, V)
-            // elements after column col; past the last column it stays there
-            // (those x_b rows are zero)
-            const T *pr = Ut + ucol_off<V>(min(col0, n - 1)) + V * gc;
-#pragma unroll
-            for (int m = 0; m < MB; m++) {
-                ur[m] = *reinterpret_cast<const VT *>(pr);
-                if constexpr (UC) ui[m] = *reinterpret_cast<const VT *>(pr + up);
-                const int col = col0 + m;
-                // branch-free (a branch here makes the compiler wait for every load)
-                pr += ((col + V) / V) * V & -(int)(col + 1 < n);
-            }
-        };
-        VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
-        load_b(0, uA, iA);
         // 1. the block's rows: groups gb .. gb+GB-1, slot gb / NT of threads
         //    gb % NT .. (one wave: GB divides 64 and gb % NT is a multiple of GB)
         {
@@ -690,12 +692,15 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
                 for (int c = 0; c < kNC; c++) acc[r][c] = 0;
             for (int t = 0; t < nbat; t += 2) {
                 __builtin_amdgcn_sched_barrier(0);
-                load_b(t + 1, uB, iB);
+                load_at(b, t + 1, uB, iB);
                 __builtin_amdgcn_sched_barrier(0);
                 fma_b(t, uA, iA, acc);
                 apply(t, acc);
                 __builtin_amdgcn_sched_barrier(0);
-                load_b(min(t + 2, nbat - 1), uA, iA);   // a repeat of the last batch at the end
+                // after the last batch: the next block's batch 0 (block 0 has no
+                // next: a repeat of its batch 0, unused)
+                const bool last = t + 2 >= nbat;
+                load_at(last ? max(b - 1, 0) : b, last ? 0 : t + 2, uA, iA);
                 __builtin_amdgcn_sched_barrier(0);
                 fma_b(t + 1, uB, iB, acc);
                 apply(t + 1, acc);
