@@ -196,25 +196,27 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 // decisions of a block of KP steps k0 .. k0+KP-1 (block b; KP = 63 or 64)
 // depend only on the pending row's columns k0 .. k0+KP-1 (window b), so wave 0
 // (the panel wave) runs those steps alone with no barrier: lane L holds column
-// k0+L, the step's pivot
-// candidates are read from lane i (a) and from the row of H (b), the window is
-// updated and its part of the block's U rows stored.  It then applies the
-// block, with the multipliers still in its registers, to window b+1 (the next
-// KP columns, handed over through LDS) and goes straight on to block b+1's
-// decisions.  Meanwhile the other NTB threads apply block b to every column
-// beyond window b+1 (each column's recurrence needs only the block's
-// multipliers and swap flags, published in LDS, and its rows of H, prefetched
-// R steps ahead across blocks) and hand window b+2 over.  A bulk thread owns
-// groups of W adjacent columns (W = 2 for FP64), so each of its loads and
-// stores moves W elements: the bulk is bound by its memory operations (loads
-// and stores share one in-order counter, so the stores' completion limits the
-// prefetch), and with KP even the first bulk column, a multiple of KP, never
-// splits a pair.  One barrier per
-// block separates the two; LDS buffers alternate by block parity.  U is
-// written with non-temporal stores (it is read back only after the kernel,
-// and a batch of shifts' U far exceeds the caches).  Each
-// element sees the reference's operations in the reference's order, as in the
-// one-step kernel above, so the results are bit-identical to it.
+// k0+L, the step's pivot candidates are read from lane i (a) and from the row
+// of H (b), the window is updated and its part of the block's U rows stored.
+// It then applies the block, with the multipliers still in its registers, to
+// window b+1 (the next KP columns, handed over through LDS) and goes straight
+// on to block b+1's decisions.  Meanwhile the other NTB threads apply block b
+// to every column beyond window b+1 (each column's recurrence needs only the
+// block's multipliers and swap flags, published in LDS, and its rows of H,
+// prefetched R steps ahead across blocks) and hand window b+2 over.  One
+// barrier per block separates the two; the LDS buffers alternate by block
+// parity.
+//
+// A bulk thread owns groups of W adjacent columns (W = 2 for FP64), so each of
+// its loads and stores moves W elements: the bulk is bound by its memory
+// operations (loads and stores share one in-order counter, so the stores'
+// completion limits the prefetch).  With KP even, the first bulk column, a
+// multiple of KP, never splits a pair.  U is written with non-temporal stores
+// (it is read back only after the kernel, and a batch of shifts' U far exceeds
+// the caches).  Each element sees the reference's operations in the
+// reference's order, as in the one-step kernel above, so the results are
+// bit-identical to it.
+
 // W-column groups: the scalar (W = 1) or a two-element vector (W = 2)
 template <typename T, int W> struct ColGroup;
 template <typename T> struct ColGroup<T, 1> {
