@@ -708,29 +708,38 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
 // panel's rows 21 steps ahead and the bulk's 3; FP64: column pairs, 64-step
 // blocks (even, so no pair straddles the bulk's first column), 16 and 2 steps
 // ahead (measured at n = 4096: pairs take the FP64 shift LUs 3.74 -> 3.20 ms
-// and would cost FP32 10 %).
+// and would cost FP32 10 %).  448 bulk threads are faster than 960 up to
+// n = 4096 (FP64 3.19 vs 3.74 ms, FP32 2.02 vs 2.25 ms); beyond 448 threads'
+// reach (FP64 4480, FP32 8512) the panel kernel takes 960, up to n = 13440
+// (64 shifts at n = 8192 FP64: 10.6 ms against 18.2 ms in the one-step
+// kernel; 32 at n = 13440: FP32 18.6 vs 55.0 ms, FP64 32.8 vs 163.4 ms).
+// FP32 with 19 columns per thread at 960 spills 573 registers and loses to
+// the one-step kernel (n = 16384: 78.1 vs 70.9 ms).
 constexpr int kPanelBulk = 448;
+constexpr int kPanelBulkWide = 960;
 template <typename T> constexpr int panel_w() { return sizeof(T) == 8 ? 2 : 1; }
-template <typename T, int NP>
+template <typename T, int NP, int NTB = kPanelBulk>
 Status launch_panel(lhl_handle *h, int64_t nshift, const T *Ht, int64_t ldh, int n, const ShiftParam<T> *prm,
                     T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
     constexpr bool F64 = sizeof(T) == 8;
-    auto fn = k_shift_panel<T, NP, kPanelBulk, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
-    LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(64 + kPanelBulk), 0, Ht, ldh, n, prm, U, us, l, rd, ls,
+    auto fn = k_shift_panel<T, NP, NTB, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
+    LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(64 + NTB), 0, Ht, ldh, n, prm, U, us, l, rd, ls,
                sw, ss, info);
     return {};
 }
 
-// Real shifts: the panel kernel up to n = 4480 (FP64) / 8512 (FP32), its
-// column groups per bulk thread chosen from n.  Otherwise (and complex
-// shifts) the one-step kernels: 512 threads up to n = 8192, 1024 beyond.
+// Real shifts: the panel kernel up to n = 13440, its column groups per bulk
+// thread and its bulk width chosen from n.
+// Otherwise (and complex shifts) the one-step kernels: 512 threads up to
+// n = 8192, 1024 beyond.
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
     constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
     constexpr bool F32 = sizeof(T) == 4;
-    if (!cplx && n <= (F32 ? 19 : 5) * B) {
+    constexpr int BW = panel_w<T>() * kPanelBulkWide;
+    if (!cplx && n <= (F32 ? 14 : 7) * BW) {
         if constexpr (F32) {
             if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
@@ -738,12 +747,20 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
             else if (n <= 6 * B) LHL_TRY((launch_panel<T, 6>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else if (n <= 10 * B) LHL_TRY((launch_panel<T, 10>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else if (n <= 14 * B) LHL_TRY((launch_panel<T, 14>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 19>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 19 * B) LHL_TRY((launch_panel<T, 19>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 10 * BW)
+                LHL_TRY((launch_panel<T, 10, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 14, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
         } else {
             if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
             else if (n <= 3 * B) LHL_TRY((launch_panel<T, 3>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 5>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 5 * B) LHL_TRY((launch_panel<T, 5>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 3 * BW)
+                LHL_TRY((launch_panel<T, 3, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 5 * BW)
+                LHL_TRY((launch_panel<T, 5, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 7, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
         }
         LHL_HIP_TRY(hipGetLastError());
         return {};

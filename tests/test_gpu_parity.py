@@ -121,17 +121,18 @@ def test_shift_bitwise(lhl_hip, oracle, dtype, tie):
             shift_state_equal(lhl_hip, oracle, ws, s, t)
 
 
-@pytest.mark.parametrize("dtype,sizes", [(np.float32, [2500, 4096, 6000, 8192]),
-                                         (np.float64, [2500, 4096, 4480, 4481, 6000])])
-def test_shift_bitwise_large(lhl_hip, oracle, dtype, sizes):
+@pytest.mark.parametrize("dtype,n", [(np.float32, n) for n in [2500, 4096, 6000, 8192, 8513, 9601, 13440, 13441]]
+                         + [(np.float64, n) for n in [2500, 4096, 4480, 4481, 5761, 9600, 9601, 13440, 13441]])
+def test_shift_bitwise_large(lhl_hip, oracle, dtype, n):
     # the pipelined shift kernel at 6, 10, 14 and 19 columns (FP32) and 3 and 5
-    # column pairs (FP64) per bulk thread, and (FP64) both sides of its size
-    # limit, where the one-step kernel takes over
-    for n in sizes:
-        say(f"case: shift bitwise large n={n} {np.dtype(dtype).name}")
-        ws = lhl_hip.lhl(randn(n, n, dtype=dtype))
-        for s, t in ((1, -0.1), (0.5, -3)):
-            shift_state_equal(lhl_hip, oracle, ws, s, t)
+    # column pairs (FP64) per bulk thread, both sides of the switch from 448 to
+    # 960 bulk threads (FP32 8512 / 8513, FP64 4480 / 4481) and of the group
+    # counts of the 960-thread form, up to its size limit (13440), and the
+    # one-step kernel for real shifts beyond it (13441)
+    say(f"case: shift bitwise large n={n} {np.dtype(dtype).name}")
+    ws = lhl_hip.lhl(randn(n, n, dtype=dtype))
+    for s, t in ((1, -0.1), (0.5, -3)):
+        shift_state_equal(lhl_hip, oracle, ws, s, t)
 
 
 @pytest.mark.parametrize("tie", [False, True])
