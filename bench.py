@@ -268,6 +268,15 @@ def main():
                              rel=rel)
         log(f"bench: {name} step {1e3 * dt:.2f} ms, {ncol / dt:.0f} solves/s, max rel resid {rel:.2e}, "
             f"refine iters {int(iters.min())}-{int(iters.max())}")
+        if name == "fp64":
+            X64 = Xd.clone()
+        else:
+            # each shift's error against the FP64 solution (the FP64 path is
+            # bit-checked against the oracle at sizes the oracle finishes):
+            # max over its RHS of ||x32 - x64||_inf / ||x64||_inf
+            e = ((Xd - X64).abs().amax(0) / X64.abs().amax(0)).reshape(S, R).amax(1).cpu().numpy()
+            results[name]["err_vs_fp64"] = [float(f"{v:.3g}") for v in e]
+            log(f"bench: fp32+refine error vs fp64 per shift: max {e.max():.2e}, median {np.median(e):.2e}")
 
     # ---- optional: the complex + real shift sweep on one real reduction
     sweep4 = None
@@ -407,6 +416,7 @@ def main():
         "lhl_kernel_ms": lhl_kernels,
         "lhl_roofline": {p: lhl_roofline(n, 8 if p == "fp64" else 4, lhl_ms[p]) for p in lhl_ms},
         "max_rel_resid": {"fp32_refine": fp32["rel"], "fp64": fp64["rel"]},
+        "fp32_refine_err_vs_fp64_per_shift": fp32["err_vs_fp64"],
         "refine_rounds": {"fp32": [min(fp32["iters"]), max(fp32["iters"])],
                           "fp64": [min(fp64["iters"]), max(fp64["iters"])]},
         "kernel_ms_per_step": {p: {k: v["ms_per_step"] for k, v in kernels[p].items()} for p in kernels},

@@ -111,7 +111,11 @@ int lhl_refine(lhl_handle *h, lhl_shift_state *sh, void *x, const void *W, int64
  * ldx).  precision: LHL_PREC_WORKING or LHL_PREC_REFINE64 (refine with the
  * FP64 J until tol, at most max_iter rounds).  info[s] (may be NULL) receives
  * each shift's zero-pivot index, iters[s] (may be NULL) its refinement rounds.
- * B and X are host or device memory (on_device). */
+ * B and X are host or device memory (on_device).  The shifts run in chunks of
+ * equal size whose packed U factors take at most 64 GiB, or a third of the
+ * device's free memory if less (twice that with their transposed copies); the
+ * environment variable LHL_SHIFT_POOL_BYTES, read on each call, replaces that
+ * bound.  Results do not depend on the chunking. */
 int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, const double *tau,
                             int shifts_complex, const int64_t *rhs_offsets, const double *B,
                             int64_t ldb, double *X, int64_t ldx, int on_device, int precision,

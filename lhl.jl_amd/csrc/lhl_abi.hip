@@ -5,6 +5,7 @@
 #include "lhl_common.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -400,6 +401,26 @@ static Status ensure_stream2(lhl_handle *h) {
     return {};
 }
 
+// Shifts per chunk of a batched solve.  One chunk's packed U's (its
+// transposed copies take as much again) are bounded by LHL_SHIFT_POOL_BYTES
+// when that is set, else by 64 GiB or a third of the device's free memory,
+// whichever is less.  The batch splits into chunks of equal size: a full chunk
+// plus a remainder of a few shifts would pay for two full-length launches of
+// every kernel (the shift LUs' time does not depend on their count).
+static int64_t shift_chunk(int64_t nshift, size_t ubytes) {
+    size_t budget = 64ull << 30;
+    if (const char *e = std::getenv("LHL_SHIFT_POOL_BYTES")) {
+        budget = (size_t)std::strtoull(e, nullptr, 10);
+    } else {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::min(budget, std::max<size_t>(fr / 3, 1ull << 30));
+        else (void)hipGetLastError();
+    }
+    const int64_t per = std::max<int64_t>(1, (int64_t)(budget / std::max<size_t>(ubytes, 1)));
+    const int64_t nch = (nshift + per - 1) / per;
+    return (nshift + nch - 1) / nch;
+}
+
 int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, const double *tau,
                             int shifts_complex, const int64_t *rhs_offsets, const double *B, int64_t ldb, double *X,
                             int64_t ldx, int on_device, int precision, double tol, int max_iter, int64_t *info,
@@ -438,9 +459,9 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     if (!st.ok()) return fail(h, st);
     ld = ldb2;
     plane = planeb;
-    // per-chunk pools: bound the U storage to about 8 GiB
+    // per-chunk pools, reused from chunk to chunk
     const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
-    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nshift, (int64_t)((8ull << 30) / std::max<size_t>(ubytes, 1))));
+    const int64_t chunk = shift_chunk(nshift, ubytes);
     std::vector<int> hinfo((size_t)nshift, 0);
     if (iters) for (int64_t s = 0; s < nshift; s++) iters[s] = 0;
     for (int64_t s0 = 0; s0 < nshift && st.ok(); s0 += chunk) {

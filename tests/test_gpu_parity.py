@@ -385,10 +385,11 @@ def test_batched_reports_singular_shifts(lhl_hip):
     assert single == [0, 5, 0, 40, 0]
 
 
-def test_batched_two_chunks(lhl_hip):
-    # 130 FP64 shifts at n = 4096 exceed the 8 GiB bound on one chunk's packed
-    # U's (128 shifts), so the batch runs as two chunks that reuse the shift
-    # pool; every shift must match its single-shift solve.
+def test_batched_two_chunks(lhl_hip, monkeypatch):
+    # 130 FP64 shifts at n = 4096 exceed an 8 GiB bound on one chunk's packed
+    # U's (127 shifts), so the batch runs as two chunks of 65 that reuse the
+    # shift pool; every shift must match its single-shift solve.
+    monkeypatch.setenv("LHL_SHIFT_POOL_BYTES", str(8 << 30))
     n, S = 4096, 130
     say(f"case: batched two chunks n={n} shifts={S}")
     J = stiff(130, n)
@@ -397,7 +398,7 @@ def test_batched_two_chunks(lhl_hip):
     B = np.asfortranarray(np.random.default_rng(5).standard_normal((n, S)))
     X, info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, np.arange(S + 1))
     assert np.all(info == 0)
-    for s in (0, 1, 127, 128, 129):
+    for s in (0, 1, 64, 65, 128, 129):
         lhl_hip.lhl_shift_(ws, 1, -gam[s])
         x = lhl_hip.lhl_ldiv_(B[:, s].copy(), ws)
         assert relerr(X[:, s], x) <= 1e-13
