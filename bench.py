@@ -57,6 +57,82 @@ PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
         "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
 
 
+def config5(lhl_hip, torch, dev, tol, threads):
+    """The uneven batch: 64 independent stiff J (the family above) with n drawn
+    uniformly from [128, 3000], each with 4-32 real shifts (1, -gamma), gamma
+    log-spaced in [1e-4, 1e-1], and 1-8 N(0,1) right-hand sides per shift
+    (seeded).  Per matrix: lhl(J), then one batched solve refined with FP64
+    residuals to `tol`.  Each matrix has its own handle (its own stream); the
+    matrices are handed out largest first to `threads` host threads, so the
+    small ones fill in beside the large ones on the GPU."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(505)
+    M = 64
+    ns = rng.integers(128, 3001, M)
+    nsh = rng.integers(4, 33, M)
+    nrhs = rng.integers(1, 9, M)
+    mats = []
+    for m in range(M):
+        n, S, R = int(ns[m]), int(nsh[m]), int(nrhs[m])
+        J = torch.from_numpy(stiff(5000 + m, n)).to(dev)
+        B = torch.from_numpy(np.asfortranarray(rng.standard_normal((n, S * R)))).to(dev)
+        assert J.stride(0) == 1 and B.stride(0) == 1
+        mats.append(dict(n=n, S=S, R=R, J=J, B=B, gam=np.logspace(-4, -1, S),
+                         offs=np.arange(0, S * R + 1, R, dtype=np.int64)))
+    order = sorted(range(M), key=lambda m: -(mats[m]["n"] ** 3 + mats[m]["S"] * mats[m]["n"] ** 2))
+    ncol = int(sum(d["S"] * d["R"] for d in mats))
+    res = {"workload": (f"{M} independent stiff J, n uniform in [128,3000] (sum n = {int(ns.sum())}, max "
+                        f"{int(ns.max())}), 4-32 real shifts x 1-8 RHS each ({ncol} solves in all); lhl() + one "
+                        f"batched solve refined to rel resid <= {tol:g} per matrix, one handle per matrix")}
+    for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
+        wss = [lhl_hip.LHLWorkspace(d["n"], dt) for d in mats]
+        Xs = [torch.zeros_like(d["B"]) for d in mats]
+        torch.cuda.synchronize()
+        its = [None] * M
+
+        def run(m):
+            d = mats[m]
+            lhl_hip.lhl_(wss[m], d["J"])
+            _, info, it = lhl_hip.shift_solve_batched(wss[m], np.ones(d["S"]), -d["gam"], d["B"], d["offs"],
+                                                      X=Xs[m], precision="refine64", tol=tol, max_iter=30)
+            assert np.all(info == 0)
+            its[m] = it
+
+        def job(nt):
+            if nt == 1:
+                for m in order:
+                    run(m)
+            else:
+                with ThreadPoolExecutor(nt) as ex:
+                    list(ex.map(run, order))
+            for ws in wss:
+                lhl_hip.sync(ws)
+
+        job(threads)   # warm-up: the handles' buffers, rocBLAS, code objects
+        torch.cuda.synchronize()
+        t = {}
+        for nt in (1, threads):
+            t0 = time.perf_counter()
+            job(nt)
+            torch.cuda.synchronize()
+            t[nt] = time.perf_counter() - t0
+        rel = 0.0
+        for m, d in enumerate(mats):
+            g = torch.from_numpy(np.repeat(d["gam"], d["R"])).to(dev)
+            Rd = d["B"] - (Xs[m] - (d["J"] @ Xs[m]) * g[None, :])
+            rel = max(rel, (Rd.abs().amax(0) / d["B"].abs().amax(0)).max().item())
+        itv = np.concatenate(its)
+        res[name] = {"ms": 1e3 * t[threads], "solves_per_s": ncol / t[threads], "host_threads": threads,
+                     "ms_one_thread": 1e3 * t[1], "max_rel_resid": rel,
+                     "refine_rounds": [int(itv.min()), int(itv.max())]}
+        log(f"bench: config-5 {name}: {1e3 * t[threads]:.1f} ms on {threads} host threads "
+            f"({1e3 * t[1]:.1f} ms on one), {ncol / t[threads]:.0f} solves/s, max rel resid {rel:.2e}")
+        del wss, Xs
+    res["fp32_refine_over_fp64"] = res["fp32"]["solves_per_s"] / res["fp64"]["solves_per_s"]
+    return res
+
+
 def lhl_roofline(n, es, ms):
     """lhl() against two bounds.  Unblocked (the step loop run here): each of
     the n-2 steps reads the n x (n-k) trailing columns and writes back the
@@ -159,6 +235,8 @@ def main():
     ap.add_argument("--lhl-reps", type=int, default=2)
     ap.add_argument("--config4", type=int, default=0,
                     help="1 also times the n=4096 sweep of 256 complex (i*omega, -1) + 256 real shifts x 8 RHS")
+    ap.add_argument("--config5", type=int, default=0,
+                    help="T > 0 also times the uneven batch of 64 independent matrices on T host threads")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of a plain device copy run at the end (the HBM-counter calibration for --pmc runs)")
     ap.add_argument("--pmc-workload", type=int, default=0,
@@ -324,6 +402,9 @@ def main():
                 f"max rel resid {max(relc, relr):.2e}")
             del ws4
 
+    # ---- optional: the uneven batch of independent matrices
+    sweep5 = config5(lhl_hip, torch, dev, args.tol, args.config5) if args.config5 else None
+
     # ---- roofline of every kernel of the FP32+refine step; the dominant one leads
     steps = max(args.steps, 1)
     kernels = {}
@@ -427,6 +508,8 @@ def main():
     }
     if sweep4:
         out["config4_sweep"] = sweep4
+    if sweep5:
+        out["config5_uneven"] = sweep5
     if cpu:
         out["lhl_speedup_vs_cpu"] = cpu["lhl_s"] * 1e3 / lhl_ms["fp64"]
         out["lhl_speedup_vs_reference_published_1thread"] = (cpu["reference_published_lhl_1thread_s_scaled"] * 1e3 /

@@ -137,16 +137,16 @@ __device__ __forceinline__ void buf_st2_nt(Vec2<double> v, BufRsrc r, int off) {
 constexpr int kWave = 64;
 constexpr int kMaxLds = 160 * 1024;   // bytes of LDS one workgroup may hold on gfx950
 
-// LDS one workgroup may use on the current device (160 KB on gfx950).
+// LDS one workgroup may use on the current device (160 KB on gfx950),
+// queried once (thread-safe static initialisation).
 inline size_t lds_limit() {
-    static int cached = -1;
-    if (cached < 0) {
+    static const int cached = [] {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
             v = 64 * 1024;
-        cached = v < kMaxLds ? v : kMaxLds;
-    }
+        return v < kMaxLds ? v : kMaxLds;
+    }();
     return (size_t)cached;
 }
 
@@ -163,6 +163,19 @@ struct Status {
         if (e_ != hipSuccess)                                                          \
             return ::lhl::Status{LHL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
     } while (0)
+
+// Lets kernel `fn` take all the LDS a workgroup may hold beyond its static
+// LDS as dynamic LDS.  Every caller sets the same value, so handles driven
+// from other host threads cannot lower it between this call and a launch.
+template <typename F>
+inline Status allow_dynamic_lds(F fn) {
+    hipFuncAttributes a{};
+    LHL_HIP_TRY(hipFuncGetAttributes(&a, (const void *)fn));
+    const size_t lim = lds_limit();
+    const int dyn = (int)(lim > a.sharedSizeBytes ? lim - a.sharedSizeBytes : 0);
+    LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    return {};
+}
 
 #define LHL_TRY(expr)                     \
     do {                                  \

@@ -889,7 +889,7 @@ Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds) {
     int *ilog = h->ilog.as<int>();
     LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt, g, ilog);
     auto fn = k_rows_step<T, RPW, U, PASSES>;
-    LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    LHL_TRY(allow_dynamic_lds(fn));
     for (int k = 0; k + 2 < n; k++) {
         const int a = k & 1, b = a ^ 1;
         LHL_LAUNCH(h, "reduce_step", fn, dim3(G), dim3(kRowWaves * 64), lds, B, ldb, n, k,

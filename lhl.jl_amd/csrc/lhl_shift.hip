@@ -691,12 +691,12 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
     const size_t lds = sizeof(T) * (size_t)std::max(n, 1);   // the subdiagonal of H
     if (cplx) {
         auto fn = k_shift_cplx<T, NPER, NT>;
-        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        LHL_TRY(allow_dynamic_lds(fn));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
                    info);
     } else {
         auto fn = k_shift_real<T, NPER, NT>;
-        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        LHL_TRY(allow_dynamic_lds(fn));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
                    info);
     }

@@ -139,7 +139,7 @@ Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t l
         const size_t lds1 = sizeof(T) * (size_t)n;
         if (lds1 > lds_limit() - 4096) return {LHL_ERR_ARG, "applyZ: n too large for the LDS-resident sweep"};
         auto fn = k_applyz<T, TX, 1>;
-        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+        LHL_TRY(allow_dynamic_lds(fn));
         const int groups = (int)(ncol * planes);
         LHL_LAUNCH(h, "applyz", fn, dim3(groups), dim3(kSolveThreads), lds1, n, inverse,
                            (const T *)h->A.as<T>(), h->lda, (const int *)h->perm.as<int>(),
@@ -147,7 +147,7 @@ Status applyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t l
                            (const double *)h->iscale64.as<double>(), (TX *)x, ldx, xplane, planes, (int)ncol);
     } else {
         auto fn = k_applyz<T, TX, NV>;
-        LHL_HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        LHL_TRY(allow_dynamic_lds(fn));
         const int groups = (int)((ncol * planes + NV - 1) / NV);
         LHL_LAUNCH(h, "applyz", fn, dim3(groups), dim3(kSolveThreads), lds, n, inverse,
                            (const T *)h->A.as<T>(), h->lda, (const int *)h->perm.as<int>(),

@@ -485,6 +485,36 @@ def test_config4_complex_and_real_sweep(lhl_hip, dtype, precision):
         assert _rel_resid(W, Xr[:, cols], Br[:, cols]) <= bound, (s, iters[s])
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_handles_on_host_threads(lhl_hip, dtype):
+    """Handles driven from separate host threads at once (each handle on its
+    own stream, as the uneven batch of independent matrices runs) give the
+    results of the same calls made one after another: identical pivots, and
+    solutions equal to rocBLAS's run-to-run rounding."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sizes = [700, 300, 1200, 129, 520, 64]
+    say(f"case: handles on host threads {np.dtype(dtype).name} n={sizes}")
+    mats = [stiff(600 + i, n) for i, n in enumerate(sizes)]
+
+    def run(i):
+        n, S = sizes[i], 5
+        ws = lhl_hip.lhl(mats[i], dtype=dtype)
+        B = np.asfortranarray(np.random.default_rng(i).standard_normal((n, 2 * S)))
+        X, info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -np.logspace(-4, -1, S), B,
+                                                 np.arange(0, 2 * S + 1, 2), precision="refine64", tol=1e-12,
+                                                 max_iter=30)
+        assert np.all(info == 0)
+        return ws.ipiv.copy(), X
+
+    one = [run(i) for i in range(len(sizes))]
+    with ThreadPoolExecutor(3) as ex:
+        many = list(ex.map(run, range(len(sizes))))
+    for (p1, x1), (p2, x2) in zip(one, many):
+        assert np.array_equal(p1, p2)
+        assert relerr(x2, x1) <= 1e-13
+
+
 def test_config5_uneven_batches(lhl_hip):
     """Independent stiff matrices of uneven n (scaled down from 64 matrices,
     n in [128, 3000]), each with its own shifts and right-hand-side counts,
