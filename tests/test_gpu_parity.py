@@ -154,7 +154,9 @@ def test_singular_shift_reported(lhl_hip):
     assert wd.info != 0
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 40, 129, 260, 400])
+# 513, 1025 and 1100: the L sweeps' GEMMs split once (halves below 512 rows),
+# twice, and twice with uneven halves
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 40, 129, 260, 400, 513, 1025, 1100])
 def test_shifted_solves(lhl_hip, oracle, n):
     say(f"case: shifted solves n={n}")
     J = randn(n, n)
