@@ -207,10 +207,10 @@ inline int64_t upack_plane(int64_t n) { return ((upack_len(n) + 1024 + 4 * n + 6
 
 // U in packed columns for the back substitution: column j holds U[0:j+1, j]
 // from ucol_off<V>(j), a multiple of V, so V consecutive rows of a column are
-// one aligned 16-byte load (V = 4 for FP32; V = 1, no padding, for FP64).
-// Column j takes round_up(j+1, V) slots; the total over n columns is at most
+// one aligned 16-byte load (V = 4 for FP32, V = 2 for FP64).  Column j takes
+// round_up(j+1, V) slots; the total over n columns is at most
 // n(n+1)/2 + (V-1) n.
-template <typename T> constexpr int ut_vec() { return sizeof(T) == 4 ? 4 : 1; }
+template <typename T> constexpr int ut_vec() { return sizeof(T) == 4 ? 4 : 2; }
 template <int V> __host__ __device__ inline int64_t ucol_off(int64_t j) {
     const int64_t a = j / V, b = j % V;
     return V * ((V * a * (a + 1)) / 2 + b * (a + 1));
