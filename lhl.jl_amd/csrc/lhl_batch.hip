@@ -467,10 +467,10 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
 // triangle is read along U's packed rows and written along columns through
 // LDS (both sides contiguous).  One workgroup per tile (blockIdx.x,
 // upper-triangle index) and shift-plane (blockIdx.y = shift * planes + plane).
-// FP32 (V = 4): each lane writes four consecutive rows of a column as one
-// aligned 16-byte store, 16 lanes per 64-row column segment; the rows past
+// Each lane writes V consecutive rows of a column (V = 4 FP32, 2 FP64) as one
+// aligned 16-byte store, 64 / V lanes per 64-row column segment; the rows past
 // the diagonal in a diagonal tile land in the column's padding (its slots up
-// to round_up(j+1, 4)), which the back substitution never reads.  Offsets are
+// to round_up(j+1, V)), which the back substitution never reads.  Offsets are
 // 32-bit: UT is built for n <= 8192, where every packed offset is below 2^26.
 template <typename T>
 __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U, T *__restrict__ UT, int64_t ustride,
@@ -495,15 +495,20 @@ __global__ void __launch_bounds__(256) k_ut_pack(int n, const T *__restrict__ U,
         if (i < n && j < n && j >= i) tile[ty + 4 * r][tx] = Us[i * n - (i * (i + 1)) / 2 + j];
     }
     __syncthreads();
-    if constexpr (V == 4) {
-        const int l = threadIdx.x & 15, cb = threadIdx.x >> 4;
-        using F4 = T __attribute__((ext_vector_type(4)));
+    if constexpr (V > 1) {
+        // kT / V lanes per 64-row column segment, each storing V rows as one
+        // aligned 16-byte store; 256 V / kT columns per pass
+        constexpr int LC = kT / V, CP = 256 / LC;
+        const int l = threadIdx.x % LC, cb = threadIdx.x / LC;
+        using FV = T __attribute__((ext_vector_type(V)));
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int c = cb + 16 * r, j = j0 + c, ir = 4 * l, i = i0 + ir;
+        for (int r = 0; r < kT / CP; r++) {
+            const int c = cb + CP * r, j = j0 + c, ir = V * l, i = i0 + ir;
             if (j < n && i <= j) {
-                const F4 v = {tile[ir][c], tile[ir + 1][c], tile[ir + 2][c], tile[ir + 3][c]};
-                *reinterpret_cast<F4 *>(Ut + (int)ucol_off<4>(j) + i) = v;
+                FV v;
+#pragma unroll
+                for (int e = 0; e < V; e++) v[e] = tile[ir + e][c];
+                *reinterpret_cast<FV *>(Ut + (int)ucol_off<V>(j) + i) = v;
             }
         }
     } else {
