@@ -406,6 +406,29 @@ def test_batched_two_chunks(lhl_hip, monkeypatch):
         assert relerr(X[:, s], x) <= 1e-13
 
 
+def test_batched_one_shift_per_chunk(lhl_hip, monkeypatch):
+    # a 1-byte bound on a chunk's packed U's leaves one shift per chunk; with
+    # uneven right-hand-side counts every chunk's column offsets differ, and
+    # the result must equal the one-chunk call's (rocBLAS may pick other GEMM
+    # kernels for other widths, hence the tolerance)
+    n = 300
+    J = stiff(77, n)
+    gam = np.logspace(-4, -1, 6)
+    offs = np.concatenate([[0], np.cumsum([1, 2, 3, 1, 2, 4])]).astype(np.int64)
+    B = np.asfortranarray(np.random.default_rng(9).standard_normal((n, int(offs[-1]))))
+    for dtype, precision in ((np.float64, "working"), (np.float32, "refine64")):
+        say(f"case: batched one shift per chunk {np.dtype(dtype).name} {precision}")
+        ws = lhl_hip.lhl(J, dtype=dtype)
+        X1, info1, _ = lhl_hip.shift_solve_batched(ws, np.ones(6), -gam, B, offs, precision=precision, tol=1e-12,
+                                                   max_iter=30)
+        monkeypatch.setenv("LHL_SHIFT_POOL_BYTES", "1")
+        X6, info6, _ = lhl_hip.shift_solve_batched(ws, np.ones(6), -gam, B, offs, precision=precision, tol=1e-12,
+                                                   max_iter=30)
+        monkeypatch.delenv("LHL_SHIFT_POOL_BYTES")
+        assert np.array_equal(info1, info6) and np.all(info1 == 0)
+        assert relerr(X6, X1) <= 1e-12
+
+
 def test_batched_matches_single(lhl_hip):
     n = 300
     J = stiff(11, n)
