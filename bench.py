@@ -57,6 +57,48 @@ PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
         "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
 
 
+def config2(lhl_hip, torch, dev):
+    """The plain FP64 drop-in on the stiff family, n in {2000, 4096}: lhl(J)
+    with J resident in HBM, then for each of 8 real shifts (1, -gamma), gamma
+    log-spaced in [1e-4, 1e-1], one call each of lhl_shift!, lhl_ldiv! on one
+    N(0,1) right-hand side and one lhl_refine! step with the resident J, the
+    vectors in host memory as a Julia caller passes them.  Every call is timed
+    on its own (each returns with its result in host memory)."""
+    res = {"workload": "stiff J, FP64, 8 real shifts x 1 RHS, host vectors; per-call times (median over shifts)"}
+    for n in (2000, 4096):
+        Jh = stiff(2000 + n, n)
+        J = torch.from_numpy(Jh).to(dev)
+        b = np.random.default_rng(n).standard_normal(n)
+        ws = lhl_hip.LHLWorkspace(n, np.float64)
+        lhl_hip.lhl_(ws, J)
+        lhl_hip.sync(ws)
+        t0 = time.perf_counter()
+        lhl_hip.lhl_(ws, J)
+        lhl_hip.sync(ws)
+        t_lhl = time.perf_counter() - t0
+        ts, tl, tr, rel = [], [], [], 0.0
+        for g in np.logspace(-4, -1, 8):
+            a = time.perf_counter()
+            lhl_hip.lhl_shift_(ws, 1.0, -g)
+            ts.append(time.perf_counter() - a)
+            x = b.copy()
+            a = time.perf_counter()
+            lhl_hip.lhl_ldiv_(x, ws)
+            tl.append(time.perf_counter() - a)
+            a = time.perf_counter()
+            lhl_hip.lhl_refine_(x, None, b, ws, 1)
+            tr.append(time.perf_counter() - a)
+            rel = max(rel, float(np.abs(b - (x - g * (Jh @ x))).max() / np.abs(b).max()))
+        res[f"n{n}"] = {"lhl_ms": 1e3 * t_lhl, "shift_ms": 1e3 * float(np.median(ts)),
+                        "ldiv_ms": 1e3 * float(np.median(tl)), "refine_step_ms": 1e3 * float(np.median(tr)),
+                        "max_rel_resid": rel}
+        log(f"bench: config-2 n={n}: lhl {1e3 * t_lhl:.1f} ms, per shift: lhl_shift! {1e3 * np.median(ts):.2f} ms, "
+            f"lhl_ldiv! {1e3 * np.median(tl):.2f} ms, lhl_refine! step {1e3 * np.median(tr):.2f} ms, "
+            f"max rel resid {rel:.2e}")
+        del ws
+    return res
+
+
 def config5(lhl_hip, torch, dev, tol, threads):
     """The uneven batch: 64 independent stiff J (the family above) with n drawn
     uniformly from [128, 3000], each with 4-32 real shifts (1, -gamma), gamma
@@ -235,6 +277,9 @@ def main():
     ap.add_argument("--lhl-reps", type=int, default=2)
     ap.add_argument("--config4", type=int, default=0,
                     help="1 also times the n=4096 sweep of 256 complex (i*omega, -1) + 256 real shifts x 8 RHS")
+    ap.add_argument("--config2", type=int, default=0,
+                    help="1 also times the plain FP64 drop-in calls (lhl, lhl_shift!, lhl_ldiv!, lhl_refine!) "
+                         "at n = 2000 and 4096")
     ap.add_argument("--config5", type=int, default=0,
                     help="T > 0 also times the uneven batch of 64 independent matrices on T host threads")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -402,6 +447,9 @@ def main():
                 f"max rel resid {max(relc, relr):.2e}")
             del ws4
 
+    # ---- optional: the plain drop-in calls, one shift at a time
+    sweep2 = config2(lhl_hip, torch, dev) if args.config2 else None
+
     # ---- optional: the uneven batch of independent matrices
     sweep5 = config5(lhl_hip, torch, dev, args.tol, args.config5) if args.config5 else None
 
@@ -508,6 +556,8 @@ def main():
     }
     if sweep4:
         out["config4_sweep"] = sweep4
+    if sweep2:
+        out["config2_dropin"] = sweep2
     if sweep5:
         out["config5_uneven"] = sweep5
     if cpu:
