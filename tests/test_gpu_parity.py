@@ -473,6 +473,33 @@ def test_precision_form_refines_to_1e12(lhl_hip):
             assert r <= 1e-12, (s, c, r, iters)
 
 
+def test_precision_form_n8192(lhl_hip):
+    """The precision form at n = 8192 (the larger size of its workload): FP32
+    factors of the stiff FP64 J, FP64-residual refinement to 1e-12 per column,
+    and agreement with the FP64 path's solutions."""
+    n, S, R = 8192, 4, 2
+    say(f"case: precision form n={n} shifts={S} rhs={R}")
+    J = stiff(8192, n)
+    gam = np.logspace(-4, -1, S)
+    offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+    B = np.asfortranarray(np.random.default_rng(8).standard_normal((n, S * R)))
+    ws32 = lhl_hip.lhl(J, dtype=np.float32)
+    X32, info, iters = lhl_hip.shift_solve_batched(ws32, np.ones(S), -gam, B, offs, precision="refine64",
+                                                   tol=1e-12, max_iter=30)
+    assert np.all(info == 0)
+    del ws32
+    ws64 = lhl_hip.lhl(J)
+    X64, info64, _ = lhl_hip.shift_solve_batched(ws64, np.ones(S), -gam, B, offs)
+    assert np.all(info64 == 0)
+    del ws64
+    JX = J @ X32
+    for s in range(S):
+        for c in range(offs[s], offs[s + 1]):
+            r = np.abs(B[:, c] - (X32[:, c] - gam[s] * JX[:, c])).max() / np.abs(B[:, c]).max()
+            assert r <= 1e-12, (s, c, r, iters)
+            assert relerr(X32[:, c], X64[:, c]) <= 1e-9, (s, c)
+
+
 def _rel_resid(W, X, B):
     R = B - W @ X
     return float(np.max(np.abs(R).max(axis=0) / np.abs(B).max(axis=0)))
