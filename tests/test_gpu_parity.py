@@ -500,6 +500,39 @@ def test_precision_form_n8192(lhl_hip):
             assert relerr(X32[:, c], X64[:, c]) <= 1e-9, (s, c)
 
 
+def test_solves_above_packed_column_limit(lhl_hip):
+    """n = 8200, above the n = 8192 up to which U is also kept in packed
+    columns: the back substitution runs from U's packed rows (real and complex
+    shifts).  Residuals of FP64 solves, FP32 solves refined to 1e-12, and a
+    complex-shift solve on the real reduction."""
+    n, S, R = 8200, 2, 2
+    say(f"case: solves above the packed-column limit n={n}")
+    J = stiff(8200, n)
+    gam = np.array([1e-3, 1e-1])
+    offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+    B = np.asfortranarray(np.random.default_rng(12).standard_normal((n, S * R)))
+
+    def resid(X, s, cols, shift):
+        W_X = X[:, cols] - shift * (J @ X[:, cols])
+        return np.abs(B[:, cols] - W_X).max() / np.abs(B[:, cols]).max()
+
+    for dtype, precision, bound in ((np.float64, "working", 1e-10), (np.float32, "refine64", 1e-12)):
+        ws = lhl_hip.lhl(J, dtype=dtype)
+        X, info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs, precision=precision, tol=1e-12,
+                                                 max_iter=30)
+        assert np.all(info == 0)
+        for s in range(S):
+            assert resid(X, s, slice(offs[s], offs[s + 1]), gam[s]) <= bound, (np.dtype(dtype).name, s)
+        del ws
+    ws = lhl_hip.lhl(J, shift=np.complex128)
+    om = 0.5
+    bc = (B[:, 0] + 1j * B[:, 1]).copy()
+    lhl_hip.lhl_shift_(ws, 1j * om, -1.0)
+    x = lhl_hip.lhl_ldiv_(bc.copy(), ws)
+    r = np.abs(bc - (1j * om * x - J @ x)).max() / np.abs(bc).max()
+    assert r <= 1e-10, r
+
+
 def _rel_resid(W, X, B):
     R = B - W @ X
     return float(np.max(np.abs(R).max(axis=0) / np.abs(B).max(axis=0)))
