@@ -184,6 +184,7 @@ int lhl_destroy(lhl_handle *h) {
     }
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->hpin) (void)hipHostFree(h->hpin);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LHL_OK;
@@ -401,6 +402,21 @@ static Status ensure_stream2(lhl_handle *h) {
     return {};
 }
 
+// The handle's pinned host buffer, at least `bytes` long.  Every transfer into
+// or out of it has completed when a call returns (each call ends by
+// synchronizing its streams), so a larger one can replace it at once.
+static Status host_pinned(lhl_handle *h, size_t bytes, char **out) {
+    if (h->hpin_bytes < bytes) {
+        if (h->hpin) LHL_HIP_TRY(hipHostFree(h->hpin));
+        h->hpin = nullptr;
+        h->hpin_bytes = 0;
+        LHL_HIP_TRY(hipHostMalloc(&h->hpin, bytes, hipHostMallocDefault));
+        h->hpin_bytes = bytes;
+    }
+    *out = static_cast<char *>(h->hpin);
+    return {};
+}
+
 // Shifts per chunk of a batched solve.  One chunk's packed U's (its
 // transposed copies take as much again) are bounded by LHL_SHIFT_POOL_BYTES
 // when that is set, else by 64 GiB or a third of the device's free memory,
@@ -462,7 +478,19 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     // per-chunk pools, reused from chunk to chunk
     const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
     const int64_t chunk = shift_chunk(nshift, ubytes);
-    std::vector<int> hinfo((size_t)nshift, 0);
+    // pinned host layout: zero-pivot flags of every shift, then one chunk's
+    // per-column shifts (4 doubles) and residual norms (2 doubles)
+    int64_t maxnc = 0;
+    for (int64_t s0 = 0; s0 < nshift; s0 += chunk)
+        maxnc = std::max(maxnc, rhs_offsets[std::min(s0 + chunk, nshift)] - rhs_offsets[s0]);
+    const size_t info_bytes = ((sizeof(int) * (size_t)nshift + 63) / 64) * 64;
+    char *pin = nullptr;
+    st = host_pinned(h, info_bytes + 6 * sizeof(double) * (size_t)std::max<int64_t>(maxnc, 1), &pin);
+    if (!st.ok()) return fail(h, st);
+    int *pinfo = reinterpret_cast<int *>(pin);
+    double *pcs = reinterpret_cast<double *>(pin + info_bytes);
+    double *pnorm = pcs + 4 * std::max<int64_t>(maxnc, 1);
+    std::memset(pinfo, 0, sizeof(int) * (size_t)nshift);
     if (iters) for (int64_t s = 0; s < nshift; s++) iters[s] = 0;
     for (int64_t s0 = 0; s0 < nshift && st.ok(); s0 += chunk) {
         const int64_t cnt = std::min(chunk, nshift - s0);
@@ -484,6 +512,22 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.ldx = ld;
         a.xplane = plane;
         a.accumulate = 0;
+        if (precision == LHL_PREC_REFINE64 && nc > 0) {
+            // each column's (sigma, tau) for the residuals, uploaded ahead of the
+            // first pass (the previous chunk's upload completed at its first
+            // residual read-back)
+            for (int64_t s = 0; s < cnt; s++)
+                for (int64_t c = cb[s]; c < cb[s + 1]; c++) {
+                    pcs[4 * c] = sigma[2 * (s0 + s)];
+                    pcs[4 * c + 1] = sigma[2 * (s0 + s) + 1];
+                    pcs[4 * c + 2] = tau[2 * (s0 + s)];
+                    pcs[4 * c + 3] = tau[2 * (s0 + s) + 1];
+                }
+            st = colshift.ensure(sizeof(double) * 4 * (size_t)maxnc);
+            if (!st.ok()) break;
+            e = hipMemcpyAsync(colshift.p, pcs, sizeof(double) * 4 * (size_t)nc, hipMemcpyHostToDevice, h->stream);
+            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
+        }
         if (nc > 0) {
             // the first pass's gather and L solve read no shift state: they run
             // on the second stream while the shift LUs run on the handle's
@@ -523,24 +567,15 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
             if (!st.ok()) break;
             a.phase = 0;
         }
-        // the shift kernel wrote dinfo on the handle's stream: wait for it
-        e = hipStreamSynchronize(h->stream);
-        if (e == hipSuccess) e = hipMemcpy(hinfo.data() + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost);
+        // the zero-pivot flags, read on the host once the call's streams have
+        // drained (the next chunk's shift LUs come after this copy on the stream)
+        e = hipMemcpyAsync(pinfo + s0, pool.dinfo.p, sizeof(int) * cnt, hipMemcpyDeviceToHost, h->stream);
         if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
         if (nc == 0) continue;
         if (precision == LHL_PREC_REFINE64) {
             st = rst.ensure(8 * (size_t)plane * planes + 64);
             if (st.ok()) st = norms.ensure(sizeof(double) * 2 * (size_t)nc);
-            std::vector<double> cs((size_t)nc * 4);
-            for (int64_t s = 0; s < cnt; s++)
-                for (int64_t c = cb[s]; c < cb[s + 1]; c++) {
-                    cs[4 * c] = sigma[2 * (s0 + s)]; cs[4 * c + 1] = sigma[2 * (s0 + s) + 1];
-                    cs[4 * c + 2] = tau[2 * (s0 + s)]; cs[4 * c + 3] = tau[2 * (s0 + s) + 1];
-                }
-            if (st.ok()) st = colshift.ensure(8 * cs.size());
             if (!st.ok()) break;
-            e = hipMemcpy(colshift.p, cs.data(), 8 * cs.size(), hipMemcpyHostToDevice);
-            if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
             ResidArgs ra;
             ra.colshift = colshift.as<double>();
             ra.b = bst.as<double>() + c0 * ld;
@@ -557,7 +592,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
             // residual).  U, l, rdiag, swap flags, Uinv and UT sit at fixed
             // per-shift strides, so a range of shifts is a pointer offset.
             const size_t es = elt_size(h->dtype);
-            std::vector<double> hn((size_t)2 * nc);
+            const double *hn = pnorm;   // the last read-back's norms
             int64_t lo = 0, hi = cnt;   // active shifts [lo, hi) of the chunk
             for (int it = 0; it < max_iter; it++) {
                 const int64_t clo = cb[lo], na = cb[hi] - clo;
@@ -569,7 +604,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                 rr.ncol = na;
                 st = residual_device(h, rr);
                 if (!st.ok()) break;
-                e = hipMemcpyAsync(hn.data(), norms.p, sizeof(double) * 2 * na, hipMemcpyDeviceToHost, h->stream);
+                e = hipMemcpyAsync(pnorm, norms.p, sizeof(double) * 2 * na, hipMemcpyDeviceToHost, h->stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
                 if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
                 int64_t nlo = -1, nhi = -1;
@@ -621,7 +656,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     for (DevBuf *b : {&bst, &rst, &norms, &colshift, &pool.U, &pool.l, &pool.rdiag, &pool.swp, &pool.dinfo,
                       &pool.params, &pool.Uinv, &pool.UT})
         b->release();
-    if (info) for (int64_t s = 0; s < nshift; s++) info[s] = hinfo[s];
+    if (info) for (int64_t s = 0; s < nshift; s++) info[s] = pinfo[s];
     if (!st.ok()) return fail(h, st);
     return LHL_OK;
 }

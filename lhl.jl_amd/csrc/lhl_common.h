@@ -341,6 +341,10 @@ struct lhl_handle {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     void *blas_stream = nullptr;    // the stream the rocBLAS handle is set to
+    // pinned host memory for the batched solve's small transfers (zero-pivot
+    // flags, per-column shifts, residual norms), grown on demand
+    void *hpin = nullptr;
+    size_t hpin_bytes = 0;
     int64_t lda = 0;            // leading dimension of A and Ht
     int64_t ldj = 0;            // leading dimension of the resident J
     int j_dtype = LHL_F64;      // element type of the resident J
