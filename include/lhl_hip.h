@@ -38,7 +38,12 @@ enum {
     LHL_ERR_DIM = 2,    /* DimensionMismatch */
     LHL_ERR_HIP = 3,    /* a HIP runtime error */
     LHL_ERR_STATE = 4,  /* call out of order (e.g. shift before reduce) */
-    LHL_ERR_NOMEM = 5
+    LHL_ERR_NOMEM = 5,
+    /* Not an error of the call: with tol > 0, a column's relative residual
+     * is still above tol after the allowed refinement rounds.  Every output is
+     * written (x, X, info, iters) as on LHL_OK.  The reference's lhl_refine!
+     * runs a fixed number of steps and has no such outcome (src:3147-3157). */
+    LHL_NOT_CONVERGED = 6
 };
 
 enum { LHL_F64 = 0, LHL_F32 = 1 };
@@ -91,11 +96,14 @@ int lhl_ldiv(lhl_handle *h, lhl_shift_state *sh, void *x, int64_t n, int64_t nrh
 
 /* lhl_refine!(x, A, b, ws, steps) / lhl_refine!(x, A, b, sh, ws, steps)
  * (src:3144-3157): up to `steps` rounds of r = b - W x ; ldiv(r) ; x += r.
- * W is the dense operator (n x n, ldw, element type w_dtype, complex if
- * x_complex) or NULL, which means sigma*I + tau*J with the handle's resident J
- * (FP64 when the handle was reduced from an FP64 J).  With tol > 0 the loop
- * stops once every column has ||b - W x||_inf / ||b||_inf <= tol; *iters
- * (may be NULL) receives the rounds done.  The residual and the update are
+ * W is the dense operator (n x n, ldw, the element type of x, x_dtype, with
+ * interleaved complex values if x_complex) or NULL, which means
+ * sigma*I + tau*J with the handle's resident J (FP64 when the handle was
+ * reduced from an FP64 J).  b has x's element type, layout and location.
+ * With tol > 0 the loop stops once every column has
+ * ||b - W x||_inf / ||b||_inf <= tol, and the solution of the last round is
+ * checked too: LHL_NOT_CONVERGED if a column is still above tol.  *iters (may
+ * be NULL) receives the rounds done.  The residual and the update are
  * computed in the precision of x. */
 int lhl_refine(lhl_handle *h, lhl_shift_state *sh, void *x, const void *W, int64_t ldw,
                const void *b, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype, int x_complex,
@@ -120,6 +128,12 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                             int shifts_complex, const int64_t *rhs_offsets, const double *B,
                             int64_t ldb, double *X, int64_t ldx, int on_device, int precision,
                             double tol, int max_iter, int64_t *info, int *iters);
+
+/* (lhl_shift_solve_batched, continued) With precision LHL_PREC_REFINE64 the
+ * solutions of the last round are checked as well: the call returns
+ * LHL_NOT_CONVERGED, with every output written, when some shift still has a
+ * column above tol after max_iter rounds.  rhs_offsets must start at 0 or
+ * above and never decrease (LHL_ERR_DIM otherwise). */
 
 /* applyZ!(x, ws) / applyZinv!(x, ws) (src:2069-2126) on nrhs columns of x
  * (element type x_dtype; complex interleaved if x_complex). */
@@ -151,6 +165,14 @@ int lhl_get_ht(lhl_handle *h, void *ht);
 int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh, void *U, void *l, uint8_t *swap,
                         void *rdiag, double *sigma, double *tau, int64_t *info);
 
+/* The scalar fields of a shift state (sh NULL = ws.shift) without a device
+ * copy: ws.info (1-based first zero pivot, 0 if none), ws.sigma, ws.tau (each
+ * {re, im}) and whether a shift is factored for the current reduction.  The
+ * property forwarding of src:182-191 (ws.info, ws.sigma, ws.tau) reads
+ * these.  Any pointer may be NULL. */
+int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double *sigma, double *tau,
+                       int *loaded);
+
 /* Synchronize the handle's stream. */
 int lhl_sync(lhl_handle *h);
 
@@ -170,6 +192,12 @@ int lhl_get_stream(lhl_handle *h, void **stream);
 int lhl_profile_enable(lhl_handle *h, int on);
 int lhl_profile_reset(lhl_handle *h);
 int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms);
+/* The same, plus the work the launches were given, summed: shift_units
+ * counts the shifts each launch processed, col_units the internal
+ * right-hand-side columns (complex columns count once per plane); 0 for
+ * launches that do not split that way.  Either pointer may be NULL. */
+int lhl_profile_read_work(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms,
+                          int64_t *shift_units, int64_t *col_units);
 
 #ifdef __cplusplus
 }

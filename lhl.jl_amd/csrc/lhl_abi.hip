@@ -119,10 +119,12 @@ void prof_drain(lhl_handle *h) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) ms = 0.f;
         auto &acc = h->prof.acc;
-        auto it = std::find_if(acc.begin(), acc.end(), [&](const auto &e) { return e.first == r.name; });
-        if (it == acc.end()) { acc.push_back({r.name, {0, 0.0}}); it = acc.end() - 1; }
-        it->second.first += 1;
-        it->second.second += ms;
+        auto it = std::find_if(acc.begin(), acc.end(), [&](const ProfAcc &e) { return e.name == r.name; });
+        if (it == acc.end()) { acc.push_back(ProfAcc{r.name}); it = acc.end() - 1; }
+        it->launches += 1;
+        it->ms += ms;
+        it->ws += r.ws;
+        it->wc += r.wc;
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
@@ -192,6 +194,7 @@ int lhl_destroy(lhl_handle *h) {
 
 int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int balance, int j_on_device) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     if (n < 0) return fail(h, LHL_ERR_DIM, "lhl_reduce: negative size");
     if (j_dtype != LHL_F64 && j_dtype != LHL_F32) return fail(h, LHL_ERR_ARG, "lhl_reduce: J must be Float64 or Float32");
     if (n > 0 && (!J || ldj < n)) return fail(h, LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n");
@@ -262,6 +265,7 @@ int lhl_shift_destroy(lhl_shift_state *s) {
 
 int lhl_shift(lhl_handle *h, lhl_shift_state *sh_, double sr, double si, double tr, double ti, int64_t *info) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     lhl_shift_state *sh = resolve_shift(h, sh_);
     if (sh->h != h) return fail(h, LHL_ERR_ARG, "lhl_shift: the shift state belongs to another workspace");
     if (!sh->is_complex && (si != 0.0 || ti != 0.0))
@@ -279,6 +283,7 @@ int lhl_shift(lhl_handle *h, lhl_shift_state *sh_, double sr, double si, double 
 int lhl_ldiv(lhl_handle *h, lhl_shift_state *sh_, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
              int x_complex, int on_device) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     lhl_shift_state *sh = resolve_shift(h, sh_);
     if (n != h->n)
         return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
@@ -311,6 +316,7 @@ int lhl_ldiv(lhl_handle *h, lhl_shift_state *sh_, void *x, int64_t n, int64_t nr
 int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int64_t ldw, const void *b, int64_t n,
                int64_t nrhs, int64_t ldx, int xdt, int x_complex, int on_device, int steps, double tol, int *iters) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     if (iters) *iters = 0;
     lhl_shift_state *sh = resolve_shift(h, sh_);
     if (n != h->n) return fail(h, LHL_ERR_DIM, "lhl_refine: x has the wrong length");
@@ -366,8 +372,10 @@ int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int6
     ra.x_complex = x_complex;
     ra.norms = norms.as<double>();
     int done = 0;
+    bool unconverged = false;
     std::vector<double> hn((size_t)2 * nrhs);
-    for (int it = 0; it < steps; it++) {
+    // with tol > 0 the solution of the last round is checked too
+    for (int it = 0; it < steps || (tol > 0 && it == steps); it++) {
         st = residual_device(h, ra);
         if (!st.ok()) break;
         if (tol > 0) {
@@ -378,6 +386,7 @@ int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int6
             for (int64_t c = 0; c < nrhs; c++)
                 if (!(hn[c] <= tol * hn[nrhs + c])) ok = false;
             if (ok) break;
+            if (it == steps) { unconverged = true; break; }
         }
         st = do_solve_staged(h, sh, rst.p, h->xstage.p, ld, plane, nrhs, sdt, x_complex, 1);
         if (!st.ok()) break;
@@ -391,6 +400,9 @@ int lhl_refine(lhl_handle *h, lhl_shift_state *sh_, void *x, const void *W, int6
     bst.release(); rst.release(); wst.release(); colshift.release(); norms.release();
     if (iters) *iters = done;
     if (!st.ok()) return fail(h, st);
+    if (unconverged)
+        return fail(h, LHL_NOT_CONVERGED, "lhl_refine: a column is still above tol after " + std::to_string(steps) +
+                                          " rounds");
     return LHL_OK;
 }
 
@@ -442,6 +454,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                             int64_t ldx, int on_device, int precision, double tol, int max_iter, int64_t *info,
                             int *iters) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     if (nshift < 0 || (nshift > 0 && (!sigma || !tau || !rhs_offsets))) return fail(h, LHL_ERR_ARG, "batched: bad shift arrays");
     if (!h->reduced) return fail(h, LHL_ERR_STATE, "batched: no reduction held");
     if (precision != LHL_PREC_WORKING && precision != LHL_PREC_REFINE64) return fail(h, LHL_ERR_ARG, "batched: bad precision");
@@ -452,9 +465,14 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
             if (sigma[2 * s + 1] != 0.0 || tau[2 * s + 1] != 0.0)
                 return fail(h, LHL_ERR_ARG, "batched: complex shift with shifts_complex = 0");
     const int64_t n = h->n;
+    if (nshift > 0 && rhs_offsets[0] < 0) return fail(h, LHL_ERR_DIM, "batched: negative right-hand-side offset");
+    for (int64_t s = 0; s < nshift; s++)
+        if (rhs_offsets[s + 1] < rhs_offsets[s])
+            return fail(h, LHL_ERR_DIM, "batched: right-hand-side offsets decrease at shift " + std::to_string(s));
     const int64_t ncol = nshift > 0 ? rhs_offsets[nshift] - rhs_offsets[0] : 0;
     if (n == 0 || nshift == 0) return LHL_OK;
-    if (ncol < 0 || ldb < n || ldx < n) return fail(h, LHL_ERR_DIM, "batched: bad right-hand-side layout");
+    if (ldb < n || ldx < n) return fail(h, LHL_ERR_DIM, "batched: bad right-hand-side layout");
+    if (ncol > 0 && (!B || !X)) return fail(h, LHL_ERR_ARG, "batched: null B or X");
     Status st = set_device(h);
     if (!st.ok()) return fail(h, st);
     const int cplx = shifts_complex ? 1 : 0;
@@ -488,6 +506,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     st = host_pinned(h, info_bytes + 6 * sizeof(double) * (size_t)std::max<int64_t>(maxnc, 1), &pin);
     if (!st.ok()) return fail(h, st);
     int *pinfo = reinterpret_cast<int *>(pin);
+    int64_t unconverged = 0;   // shifts with a column above tol after max_iter rounds
     double *pcs = reinterpret_cast<double *>(pin + info_bytes);
     double *pnorm = pcs + 4 * std::max<int64_t>(maxnc, 1);
     std::memset(pinfo, 0, sizeof(int) * (size_t)nshift);
@@ -594,7 +613,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
             const size_t es = elt_size(h->dtype);
             const double *hn = pnorm;   // the last read-back's norms
             int64_t lo = 0, hi = cnt;   // active shifts [lo, hi) of the chunk
-            for (int it = 0; it < max_iter; it++) {
+            for (int it = 0; it <= max_iter; it++) {
                 const int64_t clo = cb[lo], na = cb[hi] - clo;
                 ResidArgs rr = ra;
                 rr.colshift = ra.colshift + 4 * clo;
@@ -607,7 +626,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                 e = hipMemcpyAsync(pnorm, norms.p, sizeof(double) * 2 * na, hipMemcpyDeviceToHost, h->stream);
                 if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
                 if (e != hipSuccess) { st = Status{LHL_ERR_HIP, hipGetErrorString(e)}; break; }
-                int64_t nlo = -1, nhi = -1;
+                int64_t nlo = -1, nhi = -1, nbad = 0;
                 for (int64_t s = lo; s < hi; s++) {
                     bool ok = true;
                     for (int64_t c = cb[s]; c < cb[s + 1]; c++)
@@ -615,10 +634,15 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
                     if (!ok) {
                         if (nlo < 0) nlo = s;
                         nhi = s + 1;
-                        if (iters) iters[s0 + s] = it + 1;
+                        nbad++;
+                        if (iters && it < max_iter) iters[s0 + s] = it + 1;
                     }
                 }
                 if (nlo < 0) break;
+                if (it == max_iter) {   // the last round's solutions are checked, not refined again
+                    unconverged += nbad;
+                    break;
+                }
                 lo = nlo;
                 hi = nhi;
                 std::vector<int64_t> cbr((size_t)(hi - lo + 1));
@@ -658,12 +682,16 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         b->release();
     if (info) for (int64_t s = 0; s < nshift; s++) info[s] = pinfo[s];
     if (!st.ok()) return fail(h, st);
+    if (unconverged > 0)
+        return fail(h, LHL_NOT_CONVERGED, "batched: " + std::to_string(unconverged) + " shift(s) still have a column "
+                                          "above tol after " + std::to_string(max_iter) + " refinement rounds");
     return LHL_OK;
 }
 
 static int applyz_common(lhl_handle *h, int inverse, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
                          int x_complex, int on_device) {
     if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
     if (n != h->n)
         return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
                                         "x" + std::to_string(h->n));
@@ -752,6 +780,18 @@ int lhl_get_ht(lhl_handle *h, void *f) {
     return LHL_OK;
 }
 
+int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh_, int64_t *info, double *sigma, double *tau,
+                       int *loaded) {
+    if (!h) return LHL_ERR_ARG;
+    const lhl_shift_state *sh = resolve_shift(h, sh_);
+    if (sh->h != h) return fail(h, LHL_ERR_ARG, "lhl_get_shift_info: the shift state belongs to another workspace");
+    if (info) *info = sh->info;
+    if (sigma) { sigma[0] = sh->sigma[0]; sigma[1] = sh->sigma[1]; }
+    if (tau) { tau[0] = sh->tau[0]; tau[1] = sh->tau[1]; }
+    if (loaded) *loaded = sh->loaded && h->reduced;
+    return LHL_OK;
+}
+
 int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh_, void *U, void *l, uint8_t *swap, void *rdiag,
                         double *sigma, double *tau, int64_t *info) {
     if (!h) return LHL_ERR_ARG;
@@ -812,17 +852,23 @@ int lhl_profile_reset(lhl_handle *h) {
     return LHL_OK;
 }
 
-int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms) {
+int lhl_profile_read_work(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms,
+                          int64_t *shift_units, int64_t *col_units) {
     if (!h || !kernel) return LHL_ERR_ARG;
     if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) return LHL_ERR_HIP;
     prof_drain(h);
-    int64_t c = 0;
-    double t = 0;
-    for (const auto &e : h->prof.acc)
-        if (e.first == kernel) { c = e.second.first; t = e.second.second; }
-    if (launches) *launches = c;
-    if (total_ms) *total_ms = t;
+    ProfAcc r;
+    for (const ProfAcc &e : h->prof.acc)
+        if (e.name == kernel) r = e;
+    if (launches) *launches = r.launches;
+    if (total_ms) *total_ms = r.ms;
+    if (shift_units) *shift_units = r.ws;
+    if (col_units) *col_units = r.wc;
     return LHL_OK;
+}
+
+int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms) {
+    return lhl_profile_read_work(h, kernel, launches, total_ms, nullptr, nullptr);
 }
 
 int lhl_sync(lhl_handle *h) {

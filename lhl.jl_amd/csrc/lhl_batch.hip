@@ -1058,6 +1058,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_TRY(h->ybuf.ensure(sizeof(T) * (size_t)ldy * Q));
     LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
     T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
+    ProfWork pw(h, a.nshift, Q);
     const T *A = h->A.as<T>();
     if (a.phase != 2) {
         // y = P^-1 D^-1 x, then y' <- L'^-1 y' as a product with the explicit
@@ -1100,9 +1101,10 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     UGroup *dg = h->gbuf.as<UGroup>();
     // ordered on the handle's stream: an earlier solve's back substitution may
     // still be reading the table
-    h->host_ug.resize(sizeof(UGroup) * ug.size());
-    std::memcpy(h->host_ug.data(), ug.data(), h->host_ug.size());
-    LHL_HIP_TRY(hipMemcpyAsync(dg, h->host_ug.data(), h->host_ug.size(), hipMemcpyHostToDevice, h->stream));
+    const size_t ugb = sizeof(UGroup) * ug.size();
+    unsigned char *hug = host_keep(h, ugb);
+    std::memcpy(hug, ug.data(), ugb);
+    LHL_HIP_TRY(hipMemcpyAsync(dg, hug, ugb, hipMemcpyHostToDevice, h->stream));
 
     const int N1 = n - 1;
     // forward interchange chain in place on Yo: a scan per column
@@ -1176,6 +1178,7 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = (int64_t)nbu * kT * kT * (cplx ? 2 : 1);
     LHL_TRY(Uinv.ensure(sizeof(T) * (size_t)uis * (size_t)nshift));
+    ProfWork pw(h, nshift, 0);
     *uistride = uis;
     if (cplx)
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
@@ -1200,6 +1203,7 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     const int planes = a.x_complex ? 2 : 1;
     const int Q = (int)(a.ncol * planes);
     const int64_t ldq = ldy_for(n);
+    ProfWork pw(h, 0, Q);
     LHL_TRY(h->ybuf.ensure(sizeof(TX) * (size_t)ldq * Q));
     TX *JX = h->ybuf.as<TX>();
     TX *JXi = nullptr;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -302,14 +303,25 @@ struct RedCtl {
 // Opt-in kernel timing: while on, every launch through LHL_LAUNCH is
 // bracketed by two HIP events on the handle's stream; durations are summed per
 // kernel family when read.
+// Each launch also carries the work it was given (`ws` shifts, `wc` internal
+// right-hand-side columns; both 0 where a launch does not split that way), so
+// per-launch algorithmic bytes can be priced from what the launch processed.
 struct ProfRec {
     const char *name;
     hipEvent_t a, b;
+    int64_t ws, wc;
+};
+struct ProfAcc {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0.0;
+    int64_t ws = 0, wc = 0;   // summed work tags
 };
 struct Profiler {
     bool on = false;
     std::vector<ProfRec> pending;
-    std::vector<std::pair<std::string, std::pair<int64_t, double>>> acc;
+    std::vector<ProfAcc> acc;
+    int64_t ws = 0, wc = 0;   // work tag of the launches being issued
 };
 
 }  // namespace lhl
@@ -353,9 +365,10 @@ struct lhl_handle {
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
-    // host sources of the solves' work tables: copied to the device on the
-    // stream, kept until the next solve (every caller synchronizes between solves)
-    std::vector<unsigned char> host_ug;
+    // host sources of asynchronous uploads (the solves' work tables, the
+    // shifts' parameters): each upload reads its own block, kept until the
+    // next public call begins (every call ends by synchronizing its streams)
+    std::vector<std::vector<unsigned char>> host_kept;
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), dense unit lower, leading dimension ldlf
     lhl::DevBuf Lp1;            // I + L' as a dense unit lower matrix (solve), leading dimension ldlf
@@ -380,10 +393,39 @@ struct ProfScope {
     ~ProfScope() {
         if (a && hipEventCreate(&b) == hipSuccess) {
             (void)hipEventRecord(b, h->stream);
-            h->prof.pending.push_back(ProfRec{name, a, b});
+            h->prof.pending.push_back(ProfRec{name, a, b, h->prof.ws, h->prof.wc});
         }
     }
 };
+
+// Sets the work tag of the launches issued while it lives (restored after).
+struct ProfWork {
+    lhl_handle *h;
+    int64_t ows, owc;
+    ProfWork(lhl_handle *h_, int64_t shifts, int64_t cols) : h(h_), ows(h_->prof.ws), owc(h_->prof.wc) {
+        h->prof.ws = shifts;
+        h->prof.wc = cols;
+    }
+    ~ProfWork() {
+        h->prof.ws = ows;
+        h->prof.wc = owc;
+    }
+};
+
+// A host block of `bytes` bytes that stays valid until the handle's next
+// public call: the source of an asynchronous host-to-device copy.
+inline unsigned char *host_keep(lhl_handle *h, size_t bytes) {
+    h->host_kept.emplace_back(std::max<size_t>(bytes, 1));
+    return h->host_kept.back().data();
+}
+// Drops the blocks of the previous call (which ended synchronized).
+// (The streams are drained first in case an earlier call left by an error.)
+inline void host_keep_reset(lhl_handle *h) {
+    if (h->host_kept.empty()) return;
+    (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    h->host_kept.clear();
+}
 
 #define LHL_LAUNCH(h, name, kern, grid, block, lds, ...)                            \
     do {                                                                           \

@@ -19,6 +19,7 @@
 // (U, multipliers, swap, rdiag, info) equals the reference's bit for bit.
 #include "lhl_common.h"
 
+#include <cstring>
 #include <type_traits>
 
 namespace lhl {
@@ -789,10 +790,13 @@ Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, co
         hp[s].ti = (T)tau[2 * s + 1];
     }
     LHL_TRY(params.ensure(sizeof(ShiftParam<T>) * (size_t)nshift));
-    LHL_HIP_TRY(hipMemcpyAsync(params.p, hp.data(), sizeof(ShiftParam<T>) * (size_t)nshift,
-                               hipMemcpyHostToDevice, h->stream));
-    // the host vector must outlive the async copy
-    LHL_HIP_TRY(hipStreamSynchronize(h->stream));
+    // ordered on the stream; the host copy is kept by the handle until the
+    // next call (every call ends by synchronizing its streams)
+    unsigned char *keep = host_keep(h, sizeof(ShiftParam<T>) * (size_t)nshift);
+    std::memcpy(keep, hp.data(), sizeof(ShiftParam<T>) * (size_t)nshift);
+    LHL_HIP_TRY(hipMemcpyAsync(params.p, keep, sizeof(ShiftParam<T>) * (size_t)nshift, hipMemcpyHostToDevice,
+                               h->stream));
+    ProfWork pw(h, nshift, 0);
     return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
                              params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info);
 }

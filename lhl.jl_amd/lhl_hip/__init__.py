@@ -36,19 +36,28 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "LHLError", "LHLWorkspace", "LHLShift", "lhl", "lhl_",
     "lhl_reduce_", "lhl_shift_", "lhl_ldiv_", "lhl_refine_", "applyZ_", "applyZinv_",
     "shift_solve_batched", "library_path", "load_library", "profile_enable", "profile_reset", "profile_read",
-    "sync",
+    "profile_read_work", "sync", "LHLNotConverged",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
 
-LHL_OK, LHL_ERR_ARG, LHL_ERR_DIM, LHL_ERR_HIP, LHL_ERR_STATE, LHL_ERR_NOMEM = range(6)
+LHL_OK, LHL_ERR_ARG, LHL_ERR_DIM, LHL_ERR_HIP, LHL_ERR_STATE, LHL_ERR_NOMEM, LHL_NOT_CONVERGED = range(7)
 LHL_F64, LHL_F32 = 0, 1
 LHL_PREC_WORKING, LHL_PREC_REFINE64 = 0, 1
 
 
 class LHLError(RuntimeError):
     """A HIP or state error from the library (the reference's ErrorException)."""
+
+
+class LHLNotConverged(LHLError):
+    """A tolerance-driven refinement left a column above tol after the allowed
+    rounds.  The outputs are written; `result` holds what the call returns."""
+
+    def __init__(self, msg, result=None):
+        super().__init__(msg)
+        self.result = result
 
 
 class ArgumentError(ValueError):
@@ -73,6 +82,14 @@ def load_library():
         return _lib
     if not os.path.exists(_LIB_PATH):
         raise LHLError(f"{_LIB_PATH} is missing: build it with lhl.jl_amd/build_lib.py")
+    # PyTorch ships its own copy of the HIP runtime (same soname).  GPU tensors
+    # handed to this library must live in the runtime the library uses, and a
+    # second runtime in one process cannot open the GPU: load torch's first
+    # when it is installed, so the library binds to it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(_LIB_PATH)
     vp, i64, ci, cd = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
     sig = {
@@ -96,11 +113,14 @@ def load_library():
         "lhl_get_factors": ([vp, vp], ci),
         "lhl_get_ht": ([vp, vp], ci),
         "lhl_get_shift_state": ([vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)], ci),
+        "lhl_get_shift_info": ([vp, vp, ctypes.POINTER(i64), vp, vp, ctypes.POINTER(ci)], ci),
         "lhl_sync": ([vp], ci),
         "lhl_get_stream": ([vp, ctypes.POINTER(vp)], ci),
         "lhl_profile_enable": ([vp, ci], ci),
         "lhl_profile_reset": ([vp], ci),
         "lhl_profile_read": ([vp, ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(cd)], ci),
+        "lhl_profile_read_work": ([vp, ctypes.c_char_p, ctypes.POINTER(i64), ctypes.POINTER(cd),
+                                   ctypes.POINTER(i64), ctypes.POINTER(i64)], ci),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -110,11 +130,13 @@ def load_library():
     return L
 
 
-def _check(h, code):
+def _check(h, code, result=None):
     if code == LHL_OK:
         return
     msg = _lib.lhl_last_error(h)
     msg = msg.decode() if msg else ""
+    if code == LHL_NOT_CONVERGED:
+        raise LHLNotConverged(msg, result)
     if code == LHL_ERR_ARG:
         raise ArgumentError(msg)
     if code == LHL_ERR_DIM:
@@ -532,7 +554,17 @@ def lhl_refine_(x, A, b, *args, tol: float = 0.0):
     keep = None
     if A is not None:
         if _is_torch(A):
-            av = _Vec(A, "A")
+            # the library reads A with x's element type and location: convert
+            # (an FP64 A with an FP32 x is cast, not reinterpreted)
+            import torch
+            tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                   np.dtype(np.complex128): torch.complex128, np.dtype(np.complex64): torch.complex64}[v.dtype]
+            if not v.on_device:
+                raise ArgumentError("A is a GPU tensor but x is in host memory; pass A as a numpy array")
+            keep = A if A.dtype == tdt else A.to(tdt)
+            if keep.dim() != 2 or keep.stride(0) != 1:
+                keep = keep.t().contiguous().t()
+            av = _Vec(keep, "A")
         else:
             Aa = np.asfortranarray(np.asarray(A), dtype=v.dtype)
             if v.on_device:
@@ -547,10 +579,11 @@ def lhl_refine_(x, A, b, *args, tol: float = 0.0):
             raise DimensionMismatch("A must be n x n")
         Wp, ldw = ctypes.c_void_p(av.ptr), av.ld
     iters = ctypes.c_int()
-    _check(ws._h, _lib.lhl_refine(ws._h, _sh_ptr(sh), ctypes.c_void_p(v.ptr), Wp, ldw, ctypes.c_void_p(bv.ptr),
-                                  v.n, v.ncol, v.ld, v.code, v.complex, v.on_device, int(steps), float(tol),
-                                  ctypes.byref(iters)))
+    code = _lib.lhl_refine(ws._h, _sh_ptr(sh), ctypes.c_void_p(v.ptr), Wp, ldw, ctypes.c_void_p(bv.ptr),
+                           v.n, v.ncol, v.ld, v.code, v.complex, v.on_device, int(steps), float(tol),
+                           ctypes.byref(iters))
     del keep
+    _check(ws._h, code, x)
     return x
 
 
@@ -574,11 +607,13 @@ def applyZinv_(x, ws: LHLWorkspace):
 
 
 def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, precision="working",
-                        tol: float = 1e-12, max_iter: int = 10):
+                        tol: float = 1e-12, max_iter: int = 10, strict: bool = True):
     """Many shifts and right-hand sides in one call: for shift s, columns
     rhs_offsets[s]:rhs_offsets[s+1] of B (n x m, FP64 or complex128, column-major)
     are solved against sigma_s I + tau_s J.  precision="refine64" refines with the
-    FP64 J until ||r||_inf <= tol ||b||_inf.  Returns (X, info, iters)."""
+    FP64 J until ||r||_inf <= tol ||b||_inf.  Returns (X, info, iters).  When a
+    shift still has a column above tol after max_iter rounds, strict=True raises
+    LHLNotConverged (its .result holds the return value); strict=False returns."""
     sig = np.asarray(sigmas, dtype=np.complex128).reshape(-1)
     tau = np.asarray(taus, dtype=np.complex128).reshape(-1)
     if sig.shape != tau.shape:
@@ -605,6 +640,12 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
         raise ArgumentError("B and X must be Float64 (complex128 for complex shifts)")
     if bv.n != ws.n or xv.n != ws.n:
         raise DimensionMismatch("B and X must have n rows")
+    if bv.on_device != xv.on_device:
+        raise ArgumentError("B and X must both be in host memory or both on the GPU")
+    if S > 0 and (offs[0] < 0 or np.any(np.diff(offs) < 0)):
+        raise DimensionMismatch("rhs_offsets must be non-negative and non-decreasing")
+    if S > 0 and (offs[-1] > bv.ncol or offs[-1] > xv.ncol):
+        raise DimensionMismatch(f"rhs_offsets reach column {int(offs[-1])}, beyond B ({bv.ncol}) or X ({xv.ncol})")
     s2 = np.empty(2 * S)
     t2 = np.empty(2 * S)
     s2[0::2], s2[1::2] = sig.real, sig.imag
@@ -612,11 +653,24 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
     info = np.zeros(S, dtype=np.int64)
     iters = np.zeros(S, dtype=np.int32)
     prec = LHL_PREC_REFINE64 if precision == "refine64" else LHL_PREC_WORKING
-    _check(ws._h, _lib.lhl_shift_solve_batched(
+    code = _lib.lhl_shift_solve_batched(
         ws._h, S, s2.ctypes.data, t2.ctypes.data, int(cplx), offs.ctypes.data, ctypes.c_void_p(bv.ptr), bv.ld,
         ctypes.c_void_p(xv.ptr), xv.ld, bv.on_device, prec, float(tol), int(max_iter), info.ctypes.data,
-        iters.ctypes.data))
+        iters.ctypes.data)
+    if code == LHL_NOT_CONVERGED and not strict:
+        return X, info, iters
+    _check(ws._h, code, (X, info, iters))
     return X, info, iters
+
+
+def profile_read_work(ws: LHLWorkspace, kernel: str):
+    """(launches, total ms, shift units, column units) of one kernel family:
+    the summed work the launches were given."""
+    c, t = ctypes.c_int64(), ctypes.c_double()
+    su, cu = ctypes.c_int64(), ctypes.c_int64()
+    _check(ws._h, _lib.lhl_profile_read_work(ws._h, kernel.encode(), ctypes.byref(c), ctypes.byref(t),
+                                             ctypes.byref(su), ctypes.byref(cu)))
+    return int(c.value), float(t.value), int(su.value), int(cu.value)
 
 
 def sync(ws: LHLWorkspace):
