@@ -142,6 +142,22 @@ int lhl_applyZ(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int
 int lhl_applyZinv(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype,
                   int x_complex, int x_on_device);
 
+/* applyZ!(x, ws) (inverse = 0) / applyZinv!(x, ws) (inverse = 1) without
+ * device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernels and results as
+ * lhl_applyZ / lhl_applyZinv, with the planar staging copy of x in a static
+ * 16 MiB device array of the library, one per GPU, held until the process
+ * ends; the call makes no hipMalloc, hipFree or other allocating call.
+ * x must be device memory on the workspace's device (LHL_ERR_ARG otherwise),
+ * and n * nrhs * (2 if x_complex) * 8 bytes (4 when both the workspace and x
+ * are FP32) must fit the 16 MiB array (LHL_ERR_ARG otherwise).  The call
+ * returns with x written.  Every call on one GPU uses that GPU's array: calls
+ * that overlap on one GPU (several host threads) race.  Nothing in the array
+ * needs to start at zero: each call overwrites what it reads. */
+int lhl_applyZ_noalloc(lhl_handle *h, int inverse, void *x, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype,
+                       int x_complex);
+
 /* ---- getters (host copies, for parity checks and the wrapper's properties) ---- */
 
 /* n, element type, shift element type and whether a reduction is held

@@ -719,6 +719,34 @@ int lhl_applyZinv(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, 
     return applyz_common(h, 1, x, n, nrhs, ldx, xdt, x_complex, on_device);
 }
 
+int lhl_applyZ_noalloc(lhl_handle *h, int inverse, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
+                       int x_complex) {
+    if (!h) return LHL_ERR_ARG;
+    if (n != h->n)
+        return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
+                                        "x" + std::to_string(h->n));
+    if (xdt != LHL_F64 && xdt != LHL_F32) return fail(h, LHL_ERR_ARG, "applyZ (no allocation): x must be Float64 or Float32");
+    if (nrhs < 0 || (n > 0 && nrhs > 0 && (!x || ldx < n)))
+        return fail(h, LHL_ERR_DIM, "applyZ (no allocation): bad leading dimension");
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "applyZ: no reduction held");
+    if (n == 0 || nrhs == 0) return LHL_OK;
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    // x must be device memory of the handle's device (host memory would fault)
+    hipPointerAttribute_t pa{};
+    if (hipPointerGetAttributes(&pa, x) != hipSuccess || pa.type != hipMemoryTypeDevice || pa.device != h->device) {
+        (void)hipGetLastError();
+        return fail(h, LHL_ERR_ARG, "applyZ (no allocation): x must be device memory on the workspace's device");
+    }
+    st = applyz_noalloc_device(h, inverse ? 1 : 0, x, nrhs, ldx, xdt, x_complex ? 1 : 0, stage_dtype(h, xdt));
+    if (st.ok()) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, std::string("applyZ (no allocation): ") + hipGetErrorString(e)};
+    }
+    if (!st.ok()) return fail(h, st);
+    return LHL_OK;
+}
+
 int lhl_get_info(const lhl_handle *h, int64_t *n, int *dtype, int *shift_complex, int *reduced) {
     if (!h) return LHL_ERR_ARG;
     if (n) *n = h->n;

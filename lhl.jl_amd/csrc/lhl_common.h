@@ -485,6 +485,11 @@ Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rd
 Status linv_device(lhl_handle *h);                          // lhl_batch.hip
 Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane,
                      int xdt, int x_complex);              // lhl_solve.hip
+// the same on device columns with its staging in the library's static scratch
+// array (no allocation); sdt is the staging element type
+Status applyz_noalloc_device(lhl_handle *h, int inverse, void *x, int64_t nrhs, int64_t ldx, int xdt,
+                             int x_complex, int sdt);      // lhl_solve.hip
+size_t applyz_noalloc_limit();                             // bytes of that scratch array
 
 // r = b - W x for ncol columns, W dense (w != nullptr, element type xdt) or
 // sigma_c*I + tau_c*J (per-column shift scalars, J resident).  All vectors are

@@ -4,6 +4,7 @@
 #include "lhl_common.h"
 
 #include <cstring>
+#include <mutex>
 
 namespace lhl {
 namespace {
@@ -199,6 +200,63 @@ Status from_planar(lhl_handle *h, const void *src, int64_t ld_src, int64_t plane
 #undef FP
     LHL_HIP_TRY(hipGetLastError());
     return {};
+}
+
+// ---------------------------------------------------------------- allocation-free applyZ
+
+// Device memory of the allocation-free entry point: the planar staging copy of
+// x.  One array per GPU, held until the process ends.  In host code the name
+// `scratch` is a host-side stand-in, not a device address: every host use goes
+// through the per-device address from hipGetSymbolAddress below.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+Status scratch_address(int dev, void **out) {
+    if (dev < 0 || dev >= kMaxDevices) return {LHL_ERR_ARG, "applyZ (no allocation): device index out of range"};
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return {LHL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return {};
+}
+}  // namespace
+
+size_t applyz_noalloc_limit() { return kScratchBytes; }
+
+// applyZ!/applyZinv! on device columns with the planar staging copy in
+// `scratch`: the same kernels and layout as lhl_applyZ's device path, no
+// device allocation, no library call.  Profiling events are not recorded here.
+Status applyz_noalloc_device(lhl_handle *h, int inverse, void *x, int64_t nrhs, int64_t ldx, int xdt, int x_complex,
+                             int sdt) {
+    const int64_t n = h->n;
+    const int planes = x_complex ? 2 : 1;
+    const size_t need = elt_size(sdt) * (size_t)n * (size_t)nrhs * planes;
+    if (need > kScratchBytes)
+        return {LHL_ERR_ARG, "applyZ (no allocation): " + std::to_string(need) + " bytes of staging exceed the " +
+                                 std::to_string(kScratchBytes) + "-byte scratch array (n * nrhs * planes * " +
+                                 std::to_string(elt_size(sdt)) + ")"};
+    void *buf = nullptr;
+    LHL_TRY(scratch_address(h->device, &buf));
+    const bool prof = h->prof.on;
+    h->prof.on = false;
+    const int64_t ld = std::max<int64_t>(n, 1), plane = ld * nrhs;
+    Status st = to_planar(h, x, ldx, xdt, x_complex, buf, ld, plane, sdt, n, nrhs);
+    if (st.ok()) st = applyz_device(h, inverse, buf, nrhs, ld, plane, sdt, x_complex);
+    if (st.ok()) st = from_planar(h, buf, ld, plane, sdt, x, ldx, xdt, x_complex, n, nrhs);
+    h->prof.on = prof;
+    return st;
 }
 
 }  // namespace lhl
