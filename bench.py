@@ -365,7 +365,8 @@ def main():
         lhl_hip.sync(ws)
         lhl_hip.profile_enable(ws, False)
         lhl_kernels[name] = {}
-        for k in ("balance", "reduce_copy", "reduce_step", "reduce_pivot", "reduce_swap", "reduce_trailing", "reduce_finish"):
+        for k in ("balance", "reduce_copy", "reduce_step", "reduce_panel", "reduce_gemm", "reduce_top", "reduce_pivot",
+                  "reduce_swap", "reduce_trailing", "reduce_finish"):
             c, t = lhl_hip.profile_read(ws, k)
             if c:
                 lhl_kernels[name][k] = {"launches": c, "total_ms": t, "avg_us": 1e3 * t / c}

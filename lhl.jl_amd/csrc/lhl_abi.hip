@@ -172,7 +172,7 @@ int lhl_destroy(lhl_handle *h) {
     blas_release(h);
     h->cache.clear();
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
-                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->Linv, &h->Lfull, &h->Lp1, &h->ybuf, &h->rbuf, &h->gbuf,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->brec, &h->bpb, &h->bact, &h->blc, &h->bu12, &h->Linv, &h->Lfull, &h->Lp1, &h->ybuf, &h->rbuf, &h->gbuf,
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {

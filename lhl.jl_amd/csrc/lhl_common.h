@@ -364,6 +364,14 @@ struct lhl_handle {
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
+    // blocked reduction: per-row panel records (ping-pong), side slots of pushed
+    // columns, active rows (two halves), panel multipliers by logical row, U12
+    lhl::DevBuf brec, bpb, bact, blc, bu12;
+    // tuning knobs (0 = the built-in default): the n from which the blocked
+    // reduction runs, its panel width and the fewest rows per step workgroup
+    int64_t tune_blocked_min = 0;
+    int tune_nb = 0, tune_rmin = 0;
+    int tune_switch = 0;   // step (thousandths of n) where the row-resident loop takes over
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     // host sources of asynchronous uploads (the solves' work tables, the
     // shifts' parameters): each upload reads its own block, kept until the
@@ -435,6 +443,10 @@ inline void host_keep_reset(lhl_handle *h) {
 
 // ---- launchers implemented in the translation units ----
 Status reduce_device(lhl_handle *h, int balance);          // lhl_reduce.hip
+// the steps of the blocked reduction on rowA (pitch ldb); nb 8 or 16
+// from step kswitch (rounded up to a panel boundary) the caller's row-resident
+// loop takes over: *kstart receives that step (n - 2 when the panels finished)
+Status blocked_device(lhl_handle *h, int64_t ldb, int nb, int rmin, int kswitch, int *kstart);   // lhl_blocked.hip
 Status balance_device(lhl_handle *h);                      // lhl_reduce.hip
 Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t nshift, const double *sig, const double *tau,
                     int64_t *info_out);                    // lhl_shift.hip (one state)

@@ -21,6 +21,8 @@
 #include "lhl_common.h"
 
 #include <climits>
+#include <cstdint>
+#include <cstdlib>
 
 namespace lhl {
 namespace {
@@ -502,6 +504,39 @@ __global__ void __launch_bounds__(256) k_rows_init(const T *__restrict__ B, int6
     }
 }
 
+// The records for step k of the row-resident loop from the rows as they
+// stand after k steps of another schedule (the delayed-update panels hand
+// over here): candidates over logical rows >= k+1 of column k, the row at
+// logical k+1.  Column k in logical order is already in place.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rows_bridge(const T *__restrict__ B, int64_t ldb, int n, int k,
+                                                     int rows_per_wg, const int *__restrict__ ilog,
+                                                     RowCand<T> *__restrict__ cout, RowNext<T> *__restrict__ rout) {
+    T bv = (T)-1, bx = 0, bk1 = 0, bp = 0;
+    int bi = INT_MAX, br = 0;
+    const int r0 = blockIdx.x * rows_per_wg, r1 = min(n, r0 + rows_per_wg);
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const T *row = B + (int64_t)r * ldb;
+        const int il = ilog[r];
+        if (il < k + 1) continue;
+        const T x = row[k];
+        if (il == k + 1) { RowNext<T> q; q.vk1 = x; q.a11 = row[k + 1]; q.r1 = r; q.pad = 0; *rout = q; }
+        const T a = dabs(x);
+        if (a == a && cand_better(a, il, bv, bi)) { bv = a; bi = il; bx = x; bk1 = row[k + 1]; bp = row[il]; br = r; }
+    }
+    __shared__ RowCand<T> sc[256];
+    RowCand<T> c;
+    c.v = bv; c.x = bx; c.ak1 = bk1; c.ap = bp; c.i = bi; c.r = br;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bt = 0;
+        for (int t = 1; t < (int)blockDim.x; t++)
+            if (cand_better(sc[t].v, sc[t].i, sc[bt].v, sc[bt].i)) bt = t;
+        cout[blockIdx.x] = sc[bt];
+    }
+}
+
 // Step k.  Wave w owns rows blockIdx.x*R + w*RPW*PASSES + i, taken in
 // PASSES passes of RPW rows; lanes run along the rows, U column groups of 64
 // per row in flight, the next group (or the next pass's first group) loaded
@@ -878,19 +913,25 @@ Status steps_typed(lhl_handle *h) {
 
 // The row-resident step loop at one (rows per wave, unroll) shape.
 template <typename T, int RPW, int U, int PASSES>
-Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds) {
+Status rows_steps(lhl_handle *h, int64_t ldb, size_t lds, int kstart) {
     const int n = (int)h->n;
     constexpr int R = kRowWaves * RPW * PASSES;
     const int G = (n + R - 1) / R;
     T *B = h->rowA.as<T>();
+    LHL_TRY(h->cand.ensure(sizeof(RowCand<T>) * 2 * (size_t)G + sizeof(RowNext<T>) * 2 + 64));
     RowCand<T> *cand = h->cand.as<RowCand<T>>();
     RowNext<T> *nxt = reinterpret_cast<RowNext<T> *>(cand + 2 * G);
     T *g = h->gcol.as<T>();
     int *ilog = h->ilog.as<int>();
-    LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt, g, ilog);
+    const int a0 = kstart & 1;
+    if (kstart == 0)
+        LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt, g, ilog);
+    else
+        LHL_LAUNCH(h, "reduce_copy", k_rows_bridge<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, kstart, R,
+                   (const int *)ilog, cand + a0 * G, nxt + a0);
     auto fn = k_rows_step<T, RPW, U, PASSES>;
     LHL_TRY(allow_dynamic_lds(fn));
-    for (int k = 0; k + 2 < n; k++) {
+    for (int k = kstart; k + 2 < n; k++) {
         const int a = k & 1, b = a ^ 1;
         LHL_LAUNCH(h, "reduce_step", fn, dim3(G), dim3(kRowWaves * 64), lds, B, ldb, n, k,
                    (const RowCand<T> *)(cand + a * G), cand + b * G, G, (const RowNext<T> *)(nxt + a), nxt + b,
@@ -907,6 +948,33 @@ inline bool rows_fit(int64_t n, int dtype) {
     return n >= 3 && n <= 16384 && (size_t)(2 * n) * elt_size(dtype) + 8 * 1024 <= lds_limit();
 }
 
+// Whether the delayed-update (blocked) step loop takes this reduction, and
+// its panel width and fewest rows per step workgroup.  The handle's tuning
+// knobs decide (the environment variables LHL_BLOCKED_MIN, LHL_NB and
+// LHL_RMIN override them, for sweeps); the defaults follow measurement.
+inline bool use_blocked(const lhl_handle *h, int n, int *nb, int *rmin, int *kswitch) {
+    auto env = [](const char *k, int64_t d) -> int64_t {
+        const char *e = std::getenv(k);
+        return e ? std::strtoll(e, nullptr, 10) : d;
+    };
+    // defaults from measurement at n = 4096 (one MI355X): FP64 runs the panels
+    // over the first 0.4 n steps (96 ms against 103 ms for the row-resident
+    // loop alone); FP32 gains nothing from them (67.6 against 67.3 ms)
+    const int64_t bdef = h->dtype == LHL_F64 ? 1024 : INT64_MAX;
+    const int64_t bmin = env("LHL_BLOCKED_MIN", h->tune_blocked_min > 0 ? h->tune_blocked_min : bdef);
+    *nb = (int)env("LHL_NB", h->tune_nb > 0 ? h->tune_nb : 16);
+    *rmin = (int)env("LHL_RMIN", h->tune_rmin > 0 ? h->tune_rmin : 16);
+    if (*nb != 8 && *nb != 16) *nb = 16;
+    if (*rmin < 1) *rmin = 1;
+    // the step from which the row-resident loop takes over (rounded up to a
+    // panel boundary), as a fraction of n in thousandths
+    const int64_t sw = env("LHL_SWITCH", h->tune_switch > 0 ? h->tune_switch : 400);
+    *kswitch = (int)std::min<int64_t>(n, (int64_t)n * std::max<int64_t>(sw, 0) / 1000);
+    // the step kernel's LDS: l (n elements) beside up to ~45 KB of fixed tables
+    if ((size_t)n * elt_size(h->dtype) + 48 * 1024 > lds_limit()) return false;
+    return n >= bmin && n >= 3;
+}
+
 template <typename TJ, typename T>
 Status rows_typed(lhl_handle *h, int balanced) {
     const int n = (int)h->n;
@@ -921,13 +989,22 @@ Status rows_typed(lhl_handle *h, int balanced) {
     LHL_LAUNCH(h, "reduce_copy", (k_rows_load<TJ, T>), gr, tb, 0, (const TJ *)h->J.as<TJ>(), h->ldj, B, ldb, n,
                (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(), balanced);
     const size_t lds = sizeof(T) * 2 * (size_t)n;
-    // rows per wave so that about 256 workgroups cover the rows; loads in
-    // flight per lane = rows per wave x unroll
-    constexpr int W = sizeof(T) == 8 ? 1 : 2;
-    if (n <= 2048) LHL_TRY((rows_steps<T, 1, 8 * W, 1>(h, ldb, lds)));
-    else if (n <= 4096) LHL_TRY((rows_steps<T, 2, 8 * W, 1>(h, ldb, lds)));
-    else if (n <= 8192) LHL_TRY((rows_steps<T, 2, 8 * W, 2>(h, ldb, lds)));
-    else LHL_TRY((rows_steps<T, 2, 8 * W, 4>(h, ldb, lds)));
+    int nb = 0, rmin = 0, kswitch = 0;
+    int kstart = 0;
+    if (use_blocked(h, n, &nb, &rmin, &kswitch)) {
+        // delayed-update panels while the trailing block is large; the
+        // row-resident loop takes over from step kstart (n - 2: none left)
+        LHL_TRY(blocked_device(h, ldb, nb, rmin, kswitch, &kstart));
+    }
+    if (kstart + 2 < n) {
+        // rows per wave so that about 256 workgroups cover the rows; loads in
+        // flight per lane = rows per wave x unroll
+        constexpr int W = sizeof(T) == 8 ? 1 : 2;
+        if (n <= 2048) LHL_TRY((rows_steps<T, 1, 8 * W, 1>(h, ldb, lds, kstart)));
+        else if (n <= 4096) LHL_TRY((rows_steps<T, 2, 8 * W, 1>(h, ldb, lds, kstart)));
+        else if (n <= 8192) LHL_TRY((rows_steps<T, 2, 8 * W, 2>(h, ldb, lds, kstart)));
+        else LHL_TRY((rows_steps<T, 2, 8 * W, 4>(h, ldb, lds, kstart)));
+    }
     LHL_LAUNCH(h, "reduce_finish", k_rows_perm, dim3((n + 255) / 256), dim3(256), 0, (const int *)h->ilog.as<int>(),
                h->perm.as<int>(), h->iperm.as<int>(), n);
     LHL_LAUNCH(h, "reduce_finish", k_rows_out<T>, gr, tb, 0, (const T *)B, ldb, (const int *)h->perm.as<int>(),
