@@ -189,6 +189,26 @@ int lhl_get_shift_state(lhl_handle *h, lhl_shift_state *sh, void *U, void *l, ui
 int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double *sigma, double *tau,
                        int *loaded);
 
+/* ---- tuning (no reference counterpart: the knobs the reference fixes as
+ * constants, _lhl_block_min / _lhl_panel_width (src:703-706), _LHL_GEMV_GROUP
+ * (src:1451), _lhl_shift_fused_min (src:2494), become per-handle settings the
+ * autotuner searches) ----
+ * knob                 meaning                                          default
+ * reduce_blocked_min   n from which lhl() runs delayed-update panels    3073 (FP64), off (FP32)
+ * reduce_nb            panel width (8 or 16)                            16
+ * reduce_rmin          fewest rows per workgroup of a panel step        16
+ * reduce_switch        step (thousandths of n) from which the
+ *                      row-resident loop takes over from the panels     400
+ *                      (0: the row-resident loop from the first step)
+ * lsweep_depth         halvings of the L sweeps' triangular products    2
+ * shift_wide_min       n from which the shift LU's bulk is 960 wide     4481 (FP64), 8513 (FP32)
+ * shift_pool_bytes     bound on one chunk's packed U's (batched solve)  min(64 GiB, free/3)
+ * A value of -1 restores the default; lhl_tune_get reports -1 for a knob at
+ * its default.  Results do not depend on the knobs beyond rounding (the
+ * pivots of a reduction match for every panel setting). */
+int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value);
+int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value);
+
 /* Synchronize the handle's stream. */
 int lhl_sync(lhl_handle *h);
 

@@ -5,6 +5,7 @@
 #include "lhl_common.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -435,10 +436,12 @@ static Status host_pinned(lhl_handle *h, size_t bytes, char **out) {
 // whichever is less.  The batch splits into chunks of equal size: a full chunk
 // plus a remainder of a few shifts would pay for two full-length launches of
 // every kernel (the shift LUs' time does not depend on their count).
-static int64_t shift_chunk(int64_t nshift, size_t ubytes) {
+static int64_t shift_chunk(const lhl_handle *h, int64_t nshift, size_t ubytes) {
     size_t budget = 64ull << 30;
     if (const char *e = std::getenv("LHL_SHIFT_POOL_BYTES")) {
         budget = (size_t)std::strtoull(e, nullptr, 10);
+    } else if (h->tune_pool_bytes > 0) {
+        budget = (size_t)h->tune_pool_bytes;
     } else {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = std::min(budget, std::max<size_t>(fr / 3, 1ull << 30));
@@ -495,7 +498,7 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     plane = planeb;
     // per-chunk pools, reused from chunk to chunk
     const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
-    const int64_t chunk = shift_chunk(nshift, ubytes);
+    const int64_t chunk = shift_chunk(h, nshift, ubytes);
     // pinned host layout: zero-pivot flags of every shift, then one chunk's
     // per-column shifts (4 doubles) and residual norms (2 doubles)
     int64_t maxnc = 0;
@@ -897,6 +900,59 @@ int lhl_profile_read_work(lhl_handle *h, const char *kernel, int64_t *launches, 
 
 int lhl_profile_read(lhl_handle *h, const char *kernel, int64_t *launches, double *total_ms) {
     return lhl_profile_read_work(h, kernel, launches, total_ms, nullptr, nullptr);
+}
+
+// The tuning knobs (the autotuner's search space).  -1 restores a knob's
+// built-in default.
+namespace {
+struct Knob {
+    const char *name;
+    int64_t lo, hi;
+};
+const Knob kKnobs[] = {
+    {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
+    {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
+    {"shift_pool_bytes", 1, INT64_MAX},
+};
+int64_t *knob_slot(lhl_handle *h, const std::string &k) {
+    if (k == "reduce_blocked_min") return &h->tune_blocked_min;
+    if (k == "shift_wide_min") return &h->tune_shift_wide;
+    if (k == "shift_pool_bytes") return &h->tune_pool_bytes;
+    return nullptr;
+}
+}  // namespace
+
+int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value) {
+    if (!h || !knob) return LHL_ERR_ARG;
+    const std::string k(knob);
+    const Knob *kk = nullptr;
+    for (const Knob &x : kKnobs)
+        if (k == x.name) kk = &x;
+    if (!kk) return fail(h, LHL_ERR_ARG, "lhl_tune_set: unknown knob " + k);
+    if (value != -1 && (value < kk->lo || value > kk->hi))
+        return fail(h, LHL_ERR_ARG, "lhl_tune_set: " + k + " out of range [" + std::to_string(kk->lo) + ", " +
+                                        std::to_string(kk->hi) + "]");
+    if (k == "reduce_nb" && value != -1 && value != 8 && value != 16)
+        return fail(h, LHL_ERR_ARG, "lhl_tune_set: reduce_nb must be 8 or 16");
+    if (int64_t *slot = knob_slot(h, k)) { *slot = value == -1 ? 0 : value; return LHL_OK; }
+    const int v = (int)value;
+    if (k == "reduce_nb") h->tune_nb = value == -1 ? 0 : v;
+    else if (k == "reduce_rmin") h->tune_rmin = value == -1 ? 0 : v;
+    else if (k == "reduce_switch") h->tune_switch = value == -1 ? 0 : (v == 0 ? -1000000 : v);
+    else if (k == "lsweep_depth") h->tune_lsweep = value == -1 ? -1 : v;
+    return LHL_OK;
+}
+
+int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value) {
+    if (!h || !knob || !value) return LHL_ERR_ARG;
+    const std::string k(knob);
+    if (int64_t *slot = knob_slot(h, k)) { *value = *slot > 0 ? *slot : -1; return LHL_OK; }
+    if (k == "reduce_nb") *value = h->tune_nb > 0 ? h->tune_nb : -1;
+    else if (k == "reduce_rmin") *value = h->tune_rmin > 0 ? h->tune_rmin : -1;
+    else if (k == "reduce_switch") *value = h->tune_switch > 0 ? h->tune_switch : (h->tune_switch < 0 ? 0 : -1);
+    else if (k == "lsweep_depth") *value = h->tune_lsweep;
+    else return fail(h, LHL_ERR_ARG, "lhl_tune_get: unknown knob " + k);
+    return LHL_OK;
 }
 
 int lhl_sync(lhl_handle *h) {

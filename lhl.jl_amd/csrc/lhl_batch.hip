@@ -1030,7 +1030,8 @@ Status blas_gemm_lower(lhl_handle *h, const char *name, int m, int ncol, const T
     rocblas_handle b;
     LHL_TRY(blas_handle(h, &b));
     ProfScope ps(h, name);
-    if (lower_mul<T>(b, 2, m, ncol, L, ldl, B, ldb, (T)0, C, ldc) != rocblas_status_success)
+    const int depth = h->tune_lsweep >= 0 ? h->tune_lsweep : 2;
+    if (lower_mul<T>(b, depth, m, ncol, L, ldl, B, ldb, (T)0, C, ldc) != rocblas_status_success)
         return {LHL_ERR_HIP, "rocBLAS GEMM of the solve failed"};
     return {};
 }

@@ -372,6 +372,9 @@ struct lhl_handle {
     int64_t tune_blocked_min = 0;
     int tune_nb = 0, tune_rmin = 0;
     int tune_switch = 0;   // step (thousandths of n) where the row-resident loop takes over
+    int tune_lsweep = -1;  // halvings of the L sweeps' triangular GEMMs (-1: default, 2)
+    int64_t tune_shift_wide = 0;   // n from which the shift LU's bulk is 960 threads wide (0: default)
+    int64_t tune_pool_bytes = 0;   // bound on one chunk's packed U's in a batched solve (0: default)
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     // host sources of asynchronous uploads (the solves' work tables, the
     // shifts' parameters): each upload reads its own block, kept until the

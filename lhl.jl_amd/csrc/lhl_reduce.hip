@@ -957,10 +957,12 @@ inline bool use_blocked(const lhl_handle *h, int n, int *nb, int *rmin, int *ksw
         const char *e = std::getenv(k);
         return e ? std::strtoll(e, nullptr, 10) : d;
     };
-    // defaults from measurement at n = 4096 (one MI355X): FP64 runs the panels
-    // over the first 0.4 n steps (96 ms against 103 ms for the row-resident
-    // loop alone); FP32 gains nothing from them (67.6 against 67.3 ms)
-    const int64_t bdef = h->dtype == LHL_F64 ? 1024 : INT64_MAX;
+    // defaults from measurement (one MI355X): at n = 4096 FP64 runs the panels
+    // over the first 0.4 n steps (96.9 ms against 102.8 ms for the row-resident
+    // loop alone) while FP32 gains nothing from them (67.7 against 67.3 ms); at
+    // n = 1024 and 2048 the panels lose in FP64 too (13.3 / 30.3 ms against
+    // 8.8 / 22.0 ms), so they start above 3072
+    const int64_t bdef = h->dtype == LHL_F64 ? 3073 : INT64_MAX;
     const int64_t bmin = env("LHL_BLOCKED_MIN", h->tune_blocked_min > 0 ? h->tune_blocked_min : bdef);
     *nb = (int)env("LHL_NB", h->tune_nb > 0 ? h->tune_nb : 16);
     *rmin = (int)env("LHL_RMIN", h->tune_rmin > 0 ? h->tune_rmin : 16);
@@ -968,7 +970,7 @@ inline bool use_blocked(const lhl_handle *h, int n, int *nb, int *rmin, int *ksw
     if (*rmin < 1) *rmin = 1;
     // the step from which the row-resident loop takes over (rounded up to a
     // panel boundary), as a fraction of n in thousandths
-    const int64_t sw = env("LHL_SWITCH", h->tune_switch > 0 ? h->tune_switch : 400);
+    const int64_t sw = env("LHL_SWITCH", h->tune_switch > 0 ? h->tune_switch : (h->tune_switch < 0 ? 0 : 400));
     *kswitch = (int)std::min<int64_t>(n, (int64_t)n * std::max<int64_t>(sw, 0) / 1000);
     // the step kernel's LDS: l (n elements) beside up to ~45 KB of fixed tables
     if ((size_t)n * elt_size(h->dtype) + 48 * 1024 > lds_limit()) return false;

@@ -740,6 +740,23 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int BW = panel_w<T>() * kPanelBulkWide;
+    // the tuning knob: the 960-thread bulk from n = tune_shift_wide on
+    const bool wide = h->tune_shift_wide > 0 && n >= h->tune_shift_wide;
+    if (!cplx && wide && n <= (F32 ? 14 : 7) * BW) {
+        if constexpr (F32) {
+            if (n <= 10 * BW)
+                LHL_TRY((launch_panel<T, 10, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 14, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        } else {
+            if (n <= 3 * BW)
+                LHL_TRY((launch_panel<T, 3, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else if (n <= 5 * BW)
+                LHL_TRY((launch_panel<T, 5, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+            else LHL_TRY((launch_panel<T, 7, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        }
+        LHL_HIP_TRY(hipGetLastError());
+        return {};
+    }
     if (!cplx && n <= (F32 ? 14 : 7) * BW) {
         if constexpr (F32) {
             if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));

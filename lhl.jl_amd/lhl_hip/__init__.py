@@ -36,8 +36,11 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "LHLError", "LHLWorkspace", "LHLShift", "lhl", "lhl_",
     "lhl_reduce_", "lhl_shift_", "lhl_ldiv_", "lhl_refine_", "applyZ_", "applyZinv_",
     "shift_solve_batched", "library_path", "load_library", "profile_enable", "profile_reset", "profile_read",
-    "profile_read_work", "sync", "LHLNotConverged",
+    "profile_read_work", "sync", "LHLNotConverged", "tune_set", "tune_get", "TUNING_KNOBS",
 ]
+
+TUNING_KNOBS = ("reduce_blocked_min", "reduce_nb", "reduce_rmin", "reduce_switch", "lsweep_depth",
+                "shift_wide_min", "shift_pool_bytes")
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
@@ -114,6 +117,9 @@ def load_library():
         "lhl_get_ht": ([vp, vp], ci),
         "lhl_get_shift_state": ([vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(i64)], ci),
         "lhl_get_shift_info": ([vp, vp, ctypes.POINTER(i64), vp, vp, ctypes.POINTER(ci)], ci),
+        "lhl_tune_set": ([vp, ctypes.c_char_p, i64], ci),
+        "lhl_tune_get": ([vp, ctypes.c_char_p, ctypes.POINTER(i64)], ci),
+        "lhl_applyZ_noalloc": ([vp, ci, vp, i64, i64, i64, ci, ci], ci),
         "lhl_sync": ([vp], ci),
         "lhl_get_stream": ([vp, ctypes.POINTER(vp)], ci),
         "lhl_profile_enable": ([vp, ci], ci),
@@ -441,8 +447,13 @@ def _as_dense_J(J):
 
 
 def lhl_reduce_(ws: LHLWorkspace, J, balance: bool, thread=True) -> LHLWorkspace:
-    """lhl_reduce!(ws, J, balance) (src:355): reduce J into ws; J is not modified."""
+    """lhl_reduce!(ws, J, balance) (src:355): reduce J into ws; J is not modified.
+    `thread` is accepted and ignored (the result does not depend on it, as the
+    reference's does not depend on the thread count, src:712-716).  The
+    autotuner's settings for this n and element type are applied first."""
     v = _as_dense_J(J)
+    from . import autotune
+    autotune.apply(ws, v.n)
     _check(ws._h, _lib.lhl_reduce(ws._h, ctypes.c_void_p(v.ptr), v.n, v.ld, v.code, int(bool(balance)), v.on_device))
     ws._n = v.n
     ws.shift._n = v.n
@@ -671,6 +682,17 @@ def profile_read_work(ws: LHLWorkspace, kernel: str):
     _check(ws._h, _lib.lhl_profile_read_work(ws._h, kernel.encode(), ctypes.byref(c), ctypes.byref(t),
                                              ctypes.byref(su), ctypes.byref(cu)))
     return int(c.value), float(t.value), int(su.value), int(cu.value)
+
+
+def tune_set(ws: LHLWorkspace, knob: str, value: int):
+    """Set one tuning knob of the workspace (-1 restores its default)."""
+    _check(ws._h, _lib.lhl_tune_set(ws._h, knob.encode(), int(value)))
+
+
+def tune_get(ws: LHLWorkspace, knob: str) -> int:
+    v = ctypes.c_int64()
+    _check(ws._h, _lib.lhl_tune_get(ws._h, knob.encode(), ctypes.byref(v)))
+    return int(v.value)
 
 
 def sync(ws: LHLWorkspace):

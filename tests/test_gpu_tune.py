@@ -1,0 +1,93 @@
+"""Tuning knobs change speed, not results: the shift LU stays bit-identical to
+the oracle with the wide bulk forced at every size, the reduction's pivots stay
+the oracle's under every panel setting set through the knob API, and the
+solves stay within the reference's tolerances for every L-sweep depth."""
+import numpy as np
+import pytest
+
+from inputs import randn, stiff, tie_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_knob_api_roundtrip(lhl_hip):
+    ws = lhl_hip.LHLWorkspace(10, np.float64)
+    for k in lhl_hip.TUNING_KNOBS:
+        assert lhl_hip.tune_get(ws, k) == -1
+    lhl_hip.tune_set(ws, "reduce_nb", 8)
+    assert lhl_hip.tune_get(ws, "reduce_nb") == 8
+    lhl_hip.tune_set(ws, "reduce_switch", 0)
+    assert lhl_hip.tune_get(ws, "reduce_switch") == 0
+    lhl_hip.tune_set(ws, "reduce_switch", -1)
+    assert lhl_hip.tune_get(ws, "reduce_switch") == -1
+    with pytest.raises(lhl_hip.ArgumentError):
+        lhl_hip.tune_set(ws, "reduce_nb", 12)
+    with pytest.raises(lhl_hip.ArgumentError):
+        lhl_hip.tune_set(ws, "no_such_knob", 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_wide_shift_bulk_bitwise(lhl_hip, oracle, monkeypatch, dtype):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    for n in (64, 300, 1000, 2500):
+        say(f"case: wide shift bulk {np.dtype(dtype).name} n={n}")
+        J = randn(n + 1, n, dtype=dtype)
+        ws = lhl_hip.LHLWorkspace(n, dtype)
+        lhl_hip.tune_set(ws, "shift_wide_min", 1)
+        lhl_hip.lhl_(ws, J)
+        Ht = ws.Ht
+        for s, t in ((1, -0.1), (0.5, -3)):
+            lhl_hip.lhl_shift_(ws, s, t)
+            ref = oracle.shift(Ht, s, t)
+            assert ws.info == ref.info
+            assert np.array_equal(ws.shift.swap, ref.swap.astype(bool))
+            assert np.array_equal(ws.shift.U, ref.U())
+            assert np.array_equal(ws.shift.multipliers, ref.mult())
+            assert np.array_equal(ws.shift.rdiag, ref.rdiag)
+
+
+def test_reduce_knobs_keep_pivots(lhl_hip, oracle, monkeypatch):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    n = 600
+    for J in (randn(61, n), tie_matrix(62, 200, -1, 1)):
+        m = J.shape[0]
+        red = oracle.reduce(J, blocked=0)
+        for knobs in ({"reduce_blocked_min": 3, "reduce_switch": 0}, {"reduce_blocked_min": 3, "reduce_nb": 8,
+                                                                     "reduce_rmin": 4, "reduce_switch": 500},
+                      {"reduce_blocked_min": 3, "reduce_rmin": 64}, {"reduce_blocked_min": 1 << 30}):
+            say(f"case: reduce knobs n={m} {knobs}")
+            ws = lhl_hip.LHLWorkspace(m, np.float64)
+            for k, v in knobs.items():
+                lhl_hip.tune_set(ws, k, v)
+            lhl_hip.lhl_(ws, J)
+            assert np.array_equal(ws.ipiv - 1, red.ipiv), knobs
+            rel = np.linalg.norm(ws.factors - red.A) / np.linalg.norm(red.A)
+            assert rel <= 1e-12 * m, (knobs, rel)
+
+
+def test_lsweep_depth_solves(lhl_hip, monkeypatch):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    n, S, R = 1100, 3, 2
+    J = stiff(1100, n)
+    gam = np.logspace(-4, -1, S)
+    offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+    B = np.asfortranarray(np.random.default_rng(5).standard_normal((n, S * R)))
+    ws = lhl_hip.lhl(J)
+    X0 = None
+    for depth in (0, 1, 2, 3):
+        say(f"case: lsweep depth {depth}")
+        lhl_hip.tune_set(ws, "lsweep_depth", depth)
+        X, info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs)
+        assert np.all(info == 0)
+        for s in range(S):
+            W = np.eye(n) - gam[s] * J
+            c = slice(offs[s], offs[s + 1])
+            r = np.abs(B[:, c] - W @ X[:, c]).max() / np.abs(B[:, c]).max()
+            assert r <= 1e-10, (depth, s, r)
+        if X0 is None:
+            X0 = X
+        assert np.linalg.norm(X - X0) <= 1e-11 * np.linalg.norm(X0)
