@@ -203,6 +203,8 @@ int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double
  * lsweep_depth         halvings of the L sweeps' triangular products    2
  * shift_wide_min       n from which the shift LU's bulk is 960 wide     4481 (FP64), 8513 (FP32)
  * shift_pool_bytes     bound on one chunk's packed U's (batched solve)  min(64 GiB, free/3)
+ * h2d_chunk_bytes      bytes per pinned staging chunk of a host J's     16 MiB
+ *                      upload in lhl_reduce (0: the runtime's copy)
  * A value of -1 restores the default; lhl_tune_get reports -1 for a knob at
  * its default.  Results do not depend on the knobs beyond rounding (the
  * pivots of a reduction match for every panel setting). */

@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
+#include <vector>
 
 using namespace lhl;
 
@@ -185,12 +187,66 @@ int lhl_destroy(lhl_handle *h) {
         (void)hipStreamSynchronize(h->stream2);
         (void)hipStreamDestroy(h->stream2);
     }
+    for (hipEvent_t ev : h->ev_h2d)
+        if (ev) (void)hipEventDestroy(ev);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->hpin) (void)hipHostFree(h->hpin);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LHL_OK;
+}
+
+static Status host_pinned(lhl_handle *h, size_t bytes, char **out);
+
+// Upload of a host (pageable) J, columns of `wb` bytes at pitch `spitch`, into
+// the device matrix at pitch `dpitch`: in chunks of whole columns through two
+// pinned staging halves.  The host threads copy chunk i + 1 into one half
+// while the DMA engine moves chunk i out of the other; an event per half says
+// when its last upload has left it.  `chunk` bytes per half; the pageable
+// runtime copy (chunk = 0) stages through its own buffers one transfer at a
+// time.
+static Status upload_pipelined(lhl_handle *h, void *dst, size_t dpitch, const void *src, size_t spitch, size_t wb,
+                               int64_t ncol, size_t chunk) {
+    const int64_t cc = std::max<int64_t>(1, (int64_t)(chunk / std::max<size_t>(wb, 1)));
+    if (cc >= ncol || chunk == 0) {
+        LHL_HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, wb, ncol, hipMemcpyHostToDevice, h->stream));
+        return {};
+    }
+    for (hipEvent_t &ev : h->ev_h2d)
+        if (!ev) LHL_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    const size_t half = (size_t)cc * wb;
+    char *pin = nullptr;
+    LHL_TRY(host_pinned(h, 2 * half, &pin));
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<unsigned>(std::min<unsigned>(hw, 8u), (unsigned)std::max<size_t>(1, half >> 21));
+    auto copy_cols = [&](char *out, const char *in, int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1; c++) std::memcpy(out + (size_t)c * wb, in + (size_t)c * spitch, wb);
+    };
+    for (int64_t c0 = 0, i = 0; c0 < ncol; c0 += cc, i++) {
+        const int64_t m = std::min(cc, ncol - c0);
+        const int b = (int)(i & 1);
+        char *buf = pin + (size_t)b * half;
+        if (i >= 2) LHL_HIP_TRY(hipEventSynchronize(h->ev_h2d[b]));   // its previous chunk has left
+        const char *in = static_cast<const char *>(src) + (size_t)c0 * spitch;
+        if (nt <= 1 || m < nt) {
+            copy_cols(buf, in, 0, m);
+        } else {
+            std::vector<std::thread> ts;
+            ts.reserve((size_t)nt - 1);
+            const int64_t per = (m + nt - 1) / nt;
+            for (int t = 1; t < nt; t++) {
+                const int64_t a = std::min(m, t * per), e = std::min(m, (t + 1) * per);
+                if (a < e) ts.emplace_back(copy_cols, buf, in, a, e);
+            }
+            copy_cols(buf, in, 0, std::min(m, per));
+            for (auto &t : ts) t.join();
+        }
+        LHL_HIP_TRY(hipMemcpy2DAsync(static_cast<char *>(dst) + (size_t)c0 * dpitch, dpitch, buf, wb, wb, m,
+                                     hipMemcpyHostToDevice, h->stream));
+        LHL_HIP_TRY(hipEventRecord(h->ev_h2d[b], h->stream));
+    }
+    return {};
 }
 
 int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int balance, int j_on_device) {
@@ -217,9 +273,21 @@ int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype
     if (n > 0) {
         if (j_dtype == rdt) {
             const size_t es = elt_size(rdt);
-            hipError_t e = hipMemcpy2DAsync(h->J.p, es * h->ldj, J, es * ldj, es * n, n,
-                                            j_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream);
-            if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: copy of J: ") + hipGetErrorString(e));
+            if (j_on_device) {
+                hipError_t e = hipMemcpy2DAsync(h->J.p, es * h->ldj, J, es * ldj, es * n, n, hipMemcpyDeviceToDevice,
+                                                h->stream);
+                if (e != hipSuccess)
+                    return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: copy of J: ") + hipGetErrorString(e));
+            } else {
+                // a host J: staged in 16 MiB chunks by default (the tuning knob
+                // h2d_chunk_bytes; 0 leaves the copy to the runtime)
+                const size_t chunk = h->tune_h2d_chunk >= 0 ? (size_t)h->tune_h2d_chunk : (16u << 20);
+                st = upload_pipelined(h, h->J.p, es * h->ldj, J, es * ldj, es * n, n, chunk);
+                if (!st.ok()) {
+                    (void)hipStreamSynchronize(h->stream);
+                    return fail(h, st);
+                }
+            }
         } else {
             int64_t ld, plane;
             st = stage_in(h, h->xstage, J, n, n, ldj, j_dtype, 0, j_on_device, rdt, ld, plane);
@@ -232,7 +300,10 @@ int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype
         }
     }
     st = reduce_device(h, balance);
-    if (!st.ok()) return fail(h, st);
+    if (!st.ok()) {
+        (void)hipStreamSynchronize(h->stream);   // the upload may still read the pinned staging
+        return fail(h, st);
+    }
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: ") + hipGetErrorString(e));
     h->reduced = 1;
@@ -912,7 +983,7 @@ struct Knob {
 const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
-    {"shift_pool_bytes", 1, INT64_MAX},
+    {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;
@@ -940,6 +1011,7 @@ int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value) {
     else if (k == "reduce_rmin") h->tune_rmin = value == -1 ? 0 : v;
     else if (k == "reduce_switch") h->tune_switch = value == -1 ? 0 : (v == 0 ? -1000000 : v);
     else if (k == "lsweep_depth") h->tune_lsweep = value == -1 ? -1 : v;
+    else if (k == "h2d_chunk_bytes") h->tune_h2d_chunk = value;
     return LHL_OK;
 }
 
@@ -951,6 +1023,7 @@ int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value) {
     else if (k == "reduce_rmin") *value = h->tune_rmin > 0 ? h->tune_rmin : -1;
     else if (k == "reduce_switch") *value = h->tune_switch > 0 ? h->tune_switch : (h->tune_switch < 0 ? 0 : -1);
     else if (k == "lsweep_depth") *value = h->tune_lsweep;
+    else if (k == "h2d_chunk_bytes") *value = h->tune_h2d_chunk;
     else return fail(h, LHL_ERR_ARG, "lhl_tune_get: unknown knob " + k);
     return LHL_OK;
 }

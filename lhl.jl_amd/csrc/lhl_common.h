@@ -352,6 +352,7 @@ struct lhl_handle {
     // shift-independent part runs there while the shift LUs run on `stream`
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_h2d[2] = {nullptr, nullptr};   // the host-J upload's two staging halves
     void *blas_stream = nullptr;    // the stream the rocBLAS handle is set to
     // pinned host memory for the batched solve's small transfers (zero-pivot
     // flags, per-column shifts, residual norms), grown on demand
@@ -375,6 +376,7 @@ struct lhl_handle {
     int tune_lsweep = -1;  // halvings of the L sweeps' triangular GEMMs (-1: default, 2)
     int64_t tune_shift_wide = 0;   // n from which the shift LU's bulk is 960 threads wide (0: default)
     int64_t tune_pool_bytes = 0;   // bound on one chunk's packed U's in a batched solve (0: default)
+    int64_t tune_h2d_chunk = -1;   // bytes per staged chunk of a host J's upload (-1: default, 0: runtime copy)
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     // host sources of asynchronous uploads (the solves' work tables, the
     // shifts' parameters): each upload reads its own block, kept until the

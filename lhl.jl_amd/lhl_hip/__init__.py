@@ -40,7 +40,7 @@ __all__ = [
 ]
 
 TUNING_KNOBS = ("reduce_blocked_min", "reduce_nb", "reduce_rmin", "reduce_switch", "lsweep_depth",
-                "shift_wide_min", "shift_pool_bytes")
+                "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes")
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
@@ -202,9 +202,16 @@ class _Vec:
                     raise ArgumentError(f"{name}: array must be contiguous")
                 self.n, self.ncol, self.ld = x.shape[0], 1, max(x.shape[0], 1)
             elif x.ndim == 2:
-                if not x.flags.f_contiguous:
+                isz = x.itemsize
+                if x.flags.f_contiguous:
+                    self.n, self.ncol, self.ld = x.shape[0], x.shape[1], max(x.shape[0], 1)
+                elif (x.strides[0] == isz and x.strides[1] > 0 and x.strides[1] % isz == 0
+                      and x.strides[1] >= x.shape[0] * isz):
+                    # a column-major view with padded columns (e.g. big[:n, :m] of
+                    # an order='F' array): its pointer and leading dimension
+                    self.n, self.ncol, self.ld = x.shape[0], x.shape[1], x.strides[1] // isz
+                else:
                     raise ArgumentError(f"{name}: a matrix of right-hand sides must be column-major (order='F')")
-                self.n, self.ncol, self.ld = x.shape[0], x.shape[1], max(x.shape[0], 1)
             else:
                 raise ArgumentError(f"{name}: expected a vector or a matrix")
             self.ptr = x.ctypes.data
@@ -347,7 +354,17 @@ class LHLWorkspace:
         sh._loaded = False
         sh._n = int(n)
         self._n = int(n)
-        self.reduced = False
+        self._user_knobs = set()      # knobs the caller set (the autotuner leaves them)
+        self._tuned_knobs = set()     # knobs the autotuner set last
+
+    @property
+    def reduced(self) -> bool:
+        """Whether the workspace holds a valid reduction (src:158-160): false
+        from the start of a reduction until it completes, so a failed lhl()
+        leaves it false."""
+        r = ctypes.c_int()
+        _check(self._h, _lib.lhl_get_info(self._h, None, None, None, ctypes.byref(r)))
+        return bool(r.value)
 
     def __del__(self):
         try:
@@ -443,7 +460,10 @@ def _as_dense_J(J):
         raise ArgumentError("a complex J is not supported by the GPU reduction (real J only)")
     if J.dtype not in (np.float64, np.float32):
         J = J.astype(np.float64)
-    return _Vec(np.asfortranarray(J), "J")
+    if not (J.flags.f_contiguous or (J.strides[0] == J.itemsize and J.strides[1] >= J.shape[0] * J.itemsize
+                                     and J.strides[1] % J.itemsize == 0)):
+        J = np.asfortranarray(J)
+    return _Vec(J, "J")
 
 
 def lhl_reduce_(ws: LHLWorkspace, J, balance: bool, thread=True) -> LHLWorkspace:
@@ -458,7 +478,6 @@ def lhl_reduce_(ws: LHLWorkspace, J, balance: bool, thread=True) -> LHLWorkspace
     ws._n = v.n
     ws.shift._n = v.n
     ws.shift._loaded = False
-    ws.reduced = True
     return ws
 
 
@@ -685,8 +704,14 @@ def profile_read_work(ws: LHLWorkspace, kernel: str):
 
 
 def tune_set(ws: LHLWorkspace, knob: str, value: int):
-    """Set one tuning knob of the workspace (-1 restores its default)."""
+    """Set one tuning knob of the workspace (-1 restores its default).  A knob
+    set here is the caller's: the autotuner's cached choices no longer replace
+    it (set -1 to hand it back)."""
     _check(ws._h, _lib.lhl_tune_set(ws._h, knob.encode(), int(value)))
+    if int(value) == -1:
+        ws._user_knobs.discard(knob)
+    else:
+        ws._user_knobs.add(knob)
 
 
 def tune_get(ws: LHLWorkspace, knob: str) -> int:

@@ -91,16 +91,21 @@ def lookup(dtype, n) -> dict:
 
 
 def apply(ws, n) -> dict:
-    """Set the cached knobs of (ws.dtype, n) on the workspace (others default).
-    With LHL_TUNE=0 the workspace's knobs are left as they are."""
-    from . import tune_set, TUNING_KNOBS
+    """Set the cached knobs of (ws.dtype, n) on the workspace, and return the
+    knobs this tuner set for an earlier n to their defaults.  Knobs the caller
+    set with tune_set are left alone, and LHL_TUNE=0 leaves every knob."""
+    from . import _check, _lib
     if os.environ.get("LHL_TUNE", "1") == "0":
         return {}
-    knobs = lookup(ws.dtype, n)
-    for k in TUNING_KNOBS:
-        if k in ("shift_pool_bytes",):
-            continue
-        tune_set(ws, k, int(knobs.get(k, -1)))
+    knobs = {k: v for k, v in lookup(ws.dtype, n).items() if k not in ws._user_knobs}
+
+    def put(k, v):
+        _check(ws._h, _lib.lhl_tune_set(ws._h, k.encode(), int(v)))
+    for k in ws._tuned_knobs - set(knobs) - ws._user_knobs:
+        put(k, -1)
+    for k, v in knobs.items():
+        put(k, v)
+    ws._tuned_knobs = set(knobs)
     return knobs
 
 
@@ -156,7 +161,7 @@ def tune(n, dtype=np.float64, shifts=64, rhs=16, reps=3, gain=0.01, rounds=2, lo
     dev = torch.device("cuda:0")
     J = torch.from_numpy(_stiff(4096, n)).to(dev)
     ws = LHLWorkspace(n, dtype)
-    os.environ["LHL_TUNE"] = "0"   # time the knobs as set here, not the cached ones
+    os.environ["LHL_TUNE"] = "0"   # time the knobs as set here (tune_set marks them the caller's anyway)
     try:
         def red():
             lhl_(ws, J)

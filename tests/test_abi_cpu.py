@@ -25,6 +25,25 @@ def test_library_exports_every_declared_symbol(lhl_hip):
         assert hasattr(lib, s), s
 
 
+def test_julia_bindings_call_declared_symbols(lhl_hip):
+    """Every symbol the Julia module and its LinearSolve adapter ccall is
+    declared in the header and exported by the built library."""
+    lib = ctypes.CDLL(lhl_hip.library_path())
+    decl = set(declared_symbols())
+    jdir = os.path.join(ROOT, "lhl.jl_amd", "julia")
+    called = set()
+    for f in os.listdir(jdir):
+        if f.endswith(".jl"):
+            txt = open(os.path.join(jdir, f)).read()
+            called |= set(re.findall(r"ccall\(\(:(lhl_[A-Za-z_]+)", txt))
+            called |= set(re.findall(r"\(:\w+!?, :(lhl_[A-Za-z_]+)\)", txt))
+    assert {"lhl_create", "lhl_reduce", "lhl_shift", "lhl_ldiv", "lhl_refine", "lhl_applyZ",
+            "lhl_shift_solve_batched", "lhl_get_shift_info"} <= called
+    for s in sorted(called):
+        assert s in decl, s
+        assert hasattr(lib, s), s
+
+
 def test_version_without_gpu(lhl_hip):
     assert lhl_hip.load_library().lhl_version() == 1
 
