@@ -299,6 +299,8 @@ def main():
     ap.add_argument("--pmc-workload", type=int, default=0,
                     help="K > 0: only lhl() once and K + 1 plain batched solves per precision, then exit (the "
                          "short program the rocprofv3 counter passes run)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a tuning knob on the main workload's workspaces (repeatable)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -331,6 +333,10 @@ def main():
     # ---- lhl(): FP64 and FP32 factors from the resident FP64 J
     ws64 = lhl_hip.LHLWorkspace(n, np.float64)
     ws32 = lhl_hip.LHLWorkspace(n, np.float32)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        for w_ in (ws64, ws32):
+            lhl_hip.tune_set(w_, k, int(v))
     if args.pmc_workload:
         # every kernel of a full-width solve pass, a few times, and nothing else
         # (k_rows_load in it reads the FP64 J once and writes the row-major

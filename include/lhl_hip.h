@@ -135,6 +135,34 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
  * column above tol after max_iter rounds.  rhs_offsets must start at 0 or
  * above and never decrease (LHL_ERR_DIM otherwise). */
 
+/* Plans: one batched step (lhl_shift_solve_batched's work for fixed shapes)
+ * captured in a HIP graph and replayed for new shifts.  No counterpart in the
+ * reference: it is the loop of README.md:47-51 as one graph launch, for
+ * callers that sweep many shift sets over one reduction (resolvent sweeps,
+ * implicit ODE steps).  lhl_plan_create fixes the handle, the shift count and
+ * kind, the right-hand-side offsets, B and X (FP64 device memory, interleaved
+ * complex when shifts_complex, the layout lhl_shift_solve_batched takes), the
+ * precision, tol and max_iter; every shift's factors must fit one chunk of the
+ * batched call (LHL_ERR_ARG otherwise).  lhl_plan_run(sigma, tau) computes
+ * X for those shifts (sigma/tau as in lhl_shift_solve_batched), with info[s]
+ * and iters[s] (may be NULL) and LHL_NOT_CONVERGED as there.  The refinement
+ * is decided on the device: converged shifts drop out of later rounds without
+ * a host round trip.  The first run learns the round count (one round at a
+ * time); the next captures the whole step in a graph; later runs replay it,
+ * continuing round by round if a batch needs more (the graph is then
+ * captured again).  B is read and X written at each run; the handle must stay
+ * reduced at the same n, and a plan must be destroyed before its handle.
+ * lhl_plan_stats reports the rounds in the current graph (-1: none), the
+ * replays, the captures, and the rounds run beyond a graph. */
+typedef struct lhl_plan lhl_plan;
+int lhl_plan_create(lhl_handle *h, lhl_plan **plan, int64_t nshift, int shifts_complex,
+                    const int64_t *rhs_offsets, const double *B, int64_t ldb, double *X, int64_t ldx,
+                    int precision, double tol, int max_iter);
+int lhl_plan_run(lhl_plan *plan, const double *sigma, const double *tau, int64_t *info, int *iters);
+int lhl_plan_stats(const lhl_plan *plan, int *graph_rounds, int64_t *replays, int64_t *captures,
+                   int64_t *extra_rounds);
+int lhl_plan_destroy(lhl_plan *plan);
+
 /* applyZ!(x, ws) / applyZinv!(x, ws) (src:2069-2126) on nrhs columns of x
  * (element type x_dtype; complex interleaved if x_complex). */
 int lhl_applyZ(lhl_handle *h, void *x, int64_t n, int64_t nrhs, int64_t ldx, int x_dtype,
@@ -205,6 +233,9 @@ int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double
  * shift_pool_bytes     bound on one chunk's packed U's (batched solve)  min(64 GiB, free/3)
  * h2d_chunk_bytes      bytes per pinned staging chunk of a host J's     16 MiB
  *                      upload in lhl_reduce (0: the runtime's copy)
+ * gemm_impl            the solves' dense products (L sweeps, J x):      0
+ *                      0 rocBLAS, 1 the library's MFMA kernels (a
+ *                      plan's masked refinement rounds always use them)
  * A value of -1 restores the default; lhl_tune_get reports -1 for a knob at
  * its default.  Results do not depend on the knobs beyond rounding (the
  * pivots of a reduction match for every panel setting). */

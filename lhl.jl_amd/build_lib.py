@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liblhl_hip.so")
-SOURCES = ["lhl_reduce.hip", "lhl_blocked.hip", "lhl_shift.hip", "lhl_solve.hip", "lhl_batch.hip", "lhl_abi.hip"]
+SOURCES = ["lhl_reduce.hip", "lhl_blocked.hip", "lhl_shift.hip", "lhl_solve.hip", "lhl_batch.hip", "lhl_gemm.hip",
+           "lhl_abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 
@@ -43,7 +44,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = True) -> str:
     os.makedirs(os.path.join(LIBDIR, "obj"), exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     # rocBLAS: the refinement residual's plain GEMM (J x) in one precision

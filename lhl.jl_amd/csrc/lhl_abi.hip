@@ -762,6 +762,422 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     return LHL_OK;
 }
 
+// ---- plans: one batched step captured in a HIP graph ----------------------
+//
+// A plan fixes the shapes and device arrays of a batched shift + solve +
+// refinement step (shift count, right-hand-side offsets, B and X) and
+// replays it for new shifts.  The refinement is steered on the device: after
+// each residual, k_conv retires the shifts whose columns all reached tol, and
+// every later kernel skips retired shifts' columns (column-masked MFMA
+// products, the U solves and chains per shift), so a round costs what its
+// unconverged columns need and no round waits for the host.  The first run
+// goes round by round (reading the convergence flags after each) and learns
+// how many rounds the batch needs; the next run captures the whole step with
+// that many rounds (uploads of the shifts, staging, shift LUs, the first
+// pass, the rounds, the unstaging, the read-back of the flags) into one graph,
+// and later runs replay it.  A replay whose batch needs more rounds continues
+// round by round and the graph is captured again with the larger count.
+// Every buffer the graph touches is the plan's or the handle's reduction
+// (whose addresses a reduction of the same n keeps); a reduction that moved
+// them makes the next run capture again.
+struct lhl_plan {
+    lhl_handle *h = nullptr;
+    int64_t n = 0, nshift = 0, nc = 0;
+    int cplx = 0, planes = 1, precision = 0, max_iter = 0;
+    double tol = 0;
+    std::vector<int64_t> cb;              // column offsets from the first column (nshift + 1)
+    const double *B = nullptr;
+    double *X = nullptr;
+    int64_t ldb = 0, ldx = 0, ld = 0, plane = 0;
+    DevBuf bst, xst, rst, norms, colshift, shiftpar, cbd, cact, sact, iters, groups, ybuf, rbuf, jx, jxi;
+    int64_t ngroups = 0;
+    ShiftPool pool;
+    void *pin = nullptr;                  // pinned: shifts | column shifts | info | sact | iters
+    double *p_shift = nullptr, *p_col = nullptr;
+    int *p_info = nullptr, *p_sact = nullptr, *p_iters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rounds = -1;                      // refinement rounds of the captured step (-1: learn first)
+    int graph_rounds = -1;                // rounds in the graph that `exec` holds
+    std::vector<const void *> sig;        // the handle buffers the graph reads, at capture
+    int no_graph = 0;                     // capture failed: run round by round
+    int64_t replays = 0, captures = 0, extra_rounds = 0;
+};
+
+namespace {
+
+std::vector<const void *> plan_signature(const lhl_handle *h) {
+    return {h->A.p, h->Ht.p, h->J.p, h->Lfull.p, h->perm.p, h->scale64.p, h->iscale64.p, (const void *)h->stream};
+}
+
+void plan_free_graph(lhl_plan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+    p->graph_rounds = -1;
+}
+
+SolveArgs plan_solve_args(lhl_plan *p) {
+    lhl_handle *h = p->h;
+    const int64_t n = p->n, lplane = ((n + 63) / 64) * 64;
+    SolveArgs a;
+    a.shift_complex = p->cplx;
+    a.nshift = p->nshift;
+    a.col_begin = p->cb.data();
+    a.ncol = p->nc;
+    a.x_complex = p->cplx;
+    a.xdt = LHL_F64;
+    a.ldx = p->ld;
+    a.xplane = p->plane;
+    a.U = p->pool.U.p;
+    a.l = p->pool.l.p;
+    a.rdiag = p->pool.rdiag.p;
+    a.Uinv = p->pool.Uinv.p;
+    a.uistride = (((n + 63) / 64) * 64) * 64 * p->planes;
+    a.UT = p->pool.UT.p;
+    a.swp = p->pool.swp.as<uint8_t>();
+    a.ustride = p->pool.ustride;
+    a.lstride = lplane * p->planes;
+    a.sstride = lplane;
+    a.cact = p->cact.as<int>();
+    a.sact = p->sact.as<int>();
+    a.groups = p->groups.p;
+    a.ngroups = p->ngroups;
+    a.ybuf = &p->ybuf;
+    a.rbuf = &p->rbuf;
+    (void)h;
+    return a;
+}
+
+ResidArgs plan_resid_args(lhl_plan *p) {
+    ResidArgs ra;
+    ra.colshift = p->colshift.as<double>();
+    ra.b = p->bst.p;
+    ra.x = p->xst.p;
+    ra.r = p->rst.p;
+    ra.ld = p->ld;
+    ra.plane = p->plane;
+    ra.ncol = p->nc;
+    ra.xdt = LHL_F64;
+    ra.x_complex = p->cplx;
+    ra.norms = p->norms.as<double>();
+    ra.cact = p->cact.as<int>();
+    ra.jxbuf = &p->jx;
+    ra.jxibuf = &p->jxi;
+    return ra;
+}
+
+// uploads, staging, shift LUs, the first pass, and (refining) the first
+// residual and convergence step
+Status plan_head(lhl_plan *p) {
+    lhl_handle *h = p->h;
+    LHL_HIP_TRY(hipMemcpyAsync(p->shiftpar.p, p->p_shift, 32 * (size_t)p->nshift, hipMemcpyHostToDevice, h->stream));
+    if (p->precision == LHL_PREC_REFINE64)
+        LHL_HIP_TRY(hipMemcpyAsync(p->colshift.p, p->p_col, 32 * (size_t)std::max<int64_t>(p->nc, 1),
+                                   hipMemcpyHostToDevice, h->stream));
+    int64_t ld = 0, plane = 0;
+    LHL_TRY(stage_in(h, p->bst, p->B, p->n, p->nc, p->ldb, LHL_F64, p->cplx, 1, LHL_F64, ld, plane));
+    p->ld = ld;
+    p->plane = plane;
+    LHL_TRY(fill_i32_device(h, p->cact.as<int>(), p->nc, 1));
+    LHL_TRY(fill_i32_device(h, p->sact.as<int>(), p->nshift, 1));
+    LHL_TRY(fill_i32_device(h, p->iters.as<int>(), p->nshift, 0));
+    h->dev_shiftpar = p->shiftpar.as<double>();
+    Status st = shift_pool_device(h, p->pool, p->nshift, nullptr, nullptr, p->cplx);
+    h->dev_shiftpar = nullptr;
+    LHL_TRY(st);
+    SolveArgs a = plan_solve_args(p);
+    a.xin = p->bst.p;
+    a.xout = p->xst.p;
+    a.accumulate = 0;
+    LHL_TRY(solve_device(h, a));
+    if (p->precision == LHL_PREC_REFINE64) {
+        LHL_TRY(residual_device(h, plan_resid_args(p)));
+        LHL_TRY(conv_device(h, 0, p->max_iter, p->tol, p->nshift, p->cbd.as<int64_t>(), p->norms.as<double>(), p->nc,
+                            p->sact.as<int>(), p->cact.as<int>(), p->iters.as<int>()));
+    }
+    return {};
+}
+
+// refinement round `it` (1-based): correct the columns still active with the
+// last residual, then the new residual and convergence step
+Status plan_round(lhl_plan *p, int it) {
+    lhl_handle *h = p->h;
+    SolveArgs a = plan_solve_args(p);
+    a.xin = p->rst.p;
+    a.xout = p->xst.p;
+    a.accumulate = 1;
+    LHL_TRY(solve_device(h, a));
+    LHL_TRY(residual_device(h, plan_resid_args(p)));
+    LHL_TRY(conv_device(h, it, p->max_iter, p->tol, p->nshift, p->cbd.as<int64_t>(), p->norms.as<double>(), p->nc,
+                        p->sact.as<int>(), p->cact.as<int>(), p->iters.as<int>()));
+    return {};
+}
+
+// the flags to the host
+Status plan_flags(lhl_plan *p) {
+    lhl_handle *h = p->h;
+    LHL_HIP_TRY(hipMemcpyAsync(p->p_sact, p->sact.p, sizeof(int) * (size_t)p->nshift, hipMemcpyDeviceToHost, h->stream));
+    return {};
+}
+
+// the solutions out, and info / iterations to the host
+Status plan_tail(lhl_plan *p) {
+    lhl_handle *h = p->h;
+    LHL_TRY(stage_out(h, p->xst, p->ld, p->plane, LHL_F64, p->X, p->n, p->nc, p->ldx, LHL_F64, p->cplx, 1));
+    LHL_HIP_TRY(hipMemcpyAsync(p->p_info, p->pool.dinfo.p, sizeof(int) * (size_t)p->nshift, hipMemcpyDeviceToHost,
+                               h->stream));
+    LHL_HIP_TRY(hipMemcpyAsync(p->p_iters, p->iters.p, sizeof(int) * (size_t)p->nshift, hipMemcpyDeviceToHost,
+                               h->stream));
+    return plan_flags(p);
+}
+
+bool plan_any_active(const lhl_plan *p) {
+    for (int64_t s = 0; s < p->nshift; s++)
+        if (p->p_sact[s]) return true;
+    return false;
+}
+
+Status plan_capture(lhl_plan *p, int rounds) {
+    lhl_handle *h = p->h;
+    plan_free_graph(p);
+    const bool prof = h->prof.on;
+    h->prof.on = false;   // no timing events inside the graph
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        h->prof.on = prof;
+        return {LHL_ERR_HIP, std::string("plan: begin capture: ") + hipGetErrorString(e)};
+    }
+    Status st = plan_head(p);
+    for (int it = 1; st.ok() && p->precision == LHL_PREC_REFINE64 && it <= rounds; it++) st = plan_round(p, it);
+    if (st.ok()) st = plan_tail(p);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(h->stream, &g);
+    h->prof.on = prof;
+    if (!st.ok() || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        if (st.ok()) st = Status{LHL_ERR_HIP, std::string("plan: end capture: ") + hipGetErrorString(e)};
+        return st;
+    }
+    p->graph = g;
+    e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        plan_free_graph(p);
+        return {LHL_ERR_HIP, std::string("plan: instantiate: ") + hipGetErrorString(e)};
+    }
+    p->graph_rounds = rounds;
+    p->sig = plan_signature(h);
+    p->captures++;
+    return {};
+}
+
+}  // namespace
+
+int lhl_plan_create(lhl_handle *h, lhl_plan **out, int64_t nshift, int shifts_complex, const int64_t *rhs_offsets,
+                    const double *B, int64_t ldb, double *X, int64_t ldx, int precision, double tol, int max_iter) {
+    if (!h || !out) return LHL_ERR_ARG;
+    *out = nullptr;
+    host_keep_reset(h);
+    if (nshift <= 0 || !rhs_offsets) return fail(h, LHL_ERR_ARG, "plan: need at least one shift and its offsets");
+    if (precision != LHL_PREC_WORKING && precision != LHL_PREC_REFINE64) return fail(h, LHL_ERR_ARG, "plan: bad precision");
+    if (!h->reduced) return fail(h, LHL_ERR_STATE, "plan: no reduction held");
+    if (precision == LHL_PREC_REFINE64 && h->j_dtype != LHL_F64)
+        return fail(h, LHL_ERR_STATE, "plan: FP64 refinement needs a workspace reduced from an FP64 J");
+    if (max_iter < 0) return fail(h, LHL_ERR_ARG, "plan: max_iter < 0");
+    if (rhs_offsets[0] < 0) return fail(h, LHL_ERR_DIM, "plan: negative right-hand-side offset");
+    for (int64_t s = 0; s < nshift; s++)
+        if (rhs_offsets[s + 1] < rhs_offsets[s])
+            return fail(h, LHL_ERR_DIM, "plan: right-hand-side offsets decrease at shift " + std::to_string(s));
+    const int64_t n = h->n, nc = rhs_offsets[nshift] - rhs_offsets[0];
+    if (n == 0 || nc == 0) return fail(h, LHL_ERR_DIM, "plan: no rows or no right-hand sides");
+    if (ldb < n || ldx < n || !B || !X) return fail(h, LHL_ERR_DIM, "plan: bad right-hand-side layout");
+    for (const void *ptr : {(const void *)B, (const void *)X}) {
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, ptr) != hipSuccess || at.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            return fail(h, LHL_ERR_ARG, "plan: B and X must be device memory");
+        }
+    }
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    const int planes = shifts_complex ? 2 : 1;
+    // one chunk: the pool of every shift's factors must fit the bound the
+    // batched call uses for a chunk
+    const size_t ubytes = elt_size(h->dtype) * (size_t)upack_len(n) * planes;
+    if (shift_chunk(h, nshift, ubytes) < nshift)
+        return fail(h, LHL_ERR_ARG, "plan: the shifts' factors exceed one chunk's bound; split the batch");
+    lhl_plan *p = new (std::nothrow) lhl_plan;
+    if (!p) return fail(h, LHL_ERR_NOMEM, "plan: out of host memory");
+    p->h = h;
+    p->n = n;
+    p->nshift = nshift;
+    p->nc = nc;
+    p->cplx = shifts_complex ? 1 : 0;
+    p->planes = planes;
+    p->precision = precision;
+    p->tol = tol;
+    p->max_iter = precision == LHL_PREC_REFINE64 ? max_iter : 0;
+    p->cb.resize((size_t)nshift + 1);
+    for (int64_t s = 0; s <= nshift; s++) p->cb[s] = rhs_offsets[s] - rhs_offsets[0];
+    p->B = B + (size_t)rhs_offsets[0] * ldb * planes;
+    p->X = X + (size_t)rhs_offsets[0] * ldx * planes;
+    p->ldb = ldb;
+    p->ldx = ldx;
+    p->ld = std::max<int64_t>(n, 1);   // the staged planar layout (as stage_in lays it out)
+    p->plane = p->ld * nc;
+    auto ok = [&](Status s2) {
+        if (!s2.ok() && st.ok()) st = s2;
+    };
+    const size_t splane = 8 * (size_t)std::max<int64_t>(n, 1) * nc * planes + 64;
+    ok(p->bst.ensure(splane));
+    ok(p->xst.ensure(splane));
+    ok(p->rst.ensure(splane));
+    ok(p->norms.ensure(sizeof(double) * 2 * (size_t)nc));
+    ok(p->colshift.ensure(32 * (size_t)nc));
+    ok(p->shiftpar.ensure(32 * (size_t)nshift));
+    ok(p->cbd.ensure(sizeof(int64_t) * (size_t)(nshift + 1)));
+    ok(p->cact.ensure(sizeof(int) * (size_t)nc));
+    ok(p->sact.ensure(sizeof(int) * (size_t)nshift));
+    ok(p->iters.ensure(sizeof(int) * (size_t)nshift));
+    if (st.ok()) {
+        hipError_t e = hipMemcpy(p->cbd.p, p->cb.data(), sizeof(int64_t) * (size_t)(nshift + 1), hipMemcpyHostToDevice);
+        if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+    }
+    if (st.ok()) st = groups_device(h, nshift, p->cb.data(), planes, p->groups, &p->ngroups);
+    if (st.ok()) {
+        const size_t bytes = 32 * (size_t)nshift + 32 * (size_t)nc + 3 * sizeof(int) * (size_t)nshift + 256;
+        hipError_t e = hipHostMalloc(&p->pin, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) st = Status{LHL_ERR_NOMEM, std::string("plan: pinned memory: ") + hipGetErrorString(e)};
+        else {
+            char *c = static_cast<char *>(p->pin);
+            p->p_shift = reinterpret_cast<double *>(c);
+            p->p_col = p->p_shift + 4 * nshift;
+            p->p_info = reinterpret_cast<int *>(p->p_col + 4 * nc);
+            p->p_sact = p->p_info + nshift;
+            p->p_iters = p->p_sact + nshift;
+            std::memset(p->pin, 0, bytes);
+        }
+    }
+    if (!st.ok()) {
+        lhl_plan_destroy(p);
+        return fail(h, st);
+    }
+    *out = p;
+    return LHL_OK;
+}
+
+int lhl_plan_run(lhl_plan *p, const double *sigma, const double *tau, int64_t *info, int *iters) {
+    if (!p || !p->h) return LHL_ERR_ARG;
+    lhl_handle *h = p->h;
+    host_keep_reset(h);
+    if (!sigma || !tau) return fail(h, LHL_ERR_ARG, "plan: null shifts");
+    if (!h->reduced || h->n != p->n) return fail(h, LHL_ERR_STATE, "plan: the workspace no longer holds a reduction of its n");
+    if (!p->cplx)
+        for (int64_t s = 0; s < p->nshift; s++)
+            if (sigma[2 * s + 1] != 0.0 || tau[2 * s + 1] != 0.0)
+                return fail(h, LHL_ERR_ARG, "plan: complex shift on a plan for real shifts");
+    Status st = set_device(h);
+    if (!st.ok()) return fail(h, st);
+    // this run's shifts into the pinned block the uploads read
+    for (int64_t s = 0; s < p->nshift; s++) {
+        double *d = p->p_shift + 4 * s;
+        d[0] = sigma[2 * s];
+        d[1] = sigma[2 * s + 1];
+        d[2] = tau[2 * s];
+        d[3] = tau[2 * s + 1];
+        for (int64_t c = p->cb[s]; c < p->cb[s + 1]; c++) std::memcpy(p->p_col + 4 * c, d, 32);
+    }
+    const bool refine = p->precision == LHL_PREC_REFINE64;
+    int done = 0;   // refinement rounds enqueued
+    bool graphed = false;
+    if (p->rounds >= 0 && !p->no_graph) {
+        if (!p->exec || p->graph_rounds != p->rounds || p->sig != plan_signature(h)) {
+            Status cs = plan_capture(p, p->rounds);
+            if (!cs.ok()) p->no_graph = 1;   // run round by round from now on
+        }
+        if (p->exec) {
+            hipError_t e = hipGraphLaunch(p->exec, h->stream);
+            if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("plan: graph launch: ") + hipGetErrorString(e));
+            p->replays++;
+            done = p->graph_rounds;
+            graphed = true;
+        }
+    }
+    if (!graphed) {
+        st = plan_head(p);
+        if (st.ok() && refine) st = plan_flags(p);
+    }
+    hipError_t e = st.ok() ? hipStreamSynchronize(h->stream) : hipSuccess;
+    if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+    // rounds beyond the graph's (or every round, learning): one at a time
+    bool extra = false;
+    while (st.ok() && refine && done < p->max_iter && plan_any_active(p)) {
+        done++;
+        extra = true;
+        st = plan_round(p, done);
+        if (st.ok()) st = plan_flags(p);
+        if (st.ok()) {
+            e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+        }
+        if (graphed) p->extra_rounds++;
+    }
+    if (st.ok() && (!graphed || extra)) {
+        st = plan_tail(p);
+        if (st.ok()) {
+            e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+        }
+    }
+    if (!st.ok()) {
+        (void)hipStreamSynchronize(h->stream);
+        return fail(h, st);
+    }
+    int need = 0;
+    int64_t unconverged = 0;
+    for (int64_t s = 0; s < p->nshift; s++) {
+        need = std::max(need, p->p_iters[s]);
+        if (refine && p->p_sact[s]) unconverged++;
+        if (info) info[s] = p->p_info[s];
+        if (iters) iters[s] = p->p_iters[s];
+    }
+    // the next run's graph: the rounds this batch needed (never fewer than a
+    // graph that already ran without extra rounds)
+    if (refine) p->rounds = (graphed && !extra) ? std::max(need, p->graph_rounds) : need;
+    else p->rounds = 0;
+    if (unconverged > 0)
+        return fail(h, LHL_NOT_CONVERGED, "plan: " + std::to_string(unconverged) + " shift(s) still have a column "
+                                          "above tol after " + std::to_string(p->max_iter) + " refinement rounds");
+    return LHL_OK;
+}
+
+int lhl_plan_stats(const lhl_plan *p, int *graph_rounds, int64_t *replays, int64_t *captures, int64_t *extra_rounds) {
+    if (!p) return LHL_ERR_ARG;
+    if (graph_rounds) *graph_rounds = p->exec ? p->graph_rounds : -1;
+    if (replays) *replays = p->replays;
+    if (captures) *captures = p->captures;
+    if (extra_rounds) *extra_rounds = p->extra_rounds;
+    return LHL_OK;
+}
+
+int lhl_plan_destroy(lhl_plan *p) {
+    if (!p) return LHL_OK;
+    if (p->h) {
+        (void)hipSetDevice(p->h->device);
+        (void)hipStreamSynchronize(p->h->stream);
+    }
+    plan_free_graph(p);
+    for (DevBuf *b : {&p->bst, &p->xst, &p->rst, &p->norms, &p->colshift, &p->shiftpar, &p->cbd, &p->cact, &p->sact,
+                      &p->iters, &p->groups, &p->ybuf, &p->rbuf, &p->jx, &p->jxi, &p->pool.U, &p->pool.l,
+                      &p->pool.rdiag, &p->pool.swp, &p->pool.dinfo, &p->pool.params, &p->pool.Uinv, &p->pool.UT})
+        b->release();
+    if (p->pin) (void)hipHostFree(p->pin);
+    delete p;
+    return LHL_OK;
+}
+
 static int applyz_common(lhl_handle *h, int inverse, void *x, int64_t n, int64_t nrhs, int64_t ldx, int xdt,
                          int x_complex, int on_device) {
     if (!h) return LHL_ERR_ARG;
@@ -983,7 +1399,7 @@ struct Knob {
 const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
-    {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34},
+    {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34}, {"gemm_impl", 0, 1},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;
@@ -1012,6 +1428,7 @@ int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value) {
     else if (k == "reduce_switch") h->tune_switch = value == -1 ? 0 : (v == 0 ? -1000000 : v);
     else if (k == "lsweep_depth") h->tune_lsweep = value == -1 ? -1 : v;
     else if (k == "h2d_chunk_bytes") h->tune_h2d_chunk = value;
+    else if (k == "gemm_impl") h->tune_gemm = v;
     return LHL_OK;
 }
 
@@ -1024,6 +1441,7 @@ int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value) {
     else if (k == "reduce_switch") *value = h->tune_switch > 0 ? h->tune_switch : (h->tune_switch < 0 ? 0 : -1);
     else if (k == "lsweep_depth") *value = h->tune_lsweep;
     else if (k == "h2d_chunk_bytes") *value = h->tune_h2d_chunk;
+    else if (k == "gemm_impl") *value = h->tune_gemm;
     else return fail(h, LHL_ERR_ARG, "lhl_tune_get: unknown knob " + k);
     return LHL_OK;
 }

@@ -35,11 +35,12 @@ constexpr int kT = 64;   // tile edge
 template <typename T, typename TX>
 __global__ void k_gather(int n, const int *__restrict__ perm, const double *__restrict__ iscale,
                          const TX *__restrict__ x, int64_t ldx, int64_t xplane, int planes, T *__restrict__ Y,
-                         T *__restrict__ Y2, int64_t ldy) {
+                         T *__restrict__ Y2, int64_t ldy, const int *__restrict__ cact) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     if (i >= n) return;
     const int col = q / planes, pl = q % planes;
+    if (cact && !cact[col]) return;   // a converged column (refinement rounds)
     const int p = perm[i];
     const T v = (T)(x[p + (int64_t)col * ldx + pl * xplane] * (TX)iscale[p]);
     Y[i + (int64_t)q * ldy] = v;
@@ -82,11 +83,12 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
 template <typename T, typename TX>
 __global__ void k_scatter(int n, const T *__restrict__ Yn, const T *__restrict__ Y, int64_t ldy, int planes,
                           const int *__restrict__ perm, const double *__restrict__ scale, TX *__restrict__ x,
-                          int64_t ldx, int64_t xplane, int accumulate) {
+                          int64_t ldx, int64_t xplane, int accumulate, const int *__restrict__ cact) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
     if (p >= n) return;
     const int col = q / planes, pl = q % planes;
+    if (cact && !cact[col]) return;   // a converged column keeps its x
     const T yn = p == 0 ? Y[(int64_t)q * ldy] : Yn[p + (int64_t)q * ldy];
     const int dst_row = perm[p];
     TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
@@ -297,7 +299,8 @@ template <typename T> __device__ __forceinline__ V16<T> ld16u(const T *p) { retu
 template <typename T, int UC>
 __global__ void __launch_bounds__(64 * (64 / utrsm4_rows<T>()))
 k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restrict__ Uinv, int64_t uistride,
-         const UGroup *__restrict__ groups, const T *__restrict__ Yin, T *__restrict__ Yf, int64_t ldy) {
+         const UGroup *__restrict__ groups, const T *__restrict__ Yin, T *__restrict__ Yf, int64_t ldy,
+         const int *__restrict__ sact) {
     constexpr int UP = UC ? 2 : 1;
     constexpr int RW = utrsm4_rows<T>();
     constexpr int NV = RW * kNC;           // partial sums per lane
@@ -307,6 +310,7 @@ k_utrsm4(int n, const T *__restrict__ Ubase, int64_t ustride, const T *__restric
     constexpr int NPRE = kT * kT * UP / NT;   // diagonal-inverse elements per thread
     const UGroup g = groups[blockIdx.x];
     if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
+    if (sact && !sact[g.shift]) return;   // a converged shift (whole workgroup)
     const int nbu = (n + kT - 1) / kT;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const T *Us = Ubase + (int64_t)g.shift * ustride;
@@ -533,7 +537,7 @@ template <typename T, int UC, int NG, int NT>
 __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTbase, int64_t ustride,
                                                  const T *__restrict__ Uinv, int64_t uistride,
                                                  const UGroup *__restrict__ groups, const T *__restrict__ Yin,
-                                                 T *__restrict__ Yf, int64_t ldy) {
+                                                 T *__restrict__ Yf, int64_t ldy, const int *__restrict__ sact) {
     constexpr int V = ut_vec<T>();
     using VT = T __attribute__((ext_vector_type(V)));
     constexpr int UP = UC ? 2 : 1;
@@ -545,6 +549,7 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
     constexpr int GB = kT / V;         // groups per 64-row block
     const UGroup g = groups[blockIdx.x];
     if (g.nq <= 0) return;   // padding slot of the XCD placement (whole workgroup)
+    if (sact && !sact[g.shift]) return;   // a converged shift (whole workgroup)
     const int nbu = (n + kT - 1) / kT;
     const int tid = threadIdx.x;
     const T *Ut = UTbase + (int64_t)g.shift * ustride;
@@ -731,12 +736,14 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
 template <typename T, int UC>
 __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ lbase, int64_t lstride,
                                                   const uint8_t *__restrict__ swbase, int64_t sstride,
-                                                  const UGroup *__restrict__ groups, T *__restrict__ Y, int64_t ldy) {
+                                                  const UGroup *__restrict__ groups, T *__restrict__ Y, int64_t ldy,
+                                                  const int *__restrict__ sact) {
 #pragma clang fp contract(off)
     constexpr int PL = UC ? 2 : 1;               // internal columns per chain
     const UGroup g = groups[blockIdx.x];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (PL * wv >= g.nq) return;                 // whole wave: no chain here
+    if (sact && !sact[g.shift]) return;          // a converged shift
     const int steps = n - 1;
     const int bs = (steps + 63) / 64;
     const T *lr = lbase + (int64_t)g.shift * lstride;
@@ -876,10 +883,12 @@ __global__ void __launch_bounds__(256) k_gemm_jx(int n, const TW *__restrict__ J
 template <typename TX>
 __global__ void k_resid_combine(int n, const TX *__restrict__ JX, int64_t ldq, const TX *__restrict__ JXi,
                                 const TX *__restrict__ X, const TX *__restrict__ B, TX *__restrict__ R, int64_t ld,
-                                int64_t plane, int planes, const double *__restrict__ colshift, int w_complex) {
+                                int64_t plane, int planes, const double *__restrict__ colshift, int w_complex,
+                                const int *__restrict__ cact) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int col = blockIdx.y;
     if (i >= n) return;
+    if (cact && !cact[col]) return;
     const int64_t o = i + (int64_t)col * ld;
     if (colshift) {
         const TX sr = (TX)colshift[4 * col], si = (TX)colshift[4 * col + 1];
@@ -907,8 +916,9 @@ __global__ void k_resid_combine(int n, const TX *__restrict__ JX, int64_t ldq, c
 // norms[c] = ||r_c||_inf, norms[ncol + c] = ||b_c||_inf (complex: the modulus)
 template <typename TX>
 __global__ void k_colnorms(int n, const TX *__restrict__ r, const TX *__restrict__ b, int64_t ld, int64_t plane,
-                           int planes, int ncol, double *__restrict__ norms) {
+                           int planes, int ncol, double *__restrict__ norms, const int *__restrict__ cact) {
     const int c = blockIdx.x;
+    if (cact && !cact[c]) return;   // whole workgroup: the column's norms stay as they were
     double mr = 0, mb = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int64_t o = i + (int64_t)c * ld;
@@ -944,6 +954,14 @@ namespace {
 
 inline int64_t ldy_for(int64_t n) { return std::max<int64_t>(64, ((n + 63) / 64) * 64); }
 
+// The solves' dense products on the library's MFMA kernels or on rocBLAS
+// (tuning knob gemm_impl; rocBLAS by default, faster at full width).  Column
+// masks need the library's kernels, unless the caller asks for the products
+// over every column (full = 1: a round where most columns are still active).
+inline bool own_products(const lhl_handle *h, const int *cact, int full) {
+    return h->tune_gemm == 1 || (cact != nullptr && !full);
+}
+
 
 Status blas_handle(lhl_handle *h, rocblas_handle *out) {
     if (!h->blas) {
@@ -970,10 +988,16 @@ Status blas_handle(lhl_handle *h, rocblas_handle *out) {
 // W and X): one GEMM per plane, the output columns interleaved by plane.
 template <typename T>
 Status blas_jx(lhl_handle *h, const T *W, int64_t ldw, const T *X, int64_t ld, int64_t plane, int planes, int ncol,
-               T *JX, int64_t ldq) {
+               T *JX, int64_t ldq, const int *cact = nullptr) {
+    const int n = (int)h->n;
+    if (own_products(h, cact, 0)) {
+        for (int pl = 0; pl < planes; pl++)
+            LHL_TRY(mgemm(h, "gemm_jx", 0, n, ncol, n, W, ldw, X + pl * plane, ld, JX + pl * ldq, planes * ldq, cact,
+                          1));
+        return {};
+    }
     rocblas_handle b;
     LHL_TRY(blas_handle(h, &b));
-    const int n = (int)h->n;
     const T one = 1, zero = 0;
     ProfScope ps(h, "gemm_jx");
     for (int pl = 0; pl < planes; pl++) {
@@ -1050,46 +1074,29 @@ __global__ void k_unit_lower(int m, const T *__restrict__ src, int64_t lds, T *_
     }
 }
 
-template <typename T, typename TX>
-Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
+// I + L' (L'(i, m) = A(i+1, m)) as a dense unit lower matrix, the rocBLAS
+// backward sweep's operand, built once per reduction on first use.
+template <typename T>
+Status lp1_typed(lhl_handle *h) {
+    if (h->lp1_ready) return {};
     const int n = (int)h->n;
-    const int planes = a.x_complex ? 2 : 1;
-    const int Q = (int)(a.ncol * planes);
-    const int64_t ldy = ldy_for(n);
-    LHL_TRY(h->ybuf.ensure(sizeof(T) * (size_t)ldy * Q));
-    LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
-    T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
-    ProfWork pw(h, a.nshift, Q);
-    const T *A = h->A.as<T>();
-    if (a.phase != 2) {
-        // y = P^-1 D^-1 x, then y' <- L'^-1 y' as a product with the explicit
-        // inverse formed by lhl() (final rows in Yo)
-        const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
-        LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
-                   (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
-                   ldy);
-        LHL_TRY(blas_gemm_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1,
-                                   ldy, Yo + 1, ldy));
-        if (a.phase == 1) {
-            LHL_HIP_TRY(hipGetLastError());
-            return {};
-        }
-    }
-    // the back substitution's column groups (within one shift)
-    std::vector<int64_t> cb((size_t)a.nshift + 1);
-    if (a.col_begin) {
-        for (int64_t s = 0; s <= a.nshift; s++) cb[s] = a.col_begin[s];
-    } else {
-        cb[0] = 0;
-        cb[1] = a.ncol;
-    }
-    // U solve groups: kNC internal columns of one shift each.  Workgroups are
-    // dealt to the 8 XCDs round-robin (blockIdx % 8), so all groups of a shift
-    // go to slots of one residue class and share that XCD's L2 copy of U;
-    // empty slots (nq = 0) pad the classes to equal length.
+    LHL_TRY(h->Lp1.ensure(sizeof(T) * (size_t)h->ldlf * (size_t)std::max(n - 1, 1)));
+    const dim3 gl((unsigned)((n - 1 + 255) / 256), (unsigned)(n - 1));
+    LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->A.as<T>() + 1, h->lda,
+               h->Lp1.as<T>(), h->ldlf);
+    LHL_HIP_TRY(hipGetLastError());
+    h->lp1_ready = 1;
+    return {};
+}
+
+// U solve groups: kNC internal columns of one shift each.  Workgroups are
+// dealt to the 8 XCDs round-robin (blockIdx % 8), so all groups of a shift go
+// to slots of one residue class and share that XCD's L2 copy of U; empty
+// slots (nq = 0) pad the classes to equal length.
+std::vector<UGroup> build_groups(int64_t nshift, const int64_t *cb, int planes) {
     constexpr int kXcd = 8;
     std::vector<std::vector<UGroup>> per(kXcd);
-    for (int64_t s = 0; s < a.nshift; s++) {
+    for (int64_t s = 0; s < nshift; s++) {
         for (int64_t q = cb[s] * planes; q < cb[s + 1] * planes; q += kNC)
             per[s % kXcd].push_back(UGroup{(int)s, (int)q, (int)std::min<int64_t>(kNC, cb[s + 1] * planes - q), 0});
     }
@@ -1098,30 +1105,80 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     std::vector<UGroup> ug(depth * kXcd, UGroup{0, 0, 0, 0});
     for (int x = 0; x < kXcd; x++)
         for (size_t j = 0; j < per[x].size(); j++) ug[j * kXcd + x] = per[x][j];
-    LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
-    UGroup *dg = h->gbuf.as<UGroup>();
-    // ordered on the handle's stream: an earlier solve's back substitution may
-    // still be reading the table
-    const size_t ugb = sizeof(UGroup) * ug.size();
-    unsigned char *hug = host_keep(h, ugb);
-    std::memcpy(hug, ug.data(), ugb);
-    LHL_HIP_TRY(hipMemcpyAsync(dg, hug, ugb, hipMemcpyHostToDevice, h->stream));
+    return ug;
+}
+
+template <typename T, typename TX>
+Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
+    const int n = (int)h->n;
+    const int planes = a.x_complex ? 2 : 1;
+    const int Q = (int)(a.ncol * planes);
+    const int64_t ldy = ldy_for(n);
+    DevBuf &yb = a.ybuf ? *a.ybuf : h->ybuf, &rb = a.rbuf ? *a.rbuf : h->rbuf;
+    LHL_TRY(yb.ensure(sizeof(T) * (size_t)ldy * Q));
+    LHL_TRY(rb.ensure(sizeof(T) * (size_t)ldy * Q));
+    T *Yw = yb.as<T>(), *Yo = rb.as<T>();
+    ProfWork pw(h, a.nshift, Q);
+    const T *A = h->A.as<T>();
+    if (a.phase != 2) {
+        // y = P^-1 D^-1 x, then y' <- L'^-1 y' as a product with the explicit
+        // inverse formed by lhl() (final rows in Yo)
+        const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
+        LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
+                   (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
+                   ldy, a.cact);
+        if (own_products(h, a.cact, a.full_products))
+            LHL_TRY(mgemm(h, "trsm", 1, n - 1, Q, n - 1, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
+                          Yo + 1, ldy, a.cact, planes));
+        else
+            LHL_TRY(blas_gemm_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf,
+                                       (const T *)Yw + 1, ldy, Yo + 1, ldy));
+        if (a.phase == 1) {
+            LHL_HIP_TRY(hipGetLastError());
+            return {};
+        }
+    }
+    // the back substitution's column groups (within one shift)
+    const UGroup *dg;
+    size_t ngroups;
+    if (a.groups) {   // uploaded once by the caller (a plan)
+        dg = (const UGroup *)a.groups;
+        ngroups = (size_t)a.ngroups;
+    } else {
+        std::vector<int64_t> cb((size_t)a.nshift + 1);
+        if (a.col_begin) {
+            for (int64_t s = 0; s <= a.nshift; s++) cb[s] = a.col_begin[s];
+        } else {
+            cb[0] = 0;
+            cb[1] = a.ncol;
+        }
+        const std::vector<UGroup> ug = build_groups(a.nshift, cb.data(), planes);
+        LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
+        // ordered on the handle's stream: an earlier solve's back substitution
+        // may still be reading the table
+        const size_t ugb = sizeof(UGroup) * ug.size();
+        unsigned char *hug = host_keep(h, ugb);
+        std::memcpy(hug, ug.data(), ugb);
+        LHL_HIP_TRY(hipMemcpyAsync(h->gbuf.p, hug, ugb, hipMemcpyHostToDevice, h->stream));
+        dg = h->gbuf.as<UGroup>();
+        ngroups = ug.size();
+    }
 
     const int N1 = n - 1;
     // forward interchange chain in place on Yo: a scan per column
     if (n >= 2) {
         if (a.shift_complex)
-            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 1>), dim3((unsigned)ug.size()), dim3(256), 0, n,
-                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy);
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 1>), dim3((unsigned)ngroups), dim3(256), 0, n,
+                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy, a.sact);
         else
-            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0>), dim3((unsigned)ug.size()), dim3(256), 0, n,
-                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy);
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0>), dim3((unsigned)ngroups), dim3(256), 0, n,
+                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy, a.sact);
         LHL_HIP_TRY(hipGetLastError());
     }
     // x <- U^-1 y per shift (solution in Yw)
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
-    const dim3 gu((unsigned)ug.size());
+    const dim3 gu((unsigned)ngroups);
     if (a.UT && n <= 512 * 16) {
         // U in packed columns, right-hand sides in registers (512 threads:
         // measured faster than 1024 on this shape)
@@ -1130,7 +1187,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
             constexpr int UC = decltype(uc)::value, NP = decltype(np)::value;
             constexpr int NTK = UC ? 512 : NTR;
             LHL_LAUNCH(h, "utrsm", (k_utrsm_t<T, UC, NP, NTK>), gu, dim3(NTK), 0, n, (const T *)a.UT, a.ustride,
-                       (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+                       (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy, a.sact);
             return {};
         };
         using F = std::integral_constant<int, 0>;
@@ -1156,18 +1213,25 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         LHL_TRY(st);
     } else if (a.shift_complex) {
         LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 1>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U,
-                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy, a.sact);
     } else {
         LHL_LAUNCH(h, "utrsm", (k_utrsm4<T, 0>), gu, dim3(64 * (64 / utrsm4_rows<T>())), 0, n, (const T *)a.U,
-                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy);
+                   a.ustride, (const T *)a.Uinv, uis, (const UGroup *)dg, (const T *)Yo, Yw, ldy, a.sact);
     }
     LHL_HIP_TRY(hipGetLastError());
     // y_new' = (I + L') y' (Yo is free again), then x = D P y_new
-    LHL_TRY(blas_gemm_lower<T>(h, "trmm", N1, Q, (const T *)h->Lp1.as<T>(), h->ldlf, (const T *)Yw + 1, ldy, Yo + 1,
-                               ldy));
+    if (own_products(h, a.cact, a.full_products)) {
+        // (I + L') y' straight from the factors' strict lower triangle
+        LHL_TRY(mgemm(h, "trmm", 2, N1, Q, N1, (const T *)h->A.as<T>() + 1, h->lda, (const T *)Yw + 1, ldy, Yo + 1,
+                      ldy, a.cact, planes));
+    } else {
+        LHL_TRY(lp1_typed<T>(h));
+        LHL_TRY(blas_gemm_lower<T>(h, "trmm", N1, Q, (const T *)h->Lp1.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
+                                   Yo + 1, ldy));
+    }
     LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), dim3((unsigned)((n + 255) / 256), (unsigned)Q), dim3(256), 0, n,
                (const T *)Yo, (const T *)Yw, ldy, planes, (const int *)h->perm.as<int>(),
-               (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate);
+               (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate, a.cact);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -1205,18 +1269,20 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     const int Q = (int)(a.ncol * planes);
     const int64_t ldq = ldy_for(n);
     ProfWork pw(h, 0, Q);
-    LHL_TRY(h->ybuf.ensure(sizeof(TX) * (size_t)ldq * Q));
-    TX *JX = h->ybuf.as<TX>();
+    DevBuf &jb = a.jxbuf ? *a.jxbuf : h->ybuf, &jib = a.jxibuf ? *a.jxibuf : h->rbuf;
+    LHL_TRY(jb.ensure(sizeof(TX) * (size_t)ldq * Q));
+    TX *JX = jb.as<TX>();
     TX *JXi = nullptr;
     const dim3 gg((unsigned)((n + kT - 1) / kT), (unsigned)((Q + kT - 1) / kT));
     if constexpr (std::is_same<TW, TX>::value) {
         // plain GEMM in one precision: the library's tuned kernels
         const TW *W = a.W ? (const TW *)a.W : (const TW *)h->J.p;
         const int64_t ldw = a.W ? a.ldw : h->ldj;
-        LHL_TRY(blas_jx<TX>(h, W, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol, JX, ldq));
+        LHL_TRY(blas_jx<TX>(h, W, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol, JX, ldq,
+                            a.full_products ? nullptr : a.cact));
         if (a.W && a.w_complex) {
-            LHL_TRY(h->rbuf.ensure(sizeof(TX) * (size_t)ldq * Q));
-            JXi = h->rbuf.as<TX>();
+            LHL_TRY(jib.ensure(sizeof(TX) * (size_t)ldq * Q));
+            JXi = jib.as<TX>();
             LHL_TRY(blas_jx<TX>(h, W + ldw * (int64_t)n, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol,
                                 JXi, ldq));
         }
@@ -1224,8 +1290,8 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
         LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)a.W, a.ldw, (const TX *)a.x,
                    a.ld, a.plane, planes, Q, JX, ldq);
         if (a.w_complex) {
-            LHL_TRY(h->rbuf.ensure(sizeof(TX) * (size_t)ldq * Q));
-            JXi = h->rbuf.as<TX>();
+            LHL_TRY(jib.ensure(sizeof(TX) * (size_t)ldq * Q));
+            JXi = jib.as<TX>();
             const TW *Wi = (const TW *)a.W + a.ldw * (int64_t)n;
             LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, Wi, a.ldw, (const TX *)a.x, a.ld,
                        a.plane, planes, Q, JXi, ldq);
@@ -1237,9 +1303,9 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     const dim3 gc((unsigned)((n + 255) / 256), (unsigned)a.ncol);
     LHL_LAUNCH(h, "resid_combine", k_resid_combine<TX>, gc, dim3(256), 0, n, (const TX *)JX, ldq, (const TX *)JXi,
                (const TX *)a.x, (const TX *)a.b, (TX *)a.r, a.ld, a.plane, planes, a.W ? nullptr : a.colshift,
-               a.W ? a.w_complex : 0);
+               a.W ? a.w_complex : 0, a.cact);
     LHL_LAUNCH(h, "colnorms", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
-               (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms);
+               (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms, a.cact);
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
@@ -1260,21 +1326,67 @@ Status linv_typed(lhl_handle *h) {
     if (nbt > 0)
         LHL_LAUNCH(h, "reduce_finish", k_linv_full<T>, dim3((unsigned)((n - 1 + 15) / 16)), dim3(512), 0, n,
                    (const T *)h->A.as<T>(), h->lda, (const T *)h->Linv.as<T>(), h->Lfull.as<T>(), ldf);
-    // both sweeps of the solves as dense GEMMs: L'^-1 with its diagonal and
-    // upper part set, and I + L' (L'(i, m) = A(i+1, m)) as a dense copy
-    LHL_TRY(h->Lp1.ensure(sizeof(T) * (size_t)ldf * (size_t)std::max(n - 1, 1)));
+    // the forward sweep as one product: L'^-1 with its diagonal and upper part
+    // set (the backward sweep's I + L' is read from the factors, or built as a
+    // dense copy for rocBLAS on first use)
+    h->lp1_ready = 0;
     if (n >= 2) {
         const dim3 gl((unsigned)((n - 1 + 255) / 256), (unsigned)(n - 1));
         LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->Lfull.as<T>(), ldf,
                    h->Lfull.as<T>(), ldf);
-        LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->A.as<T>() + 1, h->lda,
-                   h->Lp1.as<T>(), ldf);
     }
     LHL_HIP_TRY(hipGetLastError());
     return {};
 }
 
 }  // namespace
+
+namespace {
+
+__global__ void k_fill_i32(int *__restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// The refinement's convergence decision on the device, after round `it`'s
+// residual: a shift still active whose columns all satisfy
+// ||r_c|| <= tol ||b_c|| (norms[c], norms[ncol + c]; NaN fails) is retired
+// (sact[s] = 0 and its columns' cact = 0, so later rounds skip it); a shift
+// that fails is refined once more when it < max_iter (iters[s] = it + 1) and
+// otherwise stays flagged as not converged.  One thread per shift.
+__global__ void k_conv(int it, int max_iter, double tol, int64_t nshift, const int64_t *__restrict__ cb,
+                       const double *__restrict__ norms, int64_t ncol, int *__restrict__ sact, int *__restrict__ cact,
+                       int *__restrict__ iters) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshift || !sact[s]) return;
+    bool bad = false;
+    for (int64_t c = cb[s]; c < cb[s + 1]; c++)
+        if (!(norms[c] <= tol * norms[ncol + c])) bad = true;
+    if (!bad) {
+        sact[s] = 0;
+        for (int64_t c = cb[s]; c < cb[s + 1]; c++) cact[c] = 0;
+    } else if (it < max_iter) {
+        iters[s] = it + 1;
+    }
+}
+
+}  // namespace
+
+Status fill_i32_device(lhl_handle *h, int *p, int64_t n, int v) {
+    if (n <= 0) return {};
+    LHL_LAUNCH(h, "conv", k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p, n, v);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+Status conv_device(lhl_handle *h, int it, int max_iter, double tol, int64_t nshift, const int64_t *cb,
+                   const double *norms, int64_t ncol, int *sact, int *cact, int *iters) {
+    if (nshift <= 0) return {};
+    LHL_LAUNCH(h, "conv", k_conv, dim3((unsigned)((nshift + 255) / 256)), dim3(256), 0, it, max_iter, tol, nshift, cb,
+               norms, ncol, sact, cact, iters);
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
 
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
                    int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT) {
@@ -1287,6 +1399,15 @@ Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rd
 Status linv_device(lhl_handle *h) {
     if (h->n < 2) return {};
     return h->dtype == LHL_F64 ? linv_typed<double>(h) : linv_typed<float>(h);
+}
+
+Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups) {
+    const std::vector<UGroup> ug = build_groups(nshift, cb, planes);
+    LHL_TRY(out.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
+    if (!ug.empty()) LHL_HIP_TRY(hipMemcpy(out.p, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice));
+    *ngroups = (int64_t)ug.size();
+    (void)h;
+    return {};
 }
 
 Status solve_device(lhl_handle *h, const SolveArgs &a) {

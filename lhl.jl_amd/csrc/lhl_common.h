@@ -377,6 +377,9 @@ struct lhl_handle {
     int64_t tune_shift_wide = 0;   // n from which the shift LU's bulk is 960 threads wide (0: default)
     int64_t tune_pool_bytes = 0;   // bound on one chunk's packed U's in a batched solve (0: default)
     int64_t tune_h2d_chunk = -1;   // bytes per staged chunk of a host J's upload (-1: default, 0: runtime copy)
+    int tune_gemm = -1;            // the solves' dense products: 0 rocBLAS, 1 the library's MFMA kernels (-1: default)
+    int lp1_ready = 0;             // Lp1 holds I + L' of the current reduction (the rocBLAS sweep's operand)
+    const double *dev_shiftpar = nullptr;   // while set: the batched shift LUs read (sr, si, tr, ti) from here
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
     // host sources of asynchronous uploads (the solves' work tables, the
     // shifts' parameters): each upload reads its own block, kept until the
@@ -491,8 +494,25 @@ struct SolveArgs {
     // 0: the whole solve; 1: only the gather and the L solve (no shift state
     // read); 2: the rest, after a phase-1 call on the same columns
     int phase = 0;
+    // device flags per right-hand-side column and per shift (nullptr: all):
+    // columns with cact[c] == 0 and shifts with sact[s] == 0 are skipped (the
+    // refinement's converged ones); full_products = 1 runs the dense products
+    // over every column anyway (a round where most columns are still active)
+    const int *cact = nullptr;
+    const int *sact = nullptr;
+    int full_products = 0;
+    // a caller's own buffers (a plan's, whose graph holds their addresses):
+    // the back substitution's group table on the device (ngroups entries, as
+    // build_groups lays them out) and the two work buffers
+    const void *groups = nullptr;
+    int64_t ngroups = 0;
+    DevBuf *ybuf = nullptr, *rbuf = nullptr;
 };
 Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
+// the back substitution's group table for nshift shifts with column offsets
+// cb (host, nshift + 1) and planes per column, uploaded into `out`
+// (synchronously); *ngroups receives its length
+Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups);
 // after a shift LU: the inverses of the 64x64 diagonal blocks of nshift packed
 // U's (per-shift stride returned in *uistride) and U in packed columns (UT,
 // ustride apart)
@@ -524,8 +544,26 @@ struct ResidArgs {
     int xdt = LHL_F64;
     int x_complex = 0;
     double *norms = nullptr;
+    const int *cact = nullptr;    // per column flags (nullptr: all); as in SolveArgs
+    int full_products = 0;
+    DevBuf *jxbuf = nullptr, *jxibuf = nullptr;   // a caller's own work buffers (as in SolveArgs)
 };
 void blas_release(lhl_handle *h);                          // lhl_batch.hip
+// device-side refinement control (lhl_batch.hip): p[0:n] = v; and the
+// convergence step after round `it` (cb: nshift + 1 column offsets on the
+// device; norms as residual_device writes them for ncol columns)
+Status fill_i32_device(lhl_handle *h, int *p, int64_t n, int v);
+Status conv_device(lhl_handle *h, int it, int max_iter, double tol, int64_t nshift, const int64_t *cb,
+                   const double *norms, int64_t ncol, int *sact, int *cact, int *iters);
+// C = A B on the matrix cores (lhl_gemm.hip), column-major: mode 0 a full A
+// (m x K), 1 a lower-triangular A with zeros stored above the diagonal, 2
+// C = B + tril(A, -1) B (K == m).  cact (device, nullable): per batch column
+// flags, internal column j being batch column j / planes; 64-column tiles with
+// no flagged column are skipped (their C is left as it was).
+Status mgemm(lhl_handle *h, const char *name, int mode, int m, int q, int K, const float *A, int64_t lda,
+             const float *B, int64_t ldb, float *C, int64_t ldc, const int *cact, int planes);
+Status mgemm(lhl_handle *h, const char *name, int mode, int m, int q, int K, const double *A, int64_t lda,
+             const double *B, int64_t ldb, double *C, int64_t ldc, const int *cact, int planes);
 Status residual_device(lhl_handle *h, const ResidArgs &a);  // lhl_batch.hip
 
 // Planar <-> interleaved conversions and precision casts (lhl_solve.hip).

@@ -796,9 +796,30 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     return {};
 }
 
+// ShiftParam<T> from device doubles (sr, si, tr, ti per shift): the parameter
+// upload of a captured batched step, whose shifts change between replays.
+template <typename T>
+__global__ void k_shift_params(int64_t nshift, const double *__restrict__ src, ShiftParam<T> *__restrict__ dst) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshift) return;
+    dst[s].sr = (T)src[4 * s];
+    dst[s].si = (T)src[4 * s + 1];
+    dst[s].tr = (T)src[4 * s + 2];
+    dst[s].ti = (T)src[4 * s + 3];
+}
+
 template <typename T>
 Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, const double *tau, DevBuf &params,
                   T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
+    if (h->dev_shiftpar) {
+        // the shifts are on the device already (a plan's step)
+        LHL_TRY(params.ensure(sizeof(ShiftParam<T>) * (size_t)nshift));
+        LHL_LAUNCH(h, "shift", k_shift_params<T>, dim3((unsigned)((nshift + 255) / 256)), dim3(256), 0, nshift,
+                   h->dev_shiftpar, params.as<ShiftParam<T>>());
+        ProfWork pw(h, nshift, 0);
+        return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
+                                 params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info);
+    }
     std::vector<ShiftParam<T>> hp((size_t)nshift);
     for (int64_t s = 0; s < nshift; s++) {
         hp[s].sr = (T)sig[2 * s];

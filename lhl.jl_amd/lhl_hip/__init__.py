@@ -36,11 +36,11 @@ __all__ = [
     "ArgumentError", "DimensionMismatch", "LHLError", "LHLWorkspace", "LHLShift", "lhl", "lhl_",
     "lhl_reduce_", "lhl_shift_", "lhl_ldiv_", "lhl_refine_", "applyZ_", "applyZinv_",
     "shift_solve_batched", "library_path", "load_library", "profile_enable", "profile_reset", "profile_read",
-    "profile_read_work", "sync", "LHLNotConverged", "tune_set", "tune_get", "TUNING_KNOBS",
+    "profile_read_work", "sync", "LHLNotConverged", "tune_set", "tune_get", "TUNING_KNOBS", "BatchPlan",
 ]
 
 TUNING_KNOBS = ("reduce_blocked_min", "reduce_nb", "reduce_rmin", "reduce_switch", "lsweep_depth",
-                "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes")
+                "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes", "gemm_impl")
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
@@ -120,6 +120,10 @@ def load_library():
         "lhl_tune_set": ([vp, ctypes.c_char_p, i64], ci),
         "lhl_tune_get": ([vp, ctypes.c_char_p, ctypes.POINTER(i64)], ci),
         "lhl_applyZ_noalloc": ([vp, ci, vp, i64, i64, i64, ci, ci], ci),
+        "lhl_plan_create": ([vp, ctypes.POINTER(vp), i64, ci, vp, vp, i64, vp, i64, ci, cd, ci], ci),
+        "lhl_plan_run": ([vp, vp, vp, vp, vp], ci),
+        "lhl_plan_stats": ([vp, ctypes.POINTER(ci), ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)], ci),
+        "lhl_plan_destroy": ([vp], ci),
         "lhl_sync": ([vp], ci),
         "lhl_get_stream": ([vp, ctypes.POINTER(vp)], ci),
         "lhl_profile_enable": ([vp, ci], ci),
@@ -691,6 +695,80 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
         return X, info, iters
     _check(ws._h, code, (X, info, iters))
     return X, info, iters
+
+
+class BatchPlan:
+    """A batched shift + solve + refinement step of fixed shapes, replayed for
+    new shifts as one HIP graph (lhl_plan_create / lhl_plan_run).  B and X are
+    GPU tensors (float64, or complex128 for complex shifts; column-major) that
+    the plan reads and writes at each run; rhs_offsets as in
+    shift_solve_batched.  The refinement runs on the device (converged shifts
+    drop out of later rounds); the first run learns the round count, the
+    second captures the graph, later runs replay it."""
+
+    def __init__(self, ws: LHLWorkspace, rhs_offsets, B, X, shifts_complex=False, precision="refine64",
+                 tol: float = 1e-12, max_iter: int = 30):
+        offs = np.ascontiguousarray(np.asarray(rhs_offsets, dtype=np.int64))
+        if offs.ndim != 1 or offs.shape[0] < 2:
+            raise DimensionMismatch("rhs_offsets needs at least two entries")
+        if not (_is_torch(B) and _is_torch(X)):
+            raise ArgumentError("a plan's B and X are GPU tensors")
+        bv, xv = _Vec(B, "B"), _Vec(X, "X")
+        cplx = int(bool(shifts_complex))
+        if bv.code != LHL_F64 or xv.code != LHL_F64 or bv.complex != cplx or xv.complex != cplx:
+            raise ArgumentError("B and X must be float64 (complex128 for complex shifts)")
+        if bv.n != ws.n or xv.n != ws.n:
+            raise DimensionMismatch("B and X must have n rows")
+        if offs[0] < 0 or np.any(np.diff(offs) < 0):
+            raise DimensionMismatch("rhs_offsets must be non-negative and non-decreasing")
+        if offs[-1] > bv.ncol or offs[-1] > xv.ncol:
+            raise DimensionMismatch(f"rhs_offsets reach column {int(offs[-1])}, beyond B or X")
+        self.ws, self.B, self.X = ws, B, X
+        self.nshift = offs.shape[0] - 1
+        self.cplx = cplx
+        self._offs = offs
+        prec = LHL_PREC_REFINE64 if precision == "refine64" else LHL_PREC_WORKING
+        p = ctypes.c_void_p()
+        _check(ws._h, _lib.lhl_plan_create(ws._h, ctypes.byref(p), self.nshift, cplx, offs.ctypes.data,
+                                           ctypes.c_void_p(bv.ptr), bv.ld, ctypes.c_void_p(xv.ptr), xv.ld, prec,
+                                           float(tol), int(max_iter)))
+        self._p = p
+
+    def run(self, sigmas, taus, strict: bool = True):
+        """X for these shifts; returns (info, iters) as numpy arrays."""
+        sig = np.asarray(sigmas, dtype=np.complex128).reshape(-1)
+        tau = np.asarray(taus, dtype=np.complex128).reshape(-1)
+        if sig.shape != (self.nshift,) or tau.shape != (self.nshift,):
+            raise DimensionMismatch(f"the plan takes {self.nshift} shifts")
+        s2 = np.empty(2 * self.nshift)
+        t2 = np.empty(2 * self.nshift)
+        s2[0::2], s2[1::2] = sig.real, sig.imag
+        t2[0::2], t2[1::2] = tau.real, tau.imag
+        info = np.zeros(self.nshift, dtype=np.int64)
+        iters = np.zeros(self.nshift, dtype=np.int32)
+        code = _lib.lhl_plan_run(self._p, s2.ctypes.data, t2.ctypes.data, info.ctypes.data, iters.ctypes.data)
+        if code == LHL_NOT_CONVERGED and not strict:
+            return info, iters
+        _check(self.ws._h, code, (info, iters))
+        return info, iters
+
+    def stats(self):
+        """(rounds in the current graph or -1, replays, captures, rounds run beyond a graph)."""
+        r, a, b, c = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self.ws._h, _lib.lhl_plan_stats(self._p, ctypes.byref(r), ctypes.byref(a), ctypes.byref(b),
+                                               ctypes.byref(c)))
+        return int(r.value), int(a.value), int(b.value), int(c.value)
+
+    def close(self):
+        if getattr(self, "_p", None) is not None and _lib is not None:
+            _lib.lhl_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def profile_read_work(ws: LHLWorkspace, kernel: str):
