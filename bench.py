@@ -103,10 +103,11 @@ def config5(lhl_hip, torch, dev, tol, threads):
     """The uneven batch: 64 independent stiff J (the family above) with n drawn
     uniformly from [128, 3000], each with 4-32 real shifts (1, -gamma), gamma
     log-spaced in [1e-4, 1e-1], and 1-8 N(0,1) right-hand sides per shift
-    (seeded).  Per matrix: lhl(J), then one batched solve refined with FP64
-    residuals to `tol`.  Each matrix has its own handle (its own stream); the
-    matrices are handed out largest first to `threads` host threads, so the
-    small ones fill in beside the large ones on the GPU."""
+    (seeded).  Grouped: one lhl_reduce_group call (one launch per reduction
+    step for each size class), then each matrix's batched solve refined with
+    FP64 residuals to `tol`, the solves handed out largest first to `threads`
+    host threads (each matrix's handle has its own stream).  For comparison,
+    the same work with one lhl() per matrix on the host threads."""
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(505)
@@ -125,51 +126,69 @@ def config5(lhl_hip, torch, dev, tol, threads):
     order = sorted(range(M), key=lambda m: -(mats[m]["n"] ** 3 + mats[m]["S"] * mats[m]["n"] ** 2))
     ncol = int(sum(d["S"] * d["R"] for d in mats))
     res = {"workload": (f"{M} independent stiff J, n uniform in [128,3000] (sum n = {int(ns.sum())}, max "
-                        f"{int(ns.max())}), 4-32 real shifts x 1-8 RHS each ({ncol} solves in all); lhl() + one "
-                        f"batched solve refined to rel resid <= {tol:g} per matrix, one handle per matrix")}
+                        f"{int(ns.max())}), 4-32 real shifts x 1-8 RHS each ({ncol} solves in all); grouped "
+                        f"reductions (lhl_reduce_group) + one batched solve per matrix refined to rel resid <= "
+                        f"{tol:g}")}
+    Js = [d["J"] for d in mats]
     for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
         wss = [lhl_hip.LHLWorkspace(d["n"], dt) for d in mats]
         Xs = [torch.zeros_like(d["B"]) for d in mats]
         torch.cuda.synchronize()
         its = [None] * M
 
-        def run(m):
+        def solve(m):
             d = mats[m]
-            lhl_hip.lhl_(wss[m], d["J"])
             _, info, it = lhl_hip.shift_solve_batched(wss[m], np.ones(d["S"]), -d["gam"], d["B"], d["offs"],
                                                       X=Xs[m], precision="refine64", tol=tol, max_iter=30)
             assert np.all(info == 0)
             its[m] = it
 
-        def job(nt):
+        def lone(m):
+            lhl_hip.lhl_(wss[m], mats[m]["J"])
+            solve(m)
+
+        def job(grouped, nt):
+            if grouped:
+                lhl_hip.lhl_group_(wss, Js)
+            f = solve if grouped else lone
             if nt == 1:
                 for m in order:
-                    run(m)
+                    f(m)
             else:
                 with ThreadPoolExecutor(nt) as ex:
-                    list(ex.map(run, order))
+                    list(ex.map(f, order))
             for ws in wss:
                 lhl_hip.sync(ws)
 
-        job(threads)   # warm-up: the handles' buffers, rocBLAS, code objects
+        job(True, threads)   # warm-up: the handles' buffers, rocBLAS, code objects
+        job(False, threads)
         torch.cuda.synchronize()
         t = {}
-        for nt in (1, threads):
+        for key, grouped, nt in (("grouped", True, threads), ("grouped_1", True, 1), ("lone", False, threads)):
             t0 = time.perf_counter()
-            job(nt)
+            job(grouped, nt)
             torch.cuda.synchronize()
-            t[nt] = time.perf_counter() - t0
+            t[key] = time.perf_counter() - t0
+        # the reductions alone, grouped
+        t0 = time.perf_counter()
+        lhl_hip.lhl_group_(wss, Js)
+        torch.cuda.synchronize()
+        t_red = time.perf_counter() - t0
+        job(True, threads)       # leave the grouped results in Xs
+        torch.cuda.synchronize()
         rel = 0.0
         for m, d in enumerate(mats):
             g = torch.from_numpy(np.repeat(d["gam"], d["R"])).to(dev)
             Rd = d["B"] - (Xs[m] - (d["J"] @ Xs[m]) * g[None, :])
             rel = max(rel, (Rd.abs().amax(0) / d["B"].abs().amax(0)).max().item())
         itv = np.concatenate(its)
-        res[name] = {"ms": 1e3 * t[threads], "solves_per_s": ncol / t[threads], "host_threads": threads,
-                     "ms_one_thread": 1e3 * t[1], "max_rel_resid": rel,
+        res[name] = {"ms": 1e3 * t["grouped"], "solves_per_s": ncol / t["grouped"], "host_threads": threads,
+                     "grouped_reductions_ms": 1e3 * t_red, "ms_one_thread": 1e3 * t["grouped_1"],
+                     "per_matrix_lhl_ms": 1e3 * t["lone"], "max_rel_resid": rel,
                      "refine_rounds": [int(itv.min()), int(itv.max())]}
-        log(f"bench: config-5 {name}: {1e3 * t[threads]:.1f} ms on {threads} host threads "
-            f"({1e3 * t[1]:.1f} ms on one), {ncol / t[threads]:.0f} solves/s, max rel resid {rel:.2e}")
+        log(f"bench: config-5 {name}: {1e3 * t['grouped']:.1f} ms grouped ({1e3 * t_red:.1f} ms of it the 64 "
+            f"reductions; {1e3 * t['grouped_1']:.1f} ms with one host thread), {1e3 * t['lone']:.1f} ms with one "
+            f"lhl() per matrix; {ncol / t['grouped']:.0f} solves/s, max rel resid {rel:.2e}")
         del wss, Xs
     res["fp32_refine_over_fp64"] = res["fp32"]["solves_per_s"] / res["fp64"]["solves_per_s"]
     return res
@@ -287,18 +306,19 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU leg")
     ap.add_argument("--lhl-reps", type=int, default=2)
-    ap.add_argument("--config4", type=int, default=0,
+    ap.add_argument("--config4", type=int, default=1,
                     help="1 also times the n=4096 sweep of 256 complex (i*omega, -1) + 256 real shifts x 8 RHS")
     ap.add_argument("--config2", type=int, default=0,
                     help="1 also times the plain FP64 drop-in calls (lhl, lhl_shift!, lhl_ldiv!, lhl_refine!) "
                          "at n = 2000 and 4096")
-    ap.add_argument("--config5", type=int, default=0,
+    ap.add_argument("--config5", type=int, default=8,
                     help="T > 0 also times the uneven batch of 64 independent matrices on T host threads")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB of a plain device copy run at the end (the HBM-counter calibration for --pmc runs)")
     ap.add_argument("--pmc-workload", type=int, default=0,
                     help="K > 0: only lhl() once and K + 1 plain batched solves per precision, then exit (the "
                          "short program the rocprofv3 counter passes run)")
+    ap.add_argument("--plan", type=int, default=1, help="0 skips the graph-captured (plan) leg of the main step")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a tuning knob on the main workload's workspaces (repeatable)")
     args = ap.parse_args()
@@ -440,6 +460,34 @@ def main():
             results[name]["err_vs_fp64"] = [float(f"{v:.3g}") for v in e]
             log(f"bench: fp32+refine error vs fp64 per shift: max {e.max():.2e}, median {np.median(e):.2e}")
 
+    # ---- the same step as one graph launch (a plan): device-side refinement
+    # control, replayed for the same shifts (learning run and capture untimed)
+    plan_res = {}
+    if args.plan:
+        g = torch.from_numpy(np.repeat(gam, R)).to(dev)
+        for name, ws in (("fp64", ws64), ("fp32", ws32)):
+            Xp = torch.zeros_like(Bd)
+            plan = lhl_hip.BatchPlan(ws, offs, Bd, Xp, precision="refine64", tol=args.tol, max_iter=30)
+            for _ in range(max(args.warmup, 0) + 2):
+                plan.run(sig, tau)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                info, iters = plan.run(sig, tau)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / max(args.steps, 1)
+            Rd = Bd - (Xp - (Jd @ Xp) * g[None, :])
+            rel = (Rd.abs().amax(0) / Bd.abs().amax(0)).max().item()
+            gr, replays, captures, extra = plan.stats()
+            plan_res[name] = {"ms": 1e3 * dt, "solves_per_s": ncol / dt, "max_rel_resid": rel,
+                              "refine_rounds": [int(iters.min()), int(iters.max())], "graph_rounds": gr,
+                              "replays": replays, "captures": captures, "rounds_beyond_graph": extra}
+            log(f"bench: plan {name} step {1e3 * dt:.2f} ms, {ncol / dt:.0f} solves/s, max rel resid {rel:.2e}, "
+                f"graph rounds {gr}, replays {replays}")
+            plan.close()
+            del Xp
+        plan_res["fp32_refine_over_fp64"] = plan_res["fp32"]["solves_per_s"] / plan_res["fp64"]["solves_per_s"]
+
     # ---- optional: the complex + real shift sweep on one real reduction
     sweep4 = None
     if args.config4:
@@ -461,18 +509,29 @@ def main():
             lhl_hip.lhl_(ws4, A4d)
             prec = "refine64"   # both precisions refined to the same tolerance, as in the main step
 
-            def sweep():
-                _, ic, itc = lhl_hip.shift_solve_batched(ws4, 1j * om, -np.ones(S4), Bc, offs4, X=Xc, precision=prec,
-                                                         tol=args.tol, max_iter=30)
-                _, ir, itr = lhl_hip.shift_solve_batched(ws4, np.ones(S4), -gam4, Br, offs4, X=Xr, precision=prec,
-                                                         tol=args.tol, max_iter=30)
+            # the sweep as two graph launches: plans for the complex and the
+            # real shifts (learning run and capture untimed)
+            pc = lhl_hip.BatchPlan(ws4, offs4, Bc, Xc, shifts_complex=True, precision=prec, tol=args.tol,
+                                   max_iter=30)
+            pr = lhl_hip.BatchPlan(ws4, offs4, Br, Xr, precision=prec, tol=args.tol, max_iter=30)
+
+            def sweep_plan():
+                _, itc = pc.run(1j * om, -np.ones(S4))
+                _, itr = pr.run(np.ones(S4), -gam4)
                 return np.concatenate([itc, itr])
-            sweep()
+            for _ in range(3):
+                sweep_plan()
             torch.cuda.synchronize()
+            reps4 = 3
             t0 = time.perf_counter()
-            its = sweep()
+            for _ in range(reps4):
+                its = sweep_plan()
             torch.cuda.synchronize()
-            dt4 = time.perf_counter() - t0
+            dt4 = (time.perf_counter() - t0) / reps4
+            plan_stats = {"complex": pc.stats(), "real": pr.stats()}
+            pc.close()   # the plans' factor pools go before the eager comparison allocates its own
+            pr.close()
+            torch.cuda.synchronize()
             # accuracy on the device: r = b - (sigma x + tau A x)
             sig = torch.from_numpy(np.repeat(1j * om, R4)).to(dev)
             Rc = Bc - (Xc * sig[None, :] - A4d.to(torch.complex128) @ Xc)
@@ -480,9 +539,27 @@ def main():
             g4 = torch.from_numpy(np.repeat(gam4, R4)).to(dev)
             Rr = Br - (Xr - (A4d @ Xr) * g4[None, :])
             relr = (Rr.abs().amax(0) / Br.abs().amax(0)).max().item()
+            # the same sweep through the eager batched call, for comparison
+            Xc2, Xr2 = torch.zeros_like(Bc), torch.zeros_like(Br)
+
+            def sweep():
+                lhl_hip.shift_solve_batched(ws4, 1j * om, -np.ones(S4), Bc, offs4, X=Xc2, precision=prec,
+                                            tol=args.tol, max_iter=30)
+                lhl_hip.shift_solve_batched(ws4, np.ones(S4), -gam4, Br, offs4, X=Xr2, precision=prec, tol=args.tol,
+                                            max_iter=30)
+            sweep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sweep()
+            torch.cuda.synchronize()
+            dt_eager = time.perf_counter() - t0
+            del Xc2, Xr2
             sweep4[name] = {"solves_per_s": 2 * S4 * R4 / dt4, "ms": 1e3 * dt4, "max_rel_resid": max(relc, relr),
-                            "refine_rounds": [int(its.min()), int(its.max())]}
-            log(f"bench: config-4 sweep {name}: {1e3 * dt4:.1f} ms, {2 * S4 * R4 / dt4:.0f} solves/s, "
+                            "refine_rounds": [int(its.min()), int(its.max())], "path": "plans (HIP graphs)",
+                            "eager_ms": 1e3 * dt_eager,
+                            "plan_stats_graph_rounds_replays_captures_extra": plan_stats}
+            log(f"bench: config-4 sweep {name}: {1e3 * dt4:.1f} ms as two graph launches "
+                f"({1e3 * dt_eager:.1f} ms eager), {2 * S4 * R4 / dt4:.0f} solves/s, "
                 f"max rel resid {max(relc, relr):.2e}")
             del ws4
 
@@ -609,6 +686,8 @@ def main():
                            for k, v in kernels["fp32"].items()},
         "cpu_baseline": cpu,
     }
+    if plan_res:
+        out["plan"] = plan_res
     if sweep4:
         out["config4_sweep"] = sweep4
     if sweep2:

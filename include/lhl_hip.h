@@ -135,6 +135,17 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
  * column above tol after max_iter rounds.  rhs_offsets must start at 0 or
  * above and never decrease (LHL_ERR_DIM otherwise). */
 
+/* Many reductions at once: lhl_reduce on each of hs[0..count-1] (distinct
+ * handles on one device, each with its own J[m], n[m], ldj[m]; one J element
+ * type and location for all).  No counterpart in the reference as one name:
+ * it serves batches of independent matrices of uneven n.  The matrices the
+ * row-resident step loop takes alone are grouped by element type and size
+ * class, and each group runs one launch per step for all of its matrices
+ * (instead of one per step per matrix); the results are those of lhl_reduce,
+ * bit for bit.  An error names the first handle it concerns. */
+int lhl_reduce_group(int64_t count, lhl_handle *const *hs, const void *const *J, const int64_t *n,
+                     const int64_t *ldj, int j_dtype, int balance, int j_on_device);
+
 /* Plans: one batched step (lhl_shift_solve_batched's work for fixed shapes)
  * captured in a HIP graph and replayed for new shifts.  No counterpart in the
  * reference: it is the loop of README.md:47-51 as one graph launch, for
@@ -236,6 +247,8 @@ int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double
  * gemm_impl            the solves' dense products (L sweeps, J x):      0
  *                      0 rocBLAS, 1 the library's MFMA kernels (a
  *                      plan's masked refinement rounds always use them)
+ * group_bytes          lhl_reduce_group: row-major bytes per sub-group   256 MiB
+ *                      (each size class reads its first handle's knob)
  * A value of -1 restores the default; lhl_tune_get reports -1 for a knob at
  * its default.  Results do not depend on the knobs beyond rounding (the
  * pivots of a reduction match for every panel setting). */

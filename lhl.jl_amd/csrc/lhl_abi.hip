@@ -175,7 +175,7 @@ int lhl_destroy(lhl_handle *h) {
     blas_release(h);
     h->cache.clear();
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
-                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->brec, &h->bpb, &h->bact, &h->blc, &h->bu12, &h->Linv, &h->Lfull, &h->Lp1, &h->ybuf, &h->rbuf, &h->gbuf,
+                      &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->brec, &h->bpb, &h->bact, &h->blc, &h->bu12, &h->grp, &h->Linv, &h->Lfull, &h->Lp1, &h->ybuf, &h->rbuf, &h->gbuf,
                       &h->xstage, &h->bstage})
         b->release();
     auto drop = [](lhl_shift_state *s) {
@@ -249,56 +249,52 @@ static Status upload_pipelined(lhl_handle *h, void *dst, size_t dpitch, const vo
     return {};
 }
 
-int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int balance, int j_on_device) {
-    if (!h) return LHL_ERR_ARG;
-    host_keep_reset(h);
-    if (n < 0) return fail(h, LHL_ERR_DIM, "lhl_reduce: negative size");
-    if (j_dtype != LHL_F64 && j_dtype != LHL_F32) return fail(h, LHL_ERR_ARG, "lhl_reduce: J must be Float64 or Float32");
-    if (n > 0 && (!J || ldj < n)) return fail(h, LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n");
-    if (n > 8192 + 8192) return fail(h, LHL_ERR_ARG, "lhl_reduce: n above 16384 is not supported");
-    Status st = set_device(h);
-    if (!st.ok()) return fail(h, st);
-    if (n != h->n) {
-        st = alloc_sized(h, n);
-        if (!st.ok()) return fail(h, st);
-    }
+// Checks, (re)sizing and the upload of J for a reduction (lhl_reduce and
+// lhl_reduce_group): the handle's resident J is FP64 unless J is FP32 on an
+// FP32 handle.  Leaves the handle unreduced.
+static Status reduce_prepare(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int j_on_device) {
+    if (n < 0) return {LHL_ERR_DIM, "lhl_reduce: negative size"};
+    if (j_dtype != LHL_F64 && j_dtype != LHL_F32) return {LHL_ERR_ARG, "lhl_reduce: J must be Float64 or Float32"};
+    if (n > 0 && (!J || ldj < n)) return {LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n"};
+    if (n > 8192 + 8192) return {LHL_ERR_ARG, "lhl_reduce: n above 16384 is not supported"};
+    LHL_TRY(set_device(h));
+    if (n != h->n) LHL_TRY(alloc_sized(h, n));
     h->reduced = 0;
     h->own->loaded = 0;
-    // the resident J: FP64 unless J is FP32 on an FP32 handle
     const int rdt = (j_dtype == LHL_F32 && h->dtype == LHL_F32) ? LHL_F32 : LHL_F64;
     h->j_dtype = rdt;
     h->ldj = pad_ld(n, rdt);
-    st = h->J.ensure(elt_size(rdt) * (size_t)h->ldj * (size_t)std::max<int64_t>(n, 1));
-    if (!st.ok()) return fail(h, st);
-    if (n > 0) {
-        if (j_dtype == rdt) {
-            const size_t es = elt_size(rdt);
-            if (j_on_device) {
-                hipError_t e = hipMemcpy2DAsync(h->J.p, es * h->ldj, J, es * ldj, es * n, n, hipMemcpyDeviceToDevice,
-                                                h->stream);
-                if (e != hipSuccess)
-                    return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: copy of J: ") + hipGetErrorString(e));
-            } else {
-                // a host J: staged in 16 MiB chunks by default (the tuning knob
-                // h2d_chunk_bytes; 0 leaves the copy to the runtime)
-                const size_t chunk = h->tune_h2d_chunk >= 0 ? (size_t)h->tune_h2d_chunk : (16u << 20);
-                st = upload_pipelined(h, h->J.p, es * h->ldj, J, es * ldj, es * n, n, chunk);
-                if (!st.ok()) {
-                    (void)hipStreamSynchronize(h->stream);
-                    return fail(h, st);
-                }
-            }
+    LHL_TRY(h->J.ensure(elt_size(rdt) * (size_t)h->ldj * (size_t)std::max<int64_t>(n, 1)));
+    if (n == 0) return {};
+    if (j_dtype == rdt) {
+        const size_t es = elt_size(rdt);
+        if (j_on_device) {
+            LHL_HIP_TRY(hipMemcpy2DAsync(h->J.p, es * h->ldj, J, es * ldj, es * n, n, hipMemcpyDeviceToDevice,
+                                         h->stream));
         } else {
-            int64_t ld, plane;
-            st = stage_in(h, h->xstage, J, n, n, ldj, j_dtype, 0, j_on_device, rdt, ld, plane);
-            if (st.ok()) {
-                hipError_t e = hipMemcpy2DAsync(h->J.p, 8 * h->ldj, h->xstage.p, 8 * ld, 8 * n, n,
-                                                hipMemcpyDeviceToDevice, h->stream);
-                if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
+            // a host J: staged in 16 MiB chunks by default (the tuning knob
+            // h2d_chunk_bytes; 0 leaves the copy to the runtime)
+            const size_t chunk = h->tune_h2d_chunk >= 0 ? (size_t)h->tune_h2d_chunk : (16u << 20);
+            Status st = upload_pipelined(h, h->J.p, es * h->ldj, J, es * ldj, es * n, n, chunk);
+            if (!st.ok()) {
+                (void)hipStreamSynchronize(h->stream);
+                return st;
             }
-            if (!st.ok()) return fail(h, st);
         }
+    } else {
+        int64_t ld, plane;
+        LHL_TRY(stage_in(h, h->xstage, J, n, n, ldj, j_dtype, 0, j_on_device, rdt, ld, plane));
+        LHL_HIP_TRY(hipMemcpy2DAsync(h->J.p, 8 * h->ldj, h->xstage.p, 8 * ld, 8 * n, n, hipMemcpyDeviceToDevice,
+                                     h->stream));
     }
+    return {};
+}
+
+int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int balance, int j_on_device) {
+    if (!h) return LHL_ERR_ARG;
+    host_keep_reset(h);
+    Status st = reduce_prepare(h, J, n, ldj, j_dtype, j_on_device);
+    if (!st.ok()) return fail(h, st);
     st = reduce_device(h, balance);
     if (!st.ok()) {
         (void)hipStreamSynchronize(h->stream);   // the upload may still read the pinned staging
@@ -307,6 +303,39 @@ int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(h, LHL_ERR_HIP, std::string("lhl_reduce: ") + hipGetErrorString(e));
     h->reduced = 1;
+    return LHL_OK;
+}
+
+int lhl_reduce_group(int64_t count, lhl_handle *const *hs, const void *const *J, const int64_t *n,
+                     const int64_t *ldj, int j_dtype, int balance, int j_on_device) {
+    if (count < 0 || (count > 0 && (!hs || !J || !n || !ldj))) return LHL_ERR_ARG;
+    if (count == 0) return LHL_OK;
+    for (int64_t m = 0; m < count; m++) {
+        if (!hs[m]) return LHL_ERR_ARG;
+        for (int64_t q = 0; q < m; q++)
+            if (hs[q] == hs[m]) return fail(hs[m], LHL_ERR_ARG, "lhl_reduce_group: a handle appears twice");
+        if (hs[m]->device != hs[0]->device)
+            return fail(hs[m], LHL_ERR_ARG, "lhl_reduce_group: the handles must be on one device");
+    }
+    for (int64_t m = 0; m < count; m++) {
+        host_keep_reset(hs[m]);
+        hs[m]->err.clear();   // an error below is reported on the handle it concerns
+    }
+    Status st;
+    int64_t bad = -1;
+    for (int64_t m = 0; m < count && st.ok(); m++) {
+        st = reduce_prepare(hs[m], J[m], n[m], ldj[m], j_dtype, j_on_device);
+        if (!st.ok()) bad = m;
+    }
+    if (st.ok()) st = reduce_group_device(hs, count, balance);
+    hipError_t e = hipSuccess;
+    for (int64_t m = 0; m < count; m++) {
+        const hipError_t e2 = hipStreamSynchronize(hs[m]->stream);
+        if (e == hipSuccess) e = e2;
+    }
+    if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, std::string("lhl_reduce_group: ") + hipGetErrorString(e)};
+    if (!st.ok()) return fail(hs[bad >= 0 ? bad : 0], st);
+    for (int64_t m = 0; m < count; m++) hs[m]->reduced = 1;
     return LHL_OK;
 }
 
@@ -801,13 +830,20 @@ struct lhl_plan {
     int graph_rounds = -1;                // rounds in the graph that `exec` holds
     std::vector<const void *> sig;        // the handle buffers the graph reads, at capture
     int no_graph = 0;                     // capture failed: run round by round
+    int blas_ok = 1;                      // rocBLAS products may go into the graph
+    // per refinement round: the fraction of shifts still active when it began
+    // (learned round by round); rounds with at least half run their dense
+    // products over every column through rocBLAS, the others masked
+    std::vector<double> act;
     int64_t replays = 0, captures = 0, extra_rounds = 0;
 };
 
 namespace {
 
 std::vector<const void *> plan_signature(const lhl_handle *h) {
-    return {h->A.p, h->Ht.p, h->J.p, h->Lfull.p, h->perm.p, h->scale64.p, h->iscale64.p, (const void *)h->stream};
+    return {h->A.p,        h->Ht.p,        h->J.p,          h->Lfull.p,
+            h->perm.p,     h->scale64.p,   h->iscale64.p,   (const void *)h->stream,
+            (const void *)h->stream2, (const void *)h->ev_fork, (const void *)h->ev_join};
 }
 
 void plan_free_graph(lhl_plan *p) {
@@ -816,6 +852,13 @@ void plan_free_graph(lhl_plan *p) {
     p->exec = nullptr;
     p->graph = nullptr;
     p->graph_rounds = -1;
+}
+
+// whether round `it` (0: the first pass and its residual) runs full-width products
+int plan_full(const lhl_plan *p, int it, bool capturing) {
+    if (capturing && !p->blas_ok) return 0;
+    if (it == 0) return 1;
+    return (size_t)it < p->act.size() && p->act[it] >= 0.5 ? 1 : 0;
 }
 
 SolveArgs plan_solve_args(lhl_plan *p) {
@@ -870,8 +913,9 @@ ResidArgs plan_resid_args(lhl_plan *p) {
 
 // uploads, staging, shift LUs, the first pass, and (refining) the first
 // residual and convergence step
-Status plan_head(lhl_plan *p) {
+Status plan_head(lhl_plan *p, bool capturing) {
     lhl_handle *h = p->h;
+    const int full = plan_full(p, 0, capturing);
     LHL_HIP_TRY(hipMemcpyAsync(p->shiftpar.p, p->p_shift, 32 * (size_t)p->nshift, hipMemcpyHostToDevice, h->stream));
     if (p->precision == LHL_PREC_REFINE64)
         LHL_HIP_TRY(hipMemcpyAsync(p->colshift.p, p->p_col, 32 * (size_t)std::max<int64_t>(p->nc, 1),
@@ -883,17 +927,41 @@ Status plan_head(lhl_plan *p) {
     LHL_TRY(fill_i32_device(h, p->cact.as<int>(), p->nc, 1));
     LHL_TRY(fill_i32_device(h, p->sact.as<int>(), p->nshift, 1));
     LHL_TRY(fill_i32_device(h, p->iters.as<int>(), p->nshift, 0));
-    h->dev_shiftpar = p->shiftpar.as<double>();
-    Status st = shift_pool_device(h, p->pool, p->nshift, nullptr, nullptr, p->cplx);
-    h->dev_shiftpar = nullptr;
-    LHL_TRY(st);
+    // the first pass's gather and L sweep read no shift state: they run on the
+    // second stream beside the shift LUs (a fork and a join in the graph)
     SolveArgs a = plan_solve_args(p);
     a.xin = p->bst.p;
     a.xout = p->xst.p;
     a.accumulate = 0;
+    a.full_products = full;
+    LHL_HIP_TRY(hipEventRecord(h->ev_fork, h->stream));
+    LHL_HIP_TRY(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+    {
+        SolveArgs p1 = a;
+        p1.phase = 1;
+        hipStream_t keep = h->stream;
+        h->stream = h->stream2;
+        Status s1 = solve_device(h, p1);
+        h->stream = keep;
+        LHL_TRY(s1);
+        LHL_HIP_TRY(hipEventRecord(h->ev_join, h->stream2));
+    }
+    h->dev_shiftpar = p->shiftpar.as<double>();
+    Status st = shift_pool_device(h, p->pool, p->nshift, nullptr, nullptr, p->cplx);
+    h->dev_shiftpar = nullptr;
+    LHL_HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_join, 0));   // joins the second stream even on error
+    LHL_TRY(st);
+    a = plan_solve_args(p);   // the pool's pointers, now set
+    a.xin = p->bst.p;
+    a.xout = p->xst.p;
+    a.accumulate = 0;
+    a.full_products = full;
+    a.phase = 2;
     LHL_TRY(solve_device(h, a));
     if (p->precision == LHL_PREC_REFINE64) {
-        LHL_TRY(residual_device(h, plan_resid_args(p)));
+        ResidArgs ra = plan_resid_args(p);
+        ra.full_products = full;
+        LHL_TRY(residual_device(h, ra));
         LHL_TRY(conv_device(h, 0, p->max_iter, p->tol, p->nshift, p->cbd.as<int64_t>(), p->norms.as<double>(), p->nc,
                             p->sact.as<int>(), p->cact.as<int>(), p->iters.as<int>()));
     }
@@ -902,14 +970,18 @@ Status plan_head(lhl_plan *p) {
 
 // refinement round `it` (1-based): correct the columns still active with the
 // last residual, then the new residual and convergence step
-Status plan_round(lhl_plan *p, int it) {
+Status plan_round(lhl_plan *p, int it, bool capturing) {
     lhl_handle *h = p->h;
+    const int full = capturing ? plan_full(p, it, true) : 0;
     SolveArgs a = plan_solve_args(p);
     a.xin = p->rst.p;
     a.xout = p->xst.p;
     a.accumulate = 1;
+    a.full_products = full;
     LHL_TRY(solve_device(h, a));
-    LHL_TRY(residual_device(h, plan_resid_args(p)));
+    ResidArgs ra = plan_resid_args(p);
+    ra.full_products = full;
+    LHL_TRY(residual_device(h, ra));
     LHL_TRY(conv_device(h, it, p->max_iter, p->tol, p->nshift, p->cbd.as<int64_t>(), p->norms.as<double>(), p->nc,
                         p->sact.as<int>(), p->cact.as<int>(), p->iters.as<int>()));
     return {};
@@ -949,8 +1021,8 @@ Status plan_capture(lhl_plan *p, int rounds) {
         h->prof.on = prof;
         return {LHL_ERR_HIP, std::string("plan: begin capture: ") + hipGetErrorString(e)};
     }
-    Status st = plan_head(p);
-    for (int it = 1; st.ok() && p->precision == LHL_PREC_REFINE64 && it <= rounds; it++) st = plan_round(p, it);
+    Status st = plan_head(p, true);
+    for (int it = 1; st.ok() && p->precision == LHL_PREC_REFINE64 && it <= rounds; it++) st = plan_round(p, it, true);
     if (st.ok()) st = plan_tail(p);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(h->stream, &g);
@@ -1046,6 +1118,7 @@ int lhl_plan_create(lhl_handle *h, lhl_plan **out, int64_t nshift, int shifts_co
         if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
     if (st.ok()) st = groups_device(h, nshift, p->cb.data(), planes, p->groups, &p->ngroups);
+    if (st.ok()) st = ensure_stream2(h);
     if (st.ok()) {
         const size_t bytes = 32 * (size_t)nshift + 32 * (size_t)nc + 3 * sizeof(int) * (size_t)nshift + 256;
         hipError_t e = hipHostMalloc(&p->pin, bytes, hipHostMallocDefault);
@@ -1095,6 +1168,10 @@ int lhl_plan_run(lhl_plan *p, const double *sigma, const double *tau, int64_t *i
     if (p->rounds >= 0 && !p->no_graph) {
         if (!p->exec || p->graph_rounds != p->rounds || p->sig != plan_signature(h)) {
             Status cs = plan_capture(p, p->rounds);
+            if (!cs.ok() && p->blas_ok) {   // without the library's rocBLAS products
+                p->blas_ok = 0;
+                cs = plan_capture(p, p->rounds);
+            }
             if (!cs.ok()) p->no_graph = 1;   // run round by round from now on
         }
         if (p->exec) {
@@ -1106,9 +1183,12 @@ int lhl_plan_run(lhl_plan *p, const double *sigma, const double *tau, int64_t *i
         }
     }
     if (!graphed) {
-        st = plan_head(p);
+        st = plan_head(p, false);
         if (st.ok() && refine) st = plan_flags(p);
     }
+    // learning the active fraction per round (only when running round by round)
+    const bool learn = !graphed;
+    if (learn) p->act.assign(1, 1.0);
     hipError_t e = st.ok() ? hipStreamSynchronize(h->stream) : hipSuccess;
     if (st.ok() && e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     // rounds beyond the graph's (or every round, learning): one at a time
@@ -1116,7 +1196,12 @@ int lhl_plan_run(lhl_plan *p, const double *sigma, const double *tau, int64_t *i
     while (st.ok() && refine && done < p->max_iter && plan_any_active(p)) {
         done++;
         extra = true;
-        st = plan_round(p, done);
+        if (learn) {
+            int64_t na = 0;
+            for (int64_t s = 0; s < p->nshift; s++) na += p->p_sact[s] != 0;
+            p->act.push_back((double)na / (double)p->nshift);
+        }
+        st = plan_round(p, done, false);
         if (st.ok()) st = plan_flags(p);
         if (st.ok()) {
             e = hipStreamSynchronize(h->stream);
@@ -1400,11 +1485,13 @@ const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
     {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34}, {"gemm_impl", 0, 1},
+    {"group_bytes", 1, INT64_MAX},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;
     if (k == "shift_wide_min") return &h->tune_shift_wide;
     if (k == "shift_pool_bytes") return &h->tune_pool_bytes;
+    if (k == "group_bytes") return &h->tune_group_bytes;
     return nullptr;
 }
 }  // namespace

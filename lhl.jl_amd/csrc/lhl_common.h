@@ -368,6 +368,7 @@ struct lhl_handle {
     // blocked reduction: per-row panel records (ping-pong), side slots of pushed
     // columns, active rows (two halves), panel multipliers by logical row, U12
     lhl::DevBuf brec, bpb, bact, blc, bu12;
+    lhl::DevBuf grp;            // a grouped reduction's per-matrix table (this handle leading the group)
     // tuning knobs (0 = the built-in default): the n from which the blocked
     // reduction runs, its panel width and the fewest rows per step workgroup
     int64_t tune_blocked_min = 0;
@@ -378,6 +379,7 @@ struct lhl_handle {
     int64_t tune_pool_bytes = 0;   // bound on one chunk's packed U's in a batched solve (0: default)
     int64_t tune_h2d_chunk = -1;   // bytes per staged chunk of a host J's upload (-1: default, 0: runtime copy)
     int tune_gemm = -1;            // the solves' dense products: 0 rocBLAS, 1 the library's MFMA kernels (-1: default)
+    int64_t tune_group_bytes = 0;  // a grouped reduction's sub-group budget of row-major bytes (0: default)
     int lp1_ready = 0;             // Lp1 holds I + L' of the current reduction (the rocBLAS sweep's operand)
     const double *dev_shiftpar = nullptr;   // while set: the batched shift LUs read (sr, si, tr, ti) from here
     lhl::BlockCache cache;          // device blocks of the batched solves, reused between calls
@@ -451,6 +453,8 @@ inline void host_keep_reset(lhl_handle *h) {
 
 // ---- launchers implemented in the translation units ----
 Status reduce_device(lhl_handle *h, int balance);          // lhl_reduce.hip
+// many handles' reductions, grouped by size class (lhl_reduce.hip)
+Status reduce_group_device(lhl_handle *const *hs, int64_t count, int balance);
 // the steps of the blocked reduction on rowA (pitch ldb); nb 8 or 16
 // from step kswitch (rounded up to a panel boundary) the caller's row-resident
 // loop takes over: *kstart receives that step (n - 2 when the panels finished)

@@ -25,11 +25,13 @@
 // Workgroup placement: the 8 XCDs receive workgroups round-robin (blockIdx %
 // 8), each with its own L2.  Row panels are dealt to the XCDs so that every
 // workgroup of one row panel (which all read the same A panel) runs on one
-// XCD, and for a triangular A the panels of each XCD pair long with short
-// (panel x, 15 - x, 16 + x, 31 - x, ...), balancing the XCDs' work.
+// XCD.  For a triangular A a workgroup takes two row panels, the long
+// nrt - 1 - p and then the short p: every workgroup then has the same work.
 #include "lhl_common.h"
 
 #include "lhl_mfma.h"
+
+#include <type_traits>
 
 namespace lhl {
 namespace {
@@ -54,18 +56,23 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
     using M = Mfma<T>;
     constexpr bool LOWER = MODE != 0;
 
-    // tile of this workgroup
+    // tiles of this workgroup: one row panel, or for a triangular A the pair
+    // (nrt - 1 - pp, pp), whose lengths add up to the same for every pair
     const int x = blockIdx.x & 7, s = blockIdx.x >> 3;
     const int u = s / nct, ct = s - u * nct;
-    int rt;
+    int rts[2], npan;
     if (LOWER) {
-        const int p = u >> 1;
-        rt = (u & 1) ? 16 * p + 15 - x : 16 * p + x;
+        const int pp = 8 * u + x;
+        if (pp >= (nrt + 1) / 2) return;
+        rts[0] = nrt - 1 - pp;
+        rts[1] = pp;
+        npan = rts[0] != rts[1] ? 2 : 1;
     } else {
-        rt = 8 * u + x;
+        rts[0] = rts[1] = 8 * u + x;
+        if (rts[0] >= nrt) return;
+        npan = 1;
     }
-    if (rt >= nrt) return;
-    const int i0 = rt * kBM, j0 = ct * kBN;
+    const int j0 = ct * kBN;
     const int tid = threadIdx.x;
     if (cact) {
         int any = 0;
@@ -75,6 +82,9 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
         }
         if (!__syncthreads_or(any)) return;
     }
+    for (int pi = 0; pi < npan; pi++) {
+    const int rt = rts[pi];
+    const int i0 = rt * kBM;
     const int kend = LOWER ? min(K, i0 + kBM) : K;
     const int nch = (kend + BK - 1) / BK;
 
@@ -83,24 +93,27 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
     const int bj = tid / BTC, bk = (tid % BTC) * BE;
     const bool arow_full = i0 + kBM <= m;
     const bool bcol_ok = j0 + bj < q;
-    T ra[AE], rb[BE];
-    auto load = [&](int k0) {
+    // two register sets: chunk c + 2's loads are issued while chunk c is
+    // multiplied and chunk c + 1 (loaded one chunk earlier) goes to LDS
+    T ra[2][AE], rb[2][BE];
+    auto load = [&](auto par, int k0) {
+        constexpr int P = decltype(par)::value;
         const int ka = k0 + ak;
         const T *ap = A + (int64_t)ka * lda + i0 + ai;
         if (ka < kend && arow_full) {
 #pragma unroll
             for (int v = 0; v < AE; v += 4) {
                 const V4<T> t4 = ld4u(ap + v);
-                ra[v] = t4[0]; ra[v + 1] = t4[1]; ra[v + 2] = t4[2]; ra[v + 3] = t4[3];
+                ra[P][v] = t4[0]; ra[P][v + 1] = t4[1]; ra[P][v + 2] = t4[2]; ra[P][v + 3] = t4[3];
             }
         } else {
 #pragma unroll
-            for (int v = 0; v < AE; v++) ra[v] = (ka < kend && i0 + ai + v < m) ? ap[v] : (T)0;
+            for (int v = 0; v < AE; v++) ra[P][v] = (ka < kend && i0 + ai + v < m) ? ap[v] : (T)0;
         }
         if (MODE == 2 && ka >= i0 + ai) {   // keep the strict lower triangle only
 #pragma unroll
             for (int v = 0; v < AE; v++)
-                if (i0 + ai + v <= ka) ra[v] = (T)0;
+                if (i0 + ai + v <= ka) ra[P][v] = (T)0;
         }
         const int kb = k0 + bk;
         const T *bp = B + (int64_t)(j0 + bj) * ldb + kb;
@@ -108,18 +121,19 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
 #pragma unroll
             for (int v = 0; v < BE; v += 4) {
                 const V4<T> t4 = ld4u(bp + v);
-                rb[v] = t4[0]; rb[v + 1] = t4[1]; rb[v + 2] = t4[2]; rb[v + 3] = t4[3];
+                rb[P][v] = t4[0]; rb[P][v + 1] = t4[1]; rb[P][v + 2] = t4[2]; rb[P][v + 3] = t4[3];
             }
         } else {
 #pragma unroll
-            for (int v = 0; v < BE; v++) rb[v] = (bcol_ok && kb + v < kend) ? bp[v] : (T)0;
+            for (int v = 0; v < BE; v++) rb[P][v] = (bcol_ok && kb + v < kend) ? bp[v] : (T)0;
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](auto par) {
+        constexpr int P = decltype(par)::value;
 #pragma unroll
-        for (int v = 0; v < AE; v++) As[buf][ak][ai + v] = ra[v];
+        for (int v = 0; v < AE; v++) As[P][ak][ai + v] = ra[P][v];
 #pragma unroll
-        for (int v = 0; v < BE; v++) Bs[buf][bk + v][bj] = rb[v];
+        for (int v = 0; v < BE; v++) Bs[P][bk + v][bj] = rb[P][v];
     };
 
     typename M::acc_t acc[4][2];
@@ -130,28 +144,37 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
     const int lane = tid & 63, w = tid >> 6;
     const int wr = (w & 1) * 64, wc = (w >> 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
 
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int c = 0; c < nch; c++) {
-        const int cur = c & 1;
-        if (c + 1 < nch) load((c + 1) * BK);
+    // chunk c sits in LDS buffer c & 1 and register set c & 1
+    auto body = [&](auto par, int c) {
+        constexpr int P = decltype(par)::value;
+        using PN = std::integral_constant<int, 1 - P>;
+        if (c + 2 < nch) load(par, (c + 2) * BK);
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             const int k = k4 + lk;
             T a[4], b[2];
 #pragma unroll
-            for (int bi = 0; bi < 4; bi++) a[bi] = As[cur][k][wr + 16 * bi + li];
+            for (int bi = 0; bi < 4; bi++) a[bi] = As[P][k][wr + 16 * bi + li];
 #pragma unroll
-            for (int bjj = 0; bjj < 2; bjj++) b[bjj] = Bs[cur][k][wc + 16 * bjj + li];
+            for (int bjj = 0; bjj < 2; bjj++) b[bjj] = Bs[P][k][wc + 16 * bjj + li];
 #pragma unroll
             for (int bi = 0; bi < 4; bi++)
 #pragma unroll
                 for (int bjj = 0; bjj < 2; bjj++) acc[bi][bjj] = M::mac(a[bi], b[bjj], acc[bi][bjj]);
         }
-        if (c + 1 < nch) store(cur ^ 1);
-        __syncthreads();
+        if (c + 1 < nch) store(PN{});
+        lds_barrier();   // LDS only: chunk c + 2's loads stay in flight
+    };
+    load(P0{}, 0);
+    store(P0{});
+    if (nch > 1) load(P1{}, BK);
+    lds_barrier();
+    for (int c = 0; c < nch; c += 2) {
+        body(P0{}, c);
+        if (c + 1 < nch) body(P1{}, c + 1);
     }
 #pragma unroll
     for (int bi = 0; bi < 4; bi++)
@@ -169,6 +192,7 @@ __global__ void __launch_bounds__(256) k_mgemm(int m, int q, int K, const T *__r
                 }
             }
         }
+    }   // row panels
 }
 
 template <typename T>
@@ -178,9 +202,9 @@ Status mgemm_t(lhl_handle *h, const char *name, int mode, int m, int q, int K, c
     if (mode == 2 && K != m) return {LHL_ERR_ARG, "mgemm: the unit lower product needs a square A"};
     const bool lower = mode != 0;
     const int nrt = (m + kBM - 1) / kBM, nct = (q + kBN - 1) / kBN;
-    // rows per XCD: ceil(nrt / 8) panels (the triangular pairing needs them in
-    // pairs of 16); every XCD gets the same slot count, idle slots return
-    const int per = lower ? 2 * ((nrt + 15) / 16) : (nrt + 7) / 8;
+    // row panels (pairs of them for a triangular A) per XCD; every XCD gets
+    // the same slot count, idle slots return
+    const int per = lower ? (((nrt + 1) / 2) + 7) / 8 : (nrt + 7) / 8;
     const unsigned grid = 8u * (unsigned)per * (unsigned)nct;
     if (mode == 2)
         LHL_LAUNCH(h, name, (k_mgemm<T, 2>), dim3(grid), dim3(256), 0, m, q, K, A, lda, B, ldb, C, ldc, cact,

@@ -20,6 +20,10 @@
 //  * Ht = H^T (src:2024-2039) by an LDS-tiled transpose.
 #include "lhl_common.h"
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include <climits>
 #include <cstdint>
 #include <cstdlib>
@@ -546,11 +550,12 @@ __global__ void __launch_bounds__(256) k_rows_bridge(const T *__restrict__ B, in
 // columns k, k+1 of the rows, column k in logical order) are issued before the
 // decision, so one step costs about three memory round trips plus its
 // streaming.
+// (The body of the step; `bx` is the workgroup's row block.)
 template <typename T, int RPW, int U, int PASSES>
-__global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
+__device__ __forceinline__ void rows_step_body(
     T *__restrict__ B, int64_t ldb, int n, int k, const RowCand<T> *__restrict__ cin, RowCand<T> *__restrict__ cout,
     int ncand, const RowNext<T> *__restrict__ rin, RowNext<T> *__restrict__ rout, const T *__restrict__ gin,
-    T *__restrict__ gout, int *__restrict__ ilog, int *__restrict__ ipiv) {
+    T *__restrict__ gout, int *__restrict__ ilog, int *__restrict__ ipiv, const int bx) {
 #pragma clang fp contract(off)
     constexpr int RW = RPW * PASSES;            // rows per wave
     constexpr int R = kRowWaves * RW;
@@ -572,7 +577,7 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
     T ck[RW], k1[RW];
 #pragma unroll
     for (int q = 0; q < RW; q++) {
-        const int r = blockIdx.x * R + wv * RW + q;
+        const int r = bx * R + wv * RW + q;
         ok[q] = r < n;
         rr[q] = min(r, n - 1);
         ro[q] = (int64_t)rr[q] * ldb;
@@ -631,7 +636,7 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
             }
             s_p = p; s_rp = rp; s_piv = piv; s_pk = pk; s_pak1 = pak1; s_a1 = a1;
             s_skip = (piv == (T)0);
-            if (blockIdx.x == 0) ipiv[k] = p;
+            if (bx == 0) ipiv[k] = p;
         }
     }
     __syncthreads();
@@ -775,8 +780,45 @@ __global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
             const int i2 = wc[t].i;
             if (cand_better(v2, i2, bv, bi)) { bv = v2; bi = i2; bt = t; }
         }
-        cout[blockIdx.x] = wc[bt];
+        cout[bx] = wc[bt];
     }
+}
+
+template <typename T, int RPW, int U, int PASSES>
+__global__ void __launch_bounds__(kRowWaves * 64) k_rows_step(
+    T *__restrict__ B, int64_t ldb, int n, int k, const RowCand<T> *__restrict__ cin, RowCand<T> *__restrict__ cout,
+    int ncand, const RowNext<T> *__restrict__ rin, RowNext<T> *__restrict__ rout, const T *__restrict__ gin,
+    T *__restrict__ gout, int *__restrict__ ilog, int *__restrict__ ipiv) {
+    rows_step_body<T, RPW, U, PASSES>(B, ldb, n, k, cin, cout, ncand, rin, rout, gin, gout, ilog, ipiv,
+                                      (int)blockIdx.x);
+}
+
+// One matrix of a grouped reduction: its row-major working copy, its step
+// loop's candidate / next-row / column buffers (ping-pong by step parity) and
+// its labels and pivots.
+template <typename T>
+struct RowGroupDesc {
+    T *B;
+    int64_t ldb;
+    int n, G;                 // order, workgroups of its steps
+    RowCand<T> *cand;         // 2 G
+    RowNext<T> *nxt;          // 2
+    T *g;                     // 2 n
+    int *ilog, *ipiv;
+};
+
+// Step k of every matrix of a group (blockIdx.y = matrix): one launch per step
+// for matrices of one size class.  A matrix whose steps are over (k + 2 >= n)
+// or a row block past its rows returns at once; the others run the same step
+// body as a lone reduction, so the results are those of lhl_reduce.
+template <typename T, int RPW, int U, int PASSES>
+__global__ void __launch_bounds__(kRowWaves * 64) k_rows_step_grp(const RowGroupDesc<T> *__restrict__ desc, int k) {
+    const RowGroupDesc<T> d = desc[blockIdx.y];
+    if (k + 2 >= d.n || (int)blockIdx.x >= d.G) return;
+    const int a = k & 1, b = a ^ 1;
+    rows_step_body<T, RPW, U, PASSES>(d.B, d.ldb, d.n, k, d.cand + a * d.G, d.cand + b * d.G, d.G, d.nxt + a,
+                                      d.nxt + b, d.g + (int64_t)a * d.n, d.g + (int64_t)b * d.n, d.ilog, d.ipiv,
+                                      (int)blockIdx.x);
 }
 
 // perm[ilog[r]] = r (the row now at each logical position), iperm = ilog.
@@ -1043,7 +1085,146 @@ Status copy_scale_typed(lhl_handle *h, int balanced) {
     return {};
 }
 
+// A group of matrices of one element type and one step-kernel shape: each
+// matrix's prologue (scales, labels, balancing, the row-major copy, the loop's
+// start state), then one launch per step for all of them, then each matrix's
+// epilogue.  Every launch goes to the handles' streams, which the caller has
+// pointed at one stream.
+template <typename TJ, typename T, int RPW, int U, int PASSES>
+Status group_rows(const std::vector<lhl_handle *> &v, int balance) {
+    constexpr int R = kRowWaves * RPW * PASSES;
+    std::vector<RowGroupDesc<T>> d(v.size());
+    int maxn = 0, maxG = 0;
+    for (size_t m = 0; m < v.size(); m++) {
+        lhl_handle *h = v[m];
+        const int n = (int)h->n;
+        LHL_LAUNCH(h, "reduce_copy", k_fill1, dim3((n + 255) / 256), dim3(256), 0, h->scale64.as<double>(),
+                   h->iscale64.as<double>(), n);
+        LHL_LAUNCH(h, "reduce_copy", k_iota, dim3((n + 255) / 256), dim3(256), 0, h->perm.as<int>(), n);
+        if (balance) LHL_TRY(balance_device(h));
+        const int64_t ldb = pad_ld(n, h->dtype);
+        const int maxGh = (n + kRowWaves - 1) / kRowWaves;
+        LHL_TRY(h->rowA.ensure(sizeof(T) * (size_t)ldb * n));
+        LHL_TRY(h->gcol.ensure(sizeof(T) * 2 * (size_t)n));
+        LHL_TRY(h->ilog.ensure(sizeof(int) * (size_t)n));
+        LHL_TRY(h->cand.ensure(sizeof(RowCand<T>) * 2 * (size_t)maxGh + sizeof(RowNext<T>) * 2 + 64));
+        T *B = h->rowA.as<T>();
+        const int G = (n + R - 1) / R;
+        RowCand<T> *cand = h->cand.as<RowCand<T>>();
+        RowNext<T> *nxt = reinterpret_cast<RowNext<T> *>(cand + 2 * G);
+        dim3 tb(32, 8), gr((n + 31) / 32, (n + 31) / 32);
+        LHL_LAUNCH(h, "reduce_copy", (k_rows_load<TJ, T>), gr, tb, 0, (const TJ *)h->J.as<TJ>(), h->ldj, B, ldb, n,
+                   (const double *)h->scale64.as<double>(), (const double *)h->iscale64.as<double>(), balance);
+        LHL_LAUNCH(h, "reduce_copy", k_rows_init<T>, dim3(G), dim3(256), 0, (const T *)B, ldb, n, R, cand, nxt,
+                   h->gcol.as<T>(), h->ilog.as<int>());
+        d[m] = RowGroupDesc<T>{B, ldb, n, G, cand, nxt, h->gcol.as<T>(), h->ilog.as<int>(), h->ipiv.as<int>()};
+        maxn = std::max(maxn, n);
+        maxG = std::max(maxG, G);
+    }
+    lhl_handle *h0 = v[0];
+    LHL_TRY(h0->grp.ensure(sizeof(RowGroupDesc<T>) * d.size()));
+    unsigned char *keep = host_keep(h0, sizeof(RowGroupDesc<T>) * d.size());
+    std::memcpy(keep, d.data(), sizeof(RowGroupDesc<T>) * d.size());
+    LHL_HIP_TRY(hipMemcpyAsync(h0->grp.p, keep, sizeof(RowGroupDesc<T>) * d.size(), hipMemcpyHostToDevice, h0->stream));
+    auto fn = k_rows_step_grp<T, RPW, U, PASSES>;
+    LHL_TRY(allow_dynamic_lds(fn));
+    const size_t lds = sizeof(T) * 2 * (size_t)maxn;
+    const RowGroupDesc<T> *dd = h0->grp.as<RowGroupDesc<T>>();
+    for (int k = 0; k + 2 < maxn; k++)
+        LHL_LAUNCH(h0, "reduce_step", fn, dim3((unsigned)maxG, (unsigned)v.size()), dim3(kRowWaves * 64), lds, dd, k);
+    for (lhl_handle *h : v) {
+        const int n = (int)h->n;
+        dim3 tb(32, 8), gr((n + 31) / 32, (n + 31) / 32);
+        LHL_LAUNCH(h, "reduce_finish", k_rows_perm, dim3((n + 255) / 256), dim3(256), 0,
+                   (const int *)h->ilog.as<int>(), h->perm.as<int>(), h->iperm.as<int>(), n);
+        LHL_LAUNCH(h, "reduce_finish", k_rows_out<T>, gr, tb, 0, (const T *)h->rowA.as<T>(), pad_ld(n, h->dtype),
+                   (const int *)h->perm.as<int>(), h->A.as<T>(), h->lda, h->Ht.as<T>(), h->lda, n);
+        LHL_TRY(finish_typed<T>(h, 1));
+        LHL_TRY(linv_device(h));
+    }
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+template <typename T>
+Status group_class(const std::vector<lhl_handle *> &v, int cls, int balance) {
+    constexpr int W = sizeof(T) == 8 ? 1 : 2;
+    const bool j64 = v[0]->j_dtype == LHL_F64;
+    switch (cls) {
+    case 0: return j64 ? group_rows<double, T, 1, 8 * W, 1>(v, balance) : group_rows<float, T, 1, 8 * W, 1>(v, balance);
+    case 1: return j64 ? group_rows<double, T, 2, 8 * W, 1>(v, balance) : group_rows<float, T, 2, 8 * W, 1>(v, balance);
+    case 2: return j64 ? group_rows<double, T, 2, 8 * W, 2>(v, balance) : group_rows<float, T, 2, 8 * W, 2>(v, balance);
+    default: return j64 ? group_rows<double, T, 2, 8 * W, 4>(v, balance) : group_rows<float, T, 2, 8 * W, 4>(v, balance);
+    }
+}
+
 }  // namespace
+
+// Reductions of many matrices (each resident J already on its handle's
+// device): the matrices the row-resident loop takes alone (not blocked) are
+// grouped by element type, resident-J type and step-kernel shape (the size
+// classes rows_typed uses); each group runs on the stream of its first handle
+// with one launch per step for all of its matrices.  The others reduce one by
+// one on their own streams.  The caller synchronizes every handle's stream.
+Status reduce_group_device(lhl_handle *const *hs, int64_t count, int balance) {
+    std::vector<lhl_handle *> cls[2][2][4];   // [factor type][J type][shape]
+    for (int64_t m = 0; m < count; m++) {
+        lhl_handle *h = hs[m];
+        const int n = (int)h->n;
+        int nb = 0, rmin = 0, ksw = 0;
+        if (n < 3 || !rows_fit(n, h->dtype) || use_blocked(h, n, &nb, &rmin, &ksw)) {
+            LHL_TRY(reduce_device(h, balance));
+            continue;
+        }
+        const int c = n <= 2048 ? 0 : n <= 4096 ? 1 : n <= 8192 ? 2 : 3;
+        cls[h->dtype == LHL_F64 ? 0 : 1][h->j_dtype == LHL_F64 ? 0 : 1][c].push_back(h);
+    }
+    for (int t = 0; t < 2; t++)
+        for (int jt = 0; jt < 2; jt++)
+            for (int c = 0; c < 4; c++) {
+                std::vector<lhl_handle *> &v = cls[t][jt][c];
+                if (v.empty()) continue;
+                // every member on the group's stream while it is issued
+                std::vector<hipStream_t> keep(v.size());
+                hipStream_t gs = v[0]->stream;
+                for (size_t m = 0; m < v.size(); m++) {
+                    keep[m] = v[m]->stream;
+                    if (v[m]->stream != gs) {
+                        // the member's earlier work (its J upload) comes first
+                        LHL_HIP_TRY(hipStreamSynchronize(v[m]->stream));
+                        v[m]->stream = gs;
+                    }
+                }
+                // sub-groups whose row-major copies together stay within a
+                // budget (the knob group_bytes), run one after another.  Measured
+                // on 64 stiff J with n in [128, 3000] (one MI355X): 16 MiB
+                // 1455 / 1099 ms (FP64 / FP32), 64 MiB 1325 / 892, 128 MiB
+                // 1179 / 742, 256 MiB 990 / 680, all in one group 1074 / 690,
+                // one lhl() after another 1465 / 1152: the default is 256 MiB
+                const int64_t budget = v[0]->tune_group_bytes > 0 ? v[0]->tune_group_bytes : (int64_t)256 << 20;
+                std::vector<lhl_handle *> order = v;
+                std::stable_sort(order.begin(), order.end(),
+                                 [](const lhl_handle *x, const lhl_handle *y) { return x->n > y->n; });
+                Status st;
+                std::vector<lhl_handle *> sub;
+                int64_t bytes = 0;
+                for (size_t m = 0; m <= order.size() && st.ok(); m++) {
+                    const int64_t b = m < order.size() ? order[m]->n * order[m]->n * (int64_t)elt_size(order[m]->dtype) : 0;
+                    if (!sub.empty() && (m == order.size() || bytes + b > budget)) {
+                        st = t == 0 ? group_class<double>(sub, c, balance) : group_class<float>(sub, c, balance);
+                        sub.clear();
+                        bytes = 0;
+                    }
+                    if (m < order.size()) {
+                        sub.push_back(order[m]);
+                        bytes += b;
+                    }
+                }
+                for (size_t m = 0; m < v.size(); m++) v[m]->stream = keep[m];
+                LHL_TRY(st);
+            }
+    return {};
+}
 
 Status balance_device(lhl_handle *h) {
     return h->j_dtype == LHL_F64 ? balance_typed<double>(h) : balance_typed<float>(h);

@@ -37,10 +37,11 @@ __all__ = [
     "lhl_reduce_", "lhl_shift_", "lhl_ldiv_", "lhl_refine_", "applyZ_", "applyZinv_",
     "shift_solve_batched", "library_path", "load_library", "profile_enable", "profile_reset", "profile_read",
     "profile_read_work", "sync", "LHLNotConverged", "tune_set", "tune_get", "TUNING_KNOBS", "BatchPlan",
+    "lhl_group", "lhl_group_",
 ]
 
 TUNING_KNOBS = ("reduce_blocked_min", "reduce_nb", "reduce_rmin", "reduce_switch", "lsweep_depth",
-                "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes", "gemm_impl")
+                "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes", "gemm_impl", "group_bytes")
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
@@ -120,6 +121,7 @@ def load_library():
         "lhl_tune_set": ([vp, ctypes.c_char_p, i64], ci),
         "lhl_tune_get": ([vp, ctypes.c_char_p, ctypes.POINTER(i64)], ci),
         "lhl_applyZ_noalloc": ([vp, ci, vp, i64, i64, i64, ci, ci], ci),
+        "lhl_reduce_group": ([i64, vp, vp, vp, vp, ci, ci, ci], ci),
         "lhl_plan_create": ([vp, ctypes.POINTER(vp), i64, ci, vp, vp, i64, vp, i64, ci, cd, ci], ci),
         "lhl_plan_run": ([vp, vp, vp, vp, vp], ci),
         "lhl_plan_stats": ([vp, ctypes.POINTER(ci), ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i64)], ci),
@@ -497,6 +499,54 @@ def lhl(J, balance: bool = True, shift=None, thread=True, dtype=None) -> LHLWork
     T = np.dtype(dtype) if dtype is not None else _real_dtype(v.dtype)
     ws = LHLWorkspace(v.n, T, shift=shift)
     return lhl_reduce_(ws, J, balance, thread)
+
+
+def lhl_group_(wss, Js, balance: bool = True):
+    """lhl!(ws, J) for many workspaces at once (lhl_reduce_group): matrices
+    of one size class share one launch per reduction step.  Every J must have
+    the same element type and location; the results equal lhl_'s."""
+    if len(wss) != len(Js):
+        raise DimensionMismatch("one J per workspace")
+    if not wss:
+        return wss
+    vs = [_as_dense_J(J) for J in Js]
+    code0, dev0 = vs[0].code, vs[0].on_device
+    for v in vs:
+        if v.code != code0 or v.on_device != dev0:
+            raise ArgumentError("every J of a group must have one element type and location")
+    from . import autotune
+    for ws, v in zip(wss, vs):
+        autotune.apply(ws, v.n)
+    M = len(wss)
+    hs = (ctypes.c_void_p * M)(*[ws._h for ws in wss])
+    ps = (ctypes.c_void_p * M)(*[v.ptr for v in vs])
+    ns = np.array([v.n for v in vs], dtype=np.int64)
+    lds = np.array([v.ld for v in vs], dtype=np.int64)
+    code = _lib.lhl_reduce_group(M, ctypes.cast(hs, ctypes.c_void_p), ctypes.cast(ps, ctypes.c_void_p),
+                                 ns.ctypes.data, lds.ctypes.data, code0, int(bool(balance)), dev0)
+    if code != LHL_OK:
+        # the message sits on the handle the error concerns
+        for ws in wss:
+            msg = _lib.lhl_last_error(ws._h)
+            if msg:
+                _check(ws._h, code)
+        _check(wss[0]._h, code)
+    for ws, v in zip(wss, vs):
+        ws._n = v.n
+        ws.shift._n = v.n
+        ws.shift._loaded = False
+    del vs
+    return wss
+
+
+def lhl_group(Js, balance: bool = True, dtype=None):
+    """lhl(J) for many matrices at once: a list of workspaces (see lhl_group_)."""
+    wss = []
+    for J in Js:
+        v = _as_dense_J(J)
+        T = np.dtype(dtype) if dtype is not None else _real_dtype(v.dtype)
+        wss.append(LHLWorkspace(v.n, T))
+    return lhl_group_(wss, Js, balance)
 
 
 def _split_shift_args(args, fname):
