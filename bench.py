@@ -529,7 +529,7 @@ def main():
             torch.cuda.synchronize()
             dt4 = (time.perf_counter() - t0) / reps4
             plan_stats = {"complex": pc.stats(), "real": pr.stats()}
-            pc.close()   # the plans' factor pools go before the eager comparison allocates its own
+            pc.close()
             pr.close()
             torch.cuda.synchronize()
             # accuracy on the device: r = b - (sigma x + tau A x)
@@ -539,28 +539,11 @@ def main():
             g4 = torch.from_numpy(np.repeat(gam4, R4)).to(dev)
             Rr = Br - (Xr - (A4d @ Xr) * g4[None, :])
             relr = (Rr.abs().amax(0) / Br.abs().amax(0)).max().item()
-            # the same sweep through the eager batched call, for comparison
-            Xc2, Xr2 = torch.zeros_like(Bc), torch.zeros_like(Br)
-
-            def sweep():
-                lhl_hip.shift_solve_batched(ws4, 1j * om, -np.ones(S4), Bc, offs4, X=Xc2, precision=prec,
-                                            tol=args.tol, max_iter=30)
-                lhl_hip.shift_solve_batched(ws4, np.ones(S4), -gam4, Br, offs4, X=Xr2, precision=prec, tol=args.tol,
-                                            max_iter=30)
-            sweep()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            sweep()
-            torch.cuda.synchronize()
-            dt_eager = time.perf_counter() - t0
-            del Xc2, Xr2
             sweep4[name] = {"solves_per_s": 2 * S4 * R4 / dt4, "ms": 1e3 * dt4, "max_rel_resid": max(relc, relr),
                             "refine_rounds": [int(its.min()), int(its.max())], "path": "plans (HIP graphs)",
-                            "eager_ms": 1e3 * dt_eager,
                             "plan_stats_graph_rounds_replays_captures_extra": plan_stats}
-            log(f"bench: config-4 sweep {name}: {1e3 * dt4:.1f} ms as two graph launches "
-                f"({1e3 * dt_eager:.1f} ms eager), {2 * S4 * R4 / dt4:.0f} solves/s, "
-                f"max rel resid {max(relc, relr):.2e}")
+            log(f"bench: config-4 sweep {name}: {1e3 * dt4:.1f} ms as two graph launches, "
+                f"{2 * S4 * R4 / dt4:.0f} solves/s, max rel resid {max(relc, relr):.2e}")
             del ws4
 
     # ---- optional: the plain drop-in calls, one shift at a time
