@@ -1078,8 +1078,8 @@ __global__ void k_unit_lower(int m, const T *__restrict__ src, int64_t lds, T *_
 // backward sweep's operand, built once per reduction on first use.
 template <typename T>
 Status lp1_typed(lhl_handle *h) {
-    if (h->lp1_ready) return {};
     const int n = (int)h->n;
+    if (h->lp1_ready || n < 2) return {};
     LHL_TRY(h->Lp1.ensure(sizeof(T) * (size_t)h->ldlf * (size_t)std::max(n - 1, 1)));
     const dim3 gl((unsigned)((n - 1 + 255) / 256), (unsigned)(n - 1));
     LHL_LAUNCH(h, "reduce_finish", k_unit_lower<T>, gl, dim3(256), 0, n - 1, (const T *)h->A.as<T>() + 1, h->lda,
