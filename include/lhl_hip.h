@@ -46,7 +46,11 @@ enum {
     LHL_NOT_CONVERGED = 6
 };
 
-enum { LHL_F64 = 0, LHL_F32 = 1 };
+/* Element types.  LHL_C64 / LHL_C32 (ComplexF64 / ComplexF32, interleaved in
+ * host and device memory) make a complex workspace, LHLWorkspace{Complex{T}}
+ * (src:164-178 with a complex T): its reduction, shifts and solves are
+ * complex, its scales are real. */
+enum { LHL_F64 = 0, LHL_F32 = 1, LHL_C64 = 2, LHL_C32 = 3 };
 
 /* Working precision of a batched solve. */
 enum {
@@ -58,8 +62,10 @@ enum {
 int lhl_version(void);
 
 /* LHLWorkspace{T}(n; shift) (src:164-178) with T = dtype (LHL_F64/LHL_F32)
- * and shift = T (shift_complex = 0) or Complex{T} (1).  Allocates on the
- * current HIP device. */
+ * and shift = T (shift_complex = 0) or Complex{T} (1).  With dtype LHL_C64 or
+ * LHL_C32 the workspace is complex (T = Complex{Tr}) and its shifts are
+ * complex whatever shift_complex says (the reference's TG = T).  Allocates on
+ * the current HIP device. */
 int lhl_create(lhl_handle **h, int64_t n, int dtype, int shift_complex);
 int lhl_destroy(lhl_handle *h);
 const char *lhl_last_error(const lhl_handle *h);
@@ -68,6 +74,10 @@ const char *lhl_last_error(const lhl_handle *h);
  * J (n x n, leading dimension ldj) is not modified.  j_dtype is the element
  * type of J (LHL_F64 or LHL_F32); an FP64 J on an FP32 handle is balanced in
  * FP64, reduced in FP32, and kept resident in FP64 for lhl_refine's residual.
+ * A complex workspace takes a complex J (LHL_C64 / LHL_C32, interleaved) or a
+ * real one (a zero imaginary part), keeps it resident in FP64 and reduces it
+ * with the generic step loop (src:466-538; n <= 8192); a complex J on a real
+ * workspace is LHL_ERR_ARG.
  * n may differ from the handle's current size: the handle resizes
  * (src:256-275), and held shift states follow on their next lhl_shift. */
 int lhl_reduce(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype,
@@ -207,11 +217,13 @@ int lhl_get_ipiv(lhl_handle *h, int32_t *ipiv);
 int lhl_get_perm(lhl_handle *h, int32_t *perm, int32_t *iperm);
 /* ws.scale, ws.iscale as doubles (powers of two). */
 int lhl_get_scale(lhl_handle *h, double *scale, double *iscale);
-/* ws.factors (n x n, ld = n, handle's element type): H in triu(.,-1), step-c
- * multipliers in column c below the subdiagonal (src:127-129). */
+/* ws.factors (n x n, ld = n, handle's element type; interleaved complex for a
+ * complex workspace): H in triu(.,-1), step-c multipliers in column c below
+ * the subdiagonal (src:127-129). */
 int lhl_get_factors(lhl_handle *h, void *factors);
-/* ws.Ht: H transposed, n x n with ld = n; entries below the first
- * superdiagonal of Ht (i > j+1 of H) are zero. */
+/* ws.Ht: H transposed, n x n with ld = n (interleaved complex for a complex
+ * workspace); entries below the first superdiagonal of Ht (i > j+1 of H) are
+ * zero. */
 int lhl_get_ht(lhl_handle *h, void *ht);
 /* The shift state (sh NULL = ws.shift): U (n x n, ld = n, U[k, j] at
  * index k + j*n, zero below the diagonal), the multipliers l (n-1), swap

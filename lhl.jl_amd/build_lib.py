@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liblhl_hip.so")
 SOURCES = ["lhl_reduce.hip", "lhl_blocked.hip", "lhl_shift.hip", "lhl_solve.hip", "lhl_batch.hip", "lhl_gemm.hip",
-           "lhl_abi.hip"]
+           "lhl_cplx.hip", "lhl_abi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 

@@ -29,8 +29,9 @@ Status alloc_sized(lhl_handle *h, int64_t n) {
     const size_t es = elt_size(h->dtype);
     h->lda = pad_ld(n, h->dtype);
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
-    LHL_TRY(h->A.ensure(es * (size_t)h->lda * nn));
-    LHL_TRY(h->Ht.ensure(es * (size_t)h->lda * nn));
+    const size_t planes = h->cplx ? 2 : 1;   // a complex J: real and imaginary planes
+    LHL_TRY(h->A.ensure(es * (size_t)h->lda * nn * planes));
+    LHL_TRY(h->Ht.ensure(es * (size_t)h->lda * nn * planes));
     LHL_TRY(h->scale64.ensure(8 * nn));
     LHL_TRY(h->iscale64.ensure(8 * nn));
     LHL_TRY(h->scaleT.ensure(es * nn));
@@ -147,11 +148,14 @@ int lhl_create(lhl_handle **out, int64_t n, int dtype, int shift_complex) {
     if (!out) return LHL_ERR_ARG;
     *out = nullptr;
     if (n < 0) return LHL_ERR_DIM;
-    if (dtype != LHL_F64 && dtype != LHL_F32) return LHL_ERR_ARG;
+    if (dtype != LHL_F64 && dtype != LHL_F32 && dtype != LHL_C64 && dtype != LHL_C32) return LHL_ERR_ARG;
     lhl_handle *h = new (std::nothrow) lhl_handle();
     if (!h) return LHL_ERR_NOMEM;
-    h->dtype = dtype;
-    h->shift_complex = shift_complex ? 1 : 0;
+    // a complex element type: the factors' real type plus two planes; its
+    // shifts are complex (LHLWorkspace{Complex{T}} has TG = Complex{T})
+    h->cplx = (dtype == LHL_C64 || dtype == LHL_C32) ? 1 : 0;
+    h->dtype = (dtype == LHL_F64 || dtype == LHL_C64) ? LHL_F64 : LHL_F32;
+    h->shift_complex = (shift_complex || h->cplx) ? 1 : 0;
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return LHL_ERR_HIP;
@@ -162,7 +166,7 @@ int lhl_create(lhl_handle **out, int64_t n, int dtype, int shift_complex) {
     h->own->h = h;
     h->own->is_complex = h->shift_complex;
     h->own->planes = h->shift_complex ? 2 : 1;
-    h->own->dtype = dtype;
+    h->own->dtype = h->dtype;
     *out = h;
     return LHL_OK;
 }
@@ -249,12 +253,47 @@ static Status upload_pipelined(lhl_handle *h, void *dst, size_t dpitch, const vo
     return {};
 }
 
+// A complex workspace's J: the resident copy is FP64 in two planes (a real J
+// gets a zero imaginary plane), pitch pad_ld(n).
+static Status zreduce_prepare(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int j_on_device) {
+    if (n > 0 && (!J || ldj < n)) return {LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n"};
+    if (n > 8192) return {LHL_ERR_ARG, "lhl_reduce: a complex J above n = 8192 is not supported"};
+    LHL_TRY(set_device(h));
+    if (n != h->n) LHL_TRY(alloc_sized(h, n));
+    h->reduced = 0;
+    h->own->loaded = 0;
+    h->j_dtype = LHL_F64;
+    h->ldj = pad_ld(n, LHL_F64);
+    const int64_t jplane = h->ldj * std::max<int64_t>(n, 1);
+    LHL_TRY(h->J.ensure(8 * (size_t)jplane * 2));
+    if (n == 0) return {};
+    const bool jcx = j_dtype == LHL_C64 || j_dtype == LHL_C32;
+    const int rdt = (j_dtype == LHL_F64 || j_dtype == LHL_C64) ? LHL_F64 : LHL_F32;
+    const void *src = J;
+    int64_t lds = ldj;
+    if (!j_on_device) {
+        const size_t xes = elt_size(rdt) * (jcx ? 2 : 1);
+        LHL_TRY(h->bstage.ensure(xes * (size_t)n * n + 64));
+        LHL_HIP_TRY(hipMemcpy2DAsync(h->bstage.p, xes * n, J, xes * ldj, xes * n, n, hipMemcpyHostToDevice,
+                                     h->stream));
+        src = h->bstage.p;
+        lds = n;
+    }
+    if (!jcx) LHL_HIP_TRY(hipMemsetAsync(h->J.as<double>() + jplane, 0, 8 * (size_t)jplane, h->stream));
+    return to_planar(h, src, lds, rdt, jcx ? 1 : 0, h->J.p, h->ldj, jplane, LHL_F64, n, n);
+}
+
 // Checks, (re)sizing and the upload of J for a reduction (lhl_reduce and
 // lhl_reduce_group): the handle's resident J is FP64 unless J is FP32 on an
 // FP32 handle.  Leaves the handle unreduced.
 static Status reduce_prepare(lhl_handle *h, const void *J, int64_t n, int64_t ldj, int j_dtype, int j_on_device) {
     if (n < 0) return {LHL_ERR_DIM, "lhl_reduce: negative size"};
-    if (j_dtype != LHL_F64 && j_dtype != LHL_F32) return {LHL_ERR_ARG, "lhl_reduce: J must be Float64 or Float32"};
+    const bool jcx = j_dtype == LHL_C64 || j_dtype == LHL_C32;
+    if (j_dtype != LHL_F64 && j_dtype != LHL_F32 && !jcx)
+        return {LHL_ERR_ARG, "lhl_reduce: J must be Float64, Float32, ComplexF64 or ComplexF32"};
+    if (jcx && !h->cplx)
+        return {LHL_ERR_ARG, "lhl_reduce: a complex J needs a complex workspace (LHLWorkspace{Complex{T}})"};
+    if (h->cplx) return zreduce_prepare(h, J, n, ldj, j_dtype, j_on_device);
     if (n > 0 && (!J || ldj < n)) return {LHL_ERR_DIM, "lhl_reduce: J's leading dimension is smaller than n"};
     if (n > 8192 + 8192) return {LHL_ERR_ARG, "lhl_reduce: n above 16384 is not supported"};
     LHL_TRY(set_device(h));
@@ -369,6 +408,8 @@ int lhl_shift(lhl_handle *h, lhl_shift_state *sh_, double sr, double si, double 
     host_keep_reset(h);
     lhl_shift_state *sh = resolve_shift(h, sh_);
     if (sh->h != h) return fail(h, LHL_ERR_ARG, "lhl_shift: the shift state belongs to another workspace");
+    if (h->cplx && !sh->is_complex)
+        return fail(h, LHL_ERR_ARG, "a real LHLShift cannot serve a workspace of complex element type");
     if (!sh->is_complex && (si != 0.0 || ti != 0.0))
         return fail(h, LHL_ERR_ARG, "complex shift on a real shift state; build the workspace with a complex shift type "
                                     "or use a complex LHLShift");
@@ -563,6 +604,8 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
     if (precision != LHL_PREC_WORKING && precision != LHL_PREC_REFINE64) return fail(h, LHL_ERR_ARG, "batched: bad precision");
     if (precision == LHL_PREC_REFINE64 && h->j_dtype != LHL_F64)
         return fail(h, LHL_ERR_STATE, "batched: FP64 refinement needs a workspace reduced from an FP64 J");
+    if (h->cplx && !shifts_complex)
+        return fail(h, LHL_ERR_ARG, "batched: a complex workspace needs shifts_complex = 1 (complex B and X)");
     if (!shifts_complex)
         for (int64_t s = 0; s < nshift; s++)
             if (sigma[2 * s + 1] != 0.0 || tau[2 * s + 1] != 0.0)
@@ -1058,6 +1101,8 @@ int lhl_plan_create(lhl_handle *h, lhl_plan **out, int64_t nshift, int shifts_co
     if (precision == LHL_PREC_REFINE64 && h->j_dtype != LHL_F64)
         return fail(h, LHL_ERR_STATE, "plan: FP64 refinement needs a workspace reduced from an FP64 J");
     if (max_iter < 0) return fail(h, LHL_ERR_ARG, "plan: max_iter < 0");
+    if (h->cplx && !shifts_complex)
+        return fail(h, LHL_ERR_ARG, "plan: a complex workspace needs shifts_complex = 1 (complex B and X)");
     if (rhs_offsets[0] < 0) return fail(h, LHL_ERR_DIM, "plan: negative right-hand-side offset");
     for (int64_t s = 0; s < nshift; s++)
         if (rhs_offsets[s + 1] < rhs_offsets[s])
@@ -1271,13 +1316,17 @@ static int applyz_common(lhl_handle *h, int inverse, void *x, int64_t n, int64_t
         return fail(h, LHL_ERR_DIM, "x has length " + std::to_string(n) + ", the workspace is " + std::to_string(h->n) +
                                         "x" + std::to_string(h->n));
     if (!h->reduced) return fail(h, LHL_ERR_STATE, "applyZ: no reduction held");
+    if (h->cplx && !x_complex)
+        return fail(h, LHL_ERR_ARG, "applyZ: a real x cannot hold Z x of a complex workspace; use a complex vector");
     if (n == 0 || nrhs == 0) return LHL_OK;
     Status st = set_device(h);
     if (!st.ok()) return fail(h, st);
     const int sdt = stage_dtype(h, xdt);
     int64_t ld, plane;
     st = stage_in(h, h->xstage, x, n, nrhs, ldx, xdt, x_complex, on_device, sdt, ld, plane);
-    if (st.ok()) st = applyz_device(h, inverse, h->xstage.p, nrhs, ld, plane, sdt, x_complex);
+    if (st.ok())
+        st = h->cplx ? zapplyz_device(h, inverse, h->xstage.p, nrhs, ld, plane, sdt)
+                     : applyz_device(h, inverse, h->xstage.p, nrhs, ld, plane, sdt, x_complex);
     if (st.ok()) st = stage_out(h, h->xstage, ld, plane, sdt, x, n, nrhs, ldx, xdt, x_complex, on_device);
     if (st.ok()) {
         hipError_t e = hipStreamSynchronize(h->stream);
@@ -1304,6 +1353,7 @@ int lhl_applyZ_noalloc(lhl_handle *h, int inverse, void *x, int64_t n, int64_t n
     if (nrhs < 0 || (n > 0 && nrhs > 0 && (!x || ldx < n)))
         return fail(h, LHL_ERR_DIM, "applyZ (no allocation): bad leading dimension");
     if (!h->reduced) return fail(h, LHL_ERR_STATE, "applyZ: no reduction held");
+    if (h->cplx) return fail(h, LHL_ERR_ARG, "applyZ (no allocation): not available for a complex workspace");
     if (n == 0 || nrhs == 0) return LHL_OK;
     Status st = set_device(h);
     if (!st.ok()) return fail(h, st);
@@ -1325,7 +1375,7 @@ int lhl_applyZ_noalloc(lhl_handle *h, int inverse, void *x, int64_t n, int64_t n
 int lhl_get_info(const lhl_handle *h, int64_t *n, int *dtype, int *shift_complex, int *reduced) {
     if (!h) return LHL_ERR_ARG;
     if (n) *n = h->n;
-    if (dtype) *dtype = h->dtype;
+    if (dtype) *dtype = h->cplx ? (h->dtype == LHL_F64 ? LHL_C64 : LHL_C32) : h->dtype;
     if (shift_complex) *shift_complex = h->shift_complex;
     if (reduced) *reduced = h->reduced;
     return LHL_OK;
@@ -1361,10 +1411,31 @@ int lhl_get_scale(lhl_handle *h, double *scale, double *iscale) {
     return LHL_OK;
 }
 
+// a complex workspace's planar n x n matrix (pitch lda, planes lda * n apart)
+// into host memory as interleaved complex values, column-major
+static int get_planar_matrix(lhl_handle *h, const void *src, void *f, const char *who) {
+    const size_t es = elt_size(h->dtype);
+    const int64_t n = h->n;
+    std::vector<unsigned char> re(es * (size_t)n * n), im(es * (size_t)n * n);
+    const unsigned char *s0 = static_cast<const unsigned char *>(src);
+    if (hipSetDevice(h->device) != hipSuccess ||
+        hipMemcpy2D(re.data(), es * n, s0, es * h->lda, es * n, n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy2D(im.data(), es * n, s0 + es * (size_t)h->lda * n, es * h->lda, es * n, n, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return fail(h, LHL_ERR_HIP, std::string(who) + ": copy failed");
+    unsigned char *d = static_cast<unsigned char *>(f);
+    for (size_t e = 0; e < (size_t)n * n; e++) {
+        std::memcpy(d + es * 2 * e, re.data() + es * e, es);
+        std::memcpy(d + es * (2 * e + 1), im.data() + es * e, es);
+    }
+    return LHL_OK;
+}
+
 int lhl_get_factors(lhl_handle *h, void *f) {
     if (!h || !f) return LHL_ERR_ARG;
     if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
     if (h->n == 0) return LHL_OK;
+    if (h->cplx) return get_planar_matrix(h, h->A.p, f, "lhl_get_factors");
     const size_t es = elt_size(h->dtype);
     if (hipSetDevice(h->device) != hipSuccess ||
         hipMemcpy2D(f, es * h->n, h->A.p, es * h->lda, es * h->n, h->n, hipMemcpyDeviceToHost) != hipSuccess)
@@ -1376,6 +1447,7 @@ int lhl_get_ht(lhl_handle *h, void *f) {
     if (!h || !f) return LHL_ERR_ARG;
     if (!h->reduced) return fail(h, LHL_ERR_STATE, "no reduction held");
     if (h->n == 0) return LHL_OK;
+    if (h->cplx) return get_planar_matrix(h, h->Ht.p, f, "lhl_get_ht");
     const size_t es = elt_size(h->dtype);
     if (hipSetDevice(h->device) != hipSuccess ||
         hipMemcpy2D(f, es * h->n, h->Ht.p, es * h->lda, es * h->n, h->n, hipMemcpyDeviceToHost) != hipSuccess)

@@ -894,8 +894,13 @@ __global__ void k_resid_combine(int n, const TX *__restrict__ JX, int64_t ldq, c
         const TX sr = (TX)colshift[4 * col], si = (TX)colshift[4 * col + 1];
         const TX tr = (TX)colshift[4 * col + 2], ti = (TX)colshift[4 * col + 3];
         const TX xr = X[o], xi = planes > 1 ? X[o + plane] : (TX)0;
-        const TX jr = JX[i + (int64_t)(col * planes) * ldq];
-        const TX ji = planes > 1 ? JX[i + (int64_t)(col * planes + 1) * ldq] : (TX)0;
+        TX jr = JX[i + (int64_t)(col * planes) * ldq];
+        TX ji = planes > 1 ? JX[i + (int64_t)(col * planes + 1) * ldq] : (TX)0;
+        if (w_complex) {   // a complex J: J x = (Jr xr - Ji xi) + i (Jr xi + Ji xr)
+            const TX c = JXi[i + (int64_t)(col * 2) * ldq], d = JXi[i + (int64_t)(col * 2 + 1) * ldq];
+            jr = jr - d;
+            ji = ji + c;
+        }
         const TX wr = (sr * xr - si * xi) + (tr * jr - ti * ji);
         const TX wi = (sr * xi + si * xr) + (tr * ji + ti * jr);
         R[o] = B[o] - wr;
@@ -1127,7 +1132,9 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
                    (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
                    ldy, a.cact);
-        if (own_products(h, a.cact, a.full_products))
+        if (h->cplx)   // a complex L: the blocked sweep (every column; the gather skipped none of Yo's rows)
+            LHL_TRY(zsweep_device<T>(h, 1, Yw, Yo, ldy, a.ncol));
+        else if (own_products(h, a.cact, a.full_products))
             LHL_TRY(mgemm(h, "trsm", 1, n - 1, Q, n - 1, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
                           Yo + 1, ldy, a.cact, planes));
         else
@@ -1220,7 +1227,9 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     }
     LHL_HIP_TRY(hipGetLastError());
     // y_new' = (I + L') y' (Yo is free again), then x = D P y_new
-    if (own_products(h, a.cact, a.full_products)) {
+    if (h->cplx) {
+        LHL_TRY(zsweep_device<T>(h, 0, Yw, Yo, ldy, a.ncol));
+    } else if (own_products(h, a.cact, a.full_products)) {
         // (I + L') y' straight from the factors' strict lower triangle
         LHL_TRY(mgemm(h, "trmm", 2, N1, Q, N1, (const T *)h->A.as<T>() + 1, h->lda, (const T *)Yw + 1, ldy, Yo + 1,
                       ldy, a.cact, planes));
@@ -1274,13 +1283,15 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     TX *JX = jb.as<TX>();
     TX *JXi = nullptr;
     const dim3 gg((unsigned)((n + kT - 1) / kT), (unsigned)((Q + kT - 1) / kT));
+    // a complex resident J: JX = Jr X and JXi = Ji X (both planes), combined below
+    const bool jc = !a.W && h->cplx;
     if constexpr (std::is_same<TW, TX>::value) {
         // plain GEMM in one precision: the library's tuned kernels
         const TW *W = a.W ? (const TW *)a.W : (const TW *)h->J.p;
         const int64_t ldw = a.W ? a.ldw : h->ldj;
         LHL_TRY(blas_jx<TX>(h, W, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol, JX, ldq,
                             a.full_products ? nullptr : a.cact));
-        if (a.W && a.w_complex) {
+        if ((a.W && a.w_complex) || jc) {
             LHL_TRY(jib.ensure(sizeof(TX) * (size_t)ldq * Q));
             JXi = jib.as<TX>();
             LHL_TRY(blas_jx<TX>(h, W + ldw * (int64_t)n, ldw, (const TX *)a.x, a.ld, a.plane, planes, (int)a.ncol,
@@ -1299,11 +1310,18 @@ Status residual_typed(lhl_handle *h, const ResidArgs &a) {
     } else {
         LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n, (const TW *)h->J.p, h->ldj,
                    (const TX *)a.x, a.ld, a.plane, planes, Q, JX, ldq);
+        if (jc) {
+            LHL_TRY(jib.ensure(sizeof(TX) * (size_t)ldq * Q));
+            JXi = jib.as<TX>();
+            LHL_LAUNCH(h, "gemm_jx", (k_gemm_jx<TW, TX>), gg, dim3(256), 0, n,
+                       (const TW *)h->J.p + h->ldj * (int64_t)n, h->ldj, (const TX *)a.x, a.ld, a.plane, planes, Q,
+                       JXi, ldq);
+        }
     }
     const dim3 gc((unsigned)((n + 255) / 256), (unsigned)a.ncol);
     LHL_LAUNCH(h, "resid_combine", k_resid_combine<TX>, gc, dim3(256), 0, n, (const TX *)JX, ldq, (const TX *)JXi,
                (const TX *)a.x, (const TX *)a.b, (TX *)a.r, a.ld, a.plane, planes, a.W ? nullptr : a.colshift,
-               a.W ? a.w_complex : 0, a.cact);
+               a.W ? a.w_complex : (jc ? 1 : 0), a.cact);
     LHL_LAUNCH(h, "colnorms", k_colnorms<TX>, dim3((unsigned)a.ncol), dim3(256), 0, n, (const TX *)a.r,
                (const TX *)a.b, a.ld, a.plane, planes, (int)a.ncol, a.norms, a.cact);
     LHL_HIP_TRY(hipGetLastError());
@@ -1397,6 +1415,7 @@ Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rd
 }
 
 Status linv_device(lhl_handle *h) {
+    if (h->cplx) return {};   // a complex L is swept directly (lhl_cplx.hip)
     if (h->n < 2) return {};
     return h->dtype == LHL_F64 ? linv_typed<double>(h) : linv_typed<float>(h);
 }
@@ -1409,6 +1428,59 @@ Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int plane
     (void)h;
     return {};
 }
+
+// internal columns <-> planar staged columns (no permutation): the ends of
+// applyZ / applyZinv on a complex L
+template <typename T, typename TX>
+__global__ void k_zstage(int n, const TX *__restrict__ x, int64_t ldx, int64_t xplane, T *__restrict__ Y,
+                         int64_t ldy, int to_y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (i >= n) return;
+    const int col = q >> 1, pl = q & 1;
+    TX *xp = const_cast<TX *>(x) + i + (int64_t)col * ldx + pl * xplane;
+    T *yp = Y + i + (int64_t)q * ldy;
+    if (to_y) *yp = (T)*xp;
+    else *xp = (TX)*yp;
+}
+
+// applyZinv (inverse = 1): x <- L^-1 P^-1 D^-1 x; applyZ: x <- D P L x, on
+// staged planar columns of element type xdt.
+template <typename T, typename TX>
+Status zapplyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane) {
+    const int n = (int)h->n;
+    const int64_t ldy = std::max<int64_t>(64, ((n + 63) / 64) * 64);
+    const int Q = (int)(2 * ncol);
+    LHL_TRY(h->ybuf.ensure(sizeof(T) * (size_t)ldy * Q));
+    LHL_TRY(h->rbuf.ensure(sizeof(T) * (size_t)ldy * Q));
+    T *Yw = h->ybuf.as<T>(), *Yo = h->rbuf.as<T>();
+    const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
+    if (inverse) {
+        LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
+                   (const double *)h->iscale64.as<double>(), (const TX *)x, ldx, xplane, 2, Yw, Yo, ldy,
+                   (const int *)nullptr);
+        LHL_TRY(zsweep_device<T>(h, 1, Yw, Yo, ldy, ncol));
+        LHL_LAUNCH(h, "scatter", (k_zstage<T, TX>), gq, dim3(256), 0, n, (TX *)x, ldx, xplane, Yo, ldy, 0);
+    } else {
+        LHL_LAUNCH(h, "gather", (k_zstage<T, TX>), gq, dim3(256), 0, n, (const TX *)x, ldx, xplane, Yw, ldy, 1);
+        LHL_TRY(zsweep_device<T>(h, 0, Yw, Yo, ldy, ncol));
+        LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), gq, dim3(256), 0, n, (const T *)Yo, (const T *)Yo, ldy, 2,
+                   (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(), (TX *)x, ldx, xplane, 0,
+                   (const int *)nullptr);
+    }
+    LHL_HIP_TRY(hipGetLastError());
+    return {};
+}
+
+Status zapplyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane, int xdt) {
+    if (h->n == 0 || ncol == 0) return {};
+    if (h->dtype == LHL_F64) {
+        if (xdt == LHL_F64) return zapplyz_typed<double, double>(h, inverse, x, ncol, ldx, xplane);
+        return zapplyz_typed<double, float>(h, inverse, x, ncol, ldx, xplane);
+    }
+    if (xdt == LHL_F64) return zapplyz_typed<float, double>(h, inverse, x, ncol, ldx, xplane);
+    return zapplyz_typed<float, float>(h, inverse, x, ncol, ldx, xplane);
+}
+
 
 Status solve_device(lhl_handle *h, const SolveArgs &a) {
     if (h->n == 0 || a.ncol == 0) return {};

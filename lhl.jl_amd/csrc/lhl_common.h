@@ -362,6 +362,7 @@ struct lhl_handle {
     int64_t ldj = 0;            // leading dimension of the resident J
     int j_dtype = LHL_F64;      // element type of the resident J
     int reduced = 0;
+    int cplx = 0;               // a complex J (LHLWorkspace{Complex{T}}): A, Ht and J hold two planes each
     lhl::DevBuf A, Ht, J, scale64, iscale64, scaleT, iscaleT, ipiv, perm, iperm;
     lhl::DevBuf part, cand, ctl, counter, flag;   // reduction scratch
     lhl::DevBuf rowA, gcol, ilog;   // row-resident step loop: A row-major, final column k, row -> logical index
@@ -460,6 +461,18 @@ Status reduce_group_device(lhl_handle *const *hs, int64_t count, int balance);
 // loop takes over: *kstart receives that step (n - 2 when the panels finished)
 Status blocked_device(lhl_handle *h, int64_t ldb, int nb, int rmin, int kswitch, int *kstart);   // lhl_blocked.hip
 Status balance_device(lhl_handle *h);                      // lhl_reduce.hip
+// a complex J (lhl_cplx.hip): the reduction of the handle's planar J, the
+// shift LUs of its complex H (prm: sigma_re, sigma_im, tau_re, tau_im per
+// shift, in T), and the sweeps y_out <- L^-1 y_in (fwd) or L y_in of its
+// complex L on internal columns (q = 2c real plane, 2c + 1 imaginary)
+Status zreduce_device(lhl_handle *h, int balance);
+template <typename T>
+Status zshift_launch(lhl_handle *h, int64_t nshift, const T *prm, T *U, int64_t us, T *l, T *rd, int64_t ls,
+                     uint8_t *sw, int64_t ss, int *info);
+template <typename T>
+Status zsweep_device(lhl_handle *h, int fwd, const T *Yin, T *Yout, int64_t ldy, int64_t ncol);
+// applyZ (inverse 0) / applyZinv (1) of a complex J on staged planar columns (lhl_batch.hip)
+Status zapplyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane, int xdt);
 Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t nshift, const double *sig, const double *tau,
                     int64_t *info_out);                    // lhl_shift.hip (one state)
 

@@ -1172,7 +1172,7 @@ Status reduce_group_device(lhl_handle *const *hs, int64_t count, int balance) {
         lhl_handle *h = hs[m];
         const int n = (int)h->n;
         int nb = 0, rmin = 0, ksw = 0;
-        if (n < 3 || !rows_fit(n, h->dtype) || use_blocked(h, n, &nb, &rmin, &ksw)) {
+        if (h->cplx || n < 3 || !rows_fit(n, h->dtype) || use_blocked(h, n, &nb, &rmin, &ksw)) {
             LHL_TRY(reduce_device(h, balance));
             continue;
         }
@@ -1234,6 +1234,7 @@ Status balance_device(lhl_handle *h) {
 Status reduce_device(lhl_handle *h, int balance) {
     const int n = (int)h->n;
     if (n == 0) return {};
+    if (h->cplx) return zreduce_device(h, balance);
     LHL_LAUNCH(h, "reduce_copy", k_fill1, dim3((n + 255) / 256), dim3(256), 0, h->scale64.as<double>(),
                        h->iscale64.as<double>(), n);
     LHL_LAUNCH(h, "reduce_copy", k_iota, dim3((n + 255) / 256), dim3(256), 0, h->perm.as<int>(), n);

@@ -18,6 +18,7 @@
 // no contraction (mul, mul, then sub), and IEEE division, so the real state
 // (U, multipliers, swap, rdiag, info) equals the reference's bit for bit.
 #include "lhl_common.h"
+#include "lhl_cdiv.h"
 
 #include <cstring>
 #include <type_traits>
@@ -516,45 +517,6 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
 // parts, at +ustride/2 (U) and +lstride/2 (l, rdiag).  Products follow Julia's
 // term order; division and 1/x use the Baudin-Smith robust algorithm
 // ("A Robust Complex Division in Scilab", 2012).
-template <typename T>
-__device__ __forceinline__ void cdiv1(T a, T b, T c, T d, T &p, T &q) {
-#pragma clang fp contract(off)
-    const T r = d / c;
-    const T t = (T)1 / (c + d * r);
-    if (r != (T)0) {
-        const T br = b * r;
-        p = (br != (T)0) ? (a + br) * t : a * t + (b * t) * r;
-        const T ar = -a * r;
-        q = (ar != (T)0) ? (b + ar) * t : b * t + (-a * t) * r;
-    } else {
-        p = (a + d * (b / c)) * t;
-        q = (b + d * (-a / c)) * t;
-    }
-}
-template <typename T>
-__device__ __forceinline__ void cdiv(T a, T b, T c, T d, T &p, T &q) {
-    if (dabs(d) <= dabs(c)) cdiv1(a, b, c, d, p, q);
-    else { cdiv1(b, a, d, c, p, q); q = -q; }
-}
-
-// cdiv with selects in place of its branches: each chosen result is computed
-// by the same operations as cdiv's branch, so the two agree bit for bit.
-template <typename T>
-__device__ __forceinline__ void cdiv_sel(T a, T b, T c, T d, T &p, T &q) {
-#pragma clang fp contract(off)
-    const bool sw = dabs(d) > dabs(c);
-    const T A = sw ? b : a, B = sw ? a : b, C = sw ? d : c, D = sw ? c : d;
-    const T r = D / C;
-    const T t = (T)1 / (C + D * r);
-    const T br = B * r;
-    const T p1 = (A + br) * t, p2 = A * t + (B * t) * r, p3 = (A + D * (B / C)) * t;
-    const T ar = -A * r;
-    const T q1 = (B + ar) * t, q2 = B * t + (-A * t) * r, q3 = (B + D * (-A / C)) * t;
-    p = (r != (T)0) ? ((br != (T)0) ? p1 : p2) : p3;
-    const T qq = (r != (T)0) ? ((ar != (T)0) ? q1 : q2) : q3;
-    q = sw ? -qq : qq;
-}
-
 template <typename T, int NPER, int NT>
 __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int64_t ldh, int n,
                                                               const ShiftParam<T> *__restrict__ prm,
@@ -737,6 +699,9 @@ template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info) {
+    static_assert(sizeof(ShiftParam<T>) == 4 * sizeof(T), "ShiftParam is four T: (sr, si, tr, ti)");
+    if (h->cplx)   // a complex H (a complex J's reduction)
+        return zshift_launch<T>(h, nshift, reinterpret_cast<const T *>(prm), U, us, l, rd, ls, sw, ss, info);
     constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int BW = panel_w<T>() * kPanelBulkWide;

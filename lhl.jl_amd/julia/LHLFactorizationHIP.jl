@@ -9,12 +9,15 @@ module LHLFactorizationHIP
 using LinearAlgebra
 
 const liblhl = joinpath(@__DIR__, "..", "lib", "liblhl_hip.so")
-const LHL_F64, LHL_F32 = Cint(0), Cint(1)
+const LHL_F64, LHL_F32, LHL_C64, LHL_C32 = Cint(0), Cint(1), Cint(2), Cint(3)
 const LHL_PREC_WORKING, LHL_PREC_REFINE64 = Cint(0), Cint(1)
 const LHL_NOT_CONVERGED = Cint(6)
 
 dtype_code(::Type{Float64}) = LHL_F64
 dtype_code(::Type{Float32}) = LHL_F32
+dtype_code(::Type{ComplexF64}) = LHL_C64     # a complex J / complex workspace (interleaved)
+dtype_code(::Type{ComplexF32}) = LHL_C32
+const LHLElt = Union{Float32,Float64,ComplexF32,ComplexF64}
 
 function check(st::Cint, h::Ptr{Cvoid})
     st == 0 && return nothing
@@ -46,15 +49,15 @@ mutable struct LHLWorkspace{T,Tr,TG}
     shift::LHLShift{TG,Tr}
 end
 
-function LHLWorkspace{T}(n::Integer; shift::Type = T) where {T<:Union{Float32,Float64}}
-    shift === T || shift === Complex{T} ||
-        throw(ArgumentError("shift type must be $T or $(Complex{T}), got $shift"))   # src:167-169
+function LHLWorkspace{T}(n::Integer; shift::Type = T) where {T<:LHLElt}
+    shift === T || shift === Complex{real(T)} ||
+        throw(ArgumentError("shift type must be $T or $(Complex{real(T)}), got $shift"))   # src:167-169
     r = Ref{Ptr{Cvoid}}(C_NULL)
     st = ccall((:lhl_create, liblhl), Cint, (Ref{Ptr{Cvoid}}, Int64, Cint, Cint),
                r, n, dtype_code(T), shift <: Complex)
     check(st, r[])
     sh = LHLShift{shift}(n); sh.own = true
-    ws = LHLWorkspace{T,T,shift}(r[], n, sh)
+    ws = LHLWorkspace{T,real(T),shift}(r[], n, sh)
     sh.ws = ws
     finalizer(w -> ccall((:lhl_destroy, liblhl), Cint, (Ptr{Cvoid},), w.h), ws)
 end
@@ -67,7 +70,7 @@ end
 
 function _bind!(sh::LHLShift{TG}, ws::LHLWorkspace{T}) where {TG,T}
     sh.ws === ws && return sh
-    (TG === T || TG === Complex{T}) ||
+    (TG === T || TG === Complex{real(T)}) ||
         throw(ArgumentError("an LHLShift{$TG} cannot serve a workspace of element type $T"))
     r = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:lhl_shift_create, liblhl), Cint, (Ptr{Cvoid}, Ref{Ptr{Cvoid}}, Cint),
@@ -78,7 +81,7 @@ function _bind!(sh::LHLShift{TG}, ws::LHLWorkspace{T}) where {TG,T}
 end
 
 function lhl_reduce!(ws::LHLWorkspace{T}, J::StridedMatrix{TJ}, balance::Bool = true,
-                     thread = Val(true)) where {T,TJ<:Union{Float32,Float64}}
+                     thread = Val(true)) where {T,TJ<:LHLElt}
     n = LinearAlgebra.checksquare(J)
     st = GC.@preserve J ccall((:lhl_reduce, liblhl), Cint,
                (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Cint, Cint, Cint),

@@ -47,7 +47,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")
 
 LHL_OK, LHL_ERR_ARG, LHL_ERR_DIM, LHL_ERR_HIP, LHL_ERR_STATE, LHL_ERR_NOMEM, LHL_NOT_CONVERGED = range(7)
-LHL_F64, LHL_F32 = 0, 1
+LHL_F64, LHL_F32, LHL_C64, LHL_C32 = 0, 1, 2, 3
 LHL_PREC_WORKING, LHL_PREC_REFINE64 = 0, 1
 
 
@@ -169,6 +169,14 @@ def _code(dt):
     return LHL_F64 if _real_dtype(dt) == np.float64 else LHL_F32
 
 
+def _full_code(dt):
+    """The element-type code with complexity (LHL_C64 / LHL_C32 for complex)."""
+    dt = np.dtype(dt)
+    if np.issubdtype(dt, np.complexfloating):
+        return LHL_C64 if _real_dtype(dt) == np.float64 else LHL_C32
+    return _code(dt)
+
+
 def _is_torch(x):
     return hasattr(x, "data_ptr") and hasattr(x, "is_cuda")
 
@@ -281,7 +289,7 @@ class LHLShift:
         n = self._n
         if ws is None or not self._loaded:
             raise LHLError("no shift loaded")
-        rdt = ws.dtype
+        rdt = _real_dtype(ws.dtype)
         cd = np.dtype(np.complex128 if rdt == np.float64 else np.complex64) if self.is_complex else rdt
         U = np.zeros((n, n), dtype=cd, order="F")
         l = np.zeros(max(n - 1, 1), dtype=cd)
@@ -331,20 +339,24 @@ def _check_shift_type(sh, ws):
 
 
 class LHLWorkspace:
-    """LHLWorkspace{T}(n; shift = T): the reduction of one J (src:118-191)."""
+    """LHLWorkspace{T}(n; shift = T): the reduction of one J (src:118-191).
+    T is Float64, Float32 or their complex forms; a complex T reduces a complex
+    J and its shifts are T."""
 
     def __init__(self, n, dtype=np.float64, shift=None):
         L = load_library()
         self.dtype = np.dtype(dtype)
-        if self.dtype not in (np.float64, np.float32):
-            raise ArgumentError(f"workspace element type must be Float64 or Float32, got {self.dtype}")
+        if self.dtype not in (np.float64, np.float32, np.complex128, np.complex64):
+            raise ArgumentError(f"workspace element type must be Float64, Float32 or their complex forms, "
+                                f"got {self.dtype}")
         TG = np.dtype(shift) if shift is not None else self.dtype
-        cplx = np.dtype(np.complex128 if self.dtype == np.float64 else np.complex64)
+        rt = _real_dtype(self.dtype)
+        cplx = np.dtype(np.complex128 if rt == np.float64 else np.complex64)
         if TG != self.dtype and TG != cplx:
             raise ArgumentError(f"shift type must be {self.dtype} or {cplx}, got {TG}")
         self.shift_dtype = TG
         h = ctypes.c_void_p()
-        code = L.lhl_create(ctypes.byref(h), int(n), _code(self.dtype), int(TG == cplx))
+        code = L.lhl_create(ctypes.byref(h), int(n), _full_code(self.dtype), int(TG == cplx))
         if code != LHL_OK:
             raise LHLError(f"lhl_create failed with code {code}")
         self._h = h
@@ -419,13 +431,13 @@ class LHLWorkspace:
     def scale(self):
         s = np.zeros(max(self.n, 1))
         _check(self._h, _lib.lhl_get_scale(self._h, s.ctypes.data, None))
-        return s[: self.n].astype(self.dtype)
+        return s[: self.n].astype(_real_dtype(self.dtype))   # Vector{real(T)}
 
     @property
     def iscale(self):
         s = np.zeros(max(self.n, 1))
         _check(self._h, _lib.lhl_get_scale(self._h, None, s.ctypes.data))
-        return s[: self.n].astype(self.dtype)
+        return s[: self.n].astype(_real_dtype(self.dtype))
 
     # shift fields forwarded from ws.shift (src:182-191)
     @property
@@ -462,10 +474,8 @@ def _as_dense_J(J):
     J = np.asarray(J)
     if J.ndim != 2 or J.shape[0] != J.shape[1]:
         raise DimensionMismatch(f"matrix is not square: dimensions are {J.shape}")
-    if np.iscomplexobj(J):
-        raise ArgumentError("a complex J is not supported by the GPU reduction (real J only)")
-    if J.dtype not in (np.float64, np.float32):
-        J = J.astype(np.float64)
+    if J.dtype not in (np.float64, np.float32, np.complex128, np.complex64):
+        J = J.astype(np.complex128 if np.iscomplexobj(J) else np.float64)
     if not (J.flags.f_contiguous or (J.strides[0] == J.itemsize and J.strides[1] >= J.shape[0] * J.itemsize
                                      and J.strides[1] % J.itemsize == 0)):
         J = np.asfortranarray(J)
@@ -480,7 +490,8 @@ def lhl_reduce_(ws: LHLWorkspace, J, balance: bool, thread=True) -> LHLWorkspace
     v = _as_dense_J(J)
     from . import autotune
     autotune.apply(ws, v.n)
-    _check(ws._h, _lib.lhl_reduce(ws._h, ctypes.c_void_p(v.ptr), v.n, v.ld, v.code, int(bool(balance)), v.on_device))
+    _check(ws._h, _lib.lhl_reduce(ws._h, ctypes.c_void_p(v.ptr), v.n, v.ld, _full_code(v.dtype), int(bool(balance)),
+                                  v.on_device))
     ws._n = v.n
     ws.shift._n = v.n
     ws.shift._loaded = False
@@ -496,7 +507,7 @@ def lhl(J, balance: bool = True, shift=None, thread=True, dtype=None) -> LHLWork
     """lhl(J; balance, shift) (src:3179).  dtype overrides the reduction's element
     type (Float32 on a Float64 J gives the FP32 factors of the precision form)."""
     v = _as_dense_J(J)
-    T = np.dtype(dtype) if dtype is not None else _real_dtype(v.dtype)
+    T = np.dtype(dtype) if dtype is not None else np.dtype(v.dtype)
     ws = LHLWorkspace(v.n, T, shift=shift)
     return lhl_reduce_(ws, J, balance, thread)
 
@@ -510,9 +521,9 @@ def lhl_group_(wss, Js, balance: bool = True):
     if not wss:
         return wss
     vs = [_as_dense_J(J) for J in Js]
-    code0, dev0 = vs[0].code, vs[0].on_device
+    code0, dev0 = _full_code(vs[0].dtype), vs[0].on_device
     for v in vs:
-        if v.code != code0 or v.on_device != dev0:
+        if _full_code(v.dtype) != code0 or v.on_device != dev0:
             raise ArgumentError("every J of a group must have one element type and location")
     from . import autotune
     for ws, v in zip(wss, vs):
@@ -544,7 +555,7 @@ def lhl_group(Js, balance: bool = True, dtype=None):
     wss = []
     for J in Js:
         v = _as_dense_J(J)
-        T = np.dtype(dtype) if dtype is not None else _real_dtype(v.dtype)
+        T = np.dtype(dtype) if dtype is not None else np.dtype(v.dtype)
         wss.append(LHLWorkspace(v.n, T))
     return lhl_group_(wss, Js, balance)
 
@@ -703,7 +714,7 @@ def shift_solve_batched(ws: LHLWorkspace, sigmas, taus, B, rhs_offsets, X=None, 
     if sig.shape != tau.shape:
         raise DimensionMismatch("sigmas and taus differ in length")
     S = sig.shape[0]
-    cplx = bool(np.any(sig.imag != 0) or np.any(tau.imag != 0))
+    cplx = bool(np.any(sig.imag != 0) or np.any(tau.imag != 0)) or bool(np.issubdtype(ws.dtype, np.complexfloating))
     if _is_torch(B):
         cplx = cplx or bool(B.is_complex())
     else:

@@ -97,6 +97,8 @@ def apply(ws, n) -> dict:
     from . import _check, _lib
     if os.environ.get("LHL_TUNE", "1") == "0":
         return {}
+    if np.issubdtype(ws.dtype, np.complexfloating):   # the knobs tune the real paths only
+        return {}
     knobs = {k: v for k, v in lookup(ws.dtype, n).items() if k not in ws._user_knobs}
 
     def put(k, v):

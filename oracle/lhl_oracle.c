@@ -19,6 +19,11 @@
  *                           cannot be reproduced without Julia) -- dense complex solve.
  *   lhlo_ldiv_* and refine  numpy/LAPACK dense solves at rtol 1e-9 and the bwd bound,
  *                           mpmath high-precision solves on tiny inputs.
+ *   lhlo_z* (complex J)     J == Z H Z^-1 and eig(H) == eig(J) (complex LAPACK),
+ *                           |multipliers| <= 1, upper-Hessenberg closed form, the
+ *                           real restatement on a complex J with zero imaginary part
+ *                           (same pivots, factors and scales), P L U of the complex
+ *                           shift, dense complex solves, mpmath on tiny inputs.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -30,6 +35,9 @@
 #define FMA fma
 #define BLOCK_MIN 500
 #include "lhl_oracle_body.h"
+#define SQRT sqrt
+#include "lhl_oracle_cplx.h"
+#undef SQRT
 #undef REAL
 #undef SFX
 #undef ABS
@@ -42,6 +50,9 @@
 #define FMA fmaf
 #define BLOCK_MIN 1024
 #include "lhl_oracle_body.h"
+#define SQRT sqrtf
+#include "lhl_oracle_cplx.h"
+#undef SQRT
 #undef REAL
 #undef SFX
 #undef ABS

@@ -78,6 +78,31 @@ def lib():
             f = getattr(L, "lhlo_refine" + sfx)
             f.argtypes = [c_int, fp, c_int, fp, fp, c_int, _ip, _ip, fp, fp, fp, _bp, fp, fp, c_int]
             f.restype = None
+        for sfx, fp in (("_d", _dp), ("_s", _sp)):
+            R = ctypes.c_double if sfx == "_d" else ctypes.c_float
+            c_int = ctypes.c_int
+            f = getattr(L, "lhlo_zabs" + sfx)
+            f.argtypes = [R, R]
+            f.restype = R
+            f = getattr(L, "lhlo_zbalance" + sfx)
+            f.argtypes = [c_int, fp, fp, c_int, fp, fp]
+            f.restype = None
+            f = getattr(L, "lhlo_zreduce_unblocked" + sfx)
+            f.argtypes = [c_int, fp, fp, c_int, _ip]
+            f.restype = None
+            f = getattr(L, "lhlo_zreduce" + sfx)
+            f.argtypes = [c_int, fp, fp, c_int, c_int, fp, fp, c_int, _ip, _ip, _ip, fp, fp, fp, fp]
+            f.restype = None
+            f = getattr(L, "lhlo_zshift" + sfx)
+            f.argtypes = [c_int, fp, fp, R, R, R, R, fp, fp, _bp, fp, fp]
+            f.restype = c_int
+            f = getattr(L, "lhlo_zldiv" + sfx)
+            f.argtypes = [c_int, fp, fp, c_int, _ip, _ip, fp, fp, fp, fp, _bp, fp, fp, fp, fp]
+            f.restype = None
+            for nm in ("lhlo_zapplyZ", "lhlo_zapplyZinv"):
+                f = getattr(L, nm + sfx)
+                f.argtypes = [c_int, fp, fp, c_int, _ip, fp, fp, fp]
+                f.restype = None
         _lib = L
     return _lib
 
@@ -299,3 +324,147 @@ def bwd(W, x, b):
     r = np.asarray(b) - W @ np.asarray(x)
     return np.linalg.norm(r, np.inf) / (np.linalg.norm(W, np.inf) * np.linalg.norm(x, np.inf)
                                         + np.linalg.norm(b, np.inf))
+
+
+# ---- complex J (LHLWorkspace{Complex{T}}): planar arrays for the C side ----
+
+def _real_of(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.complex128:
+        return np.dtype(np.float64)
+    if dtype == np.complex64:
+        return np.dtype(np.float32)
+    raise TypeError(f"complex oracle supports complex128/complex64, got {dtype}")
+
+
+def zabs(z, dtype=np.complex128):
+    """The modulus used by the complex reduction's pivoting and balancing."""
+    rt = _real_of(dtype)
+    return getattr(lib(), "lhlo_zabs" + _sfx(rt))(float(np.real(z)), float(np.imag(z)))
+
+
+@dataclass
+class ZReduction:
+    """The oracle's reduction of a complex J (as Reduction, complex A and Ht;
+    scale and iscale real)."""
+
+    A: np.ndarray
+    ipiv: np.ndarray
+    perm: np.ndarray
+    iperm: np.ndarray
+    scale: np.ndarray
+    iscale: np.ndarray
+    Ht: np.ndarray
+
+    @property
+    def n(self):
+        return self.A.shape[0]
+
+    @property
+    def H(self):
+        return np.triu(self.A, -1)
+
+    @property
+    def L(self):
+        n = self.n
+        L = np.eye(n, dtype=self.A.dtype)
+        for c in range(max(n - 2, 0)):
+            L[c + 2:, c + 1] = self.A[c + 2:, c]
+        return L
+
+    def Z(self):
+        n = self.n
+        Z = np.eye(n, dtype=self.A.dtype, order="F")
+        for j in range(n):
+            Z[:, j] = zapplyZ(self, Z[:, j].copy())
+        return Z
+
+
+def zreduce(J, balance=True, dtype=None):
+    """lhl(J; balance) of a complex J on the oracle (unblocked step loop)."""
+    J = np.asarray(J)
+    dtype = np.dtype(dtype or (J.dtype if np.iscomplexobj(J) else np.complex128))
+    rt = _real_of(dtype)
+    n = J.shape[0]
+    m = max(n, 1)
+    Jr, Ji = _f(J.real, rt), _f(J.imag, rt)
+    Ar = np.zeros((m, m), dtype=rt, order="F")
+    Ai = np.zeros_like(Ar)
+    Hr = np.zeros_like(Ar)
+    Hi = np.zeros_like(Ar)
+    ipiv = np.zeros(max(n - 2, 1), dtype=np.int32)
+    perm = np.zeros(m, dtype=np.int32)
+    iperm = np.zeros(m, dtype=np.int32)
+    scale = np.zeros(m, dtype=rt)
+    iscale = np.zeros(m, dtype=rt)
+    if n > 0:
+        getattr(lib(), "lhlo_zreduce" + _sfx(rt))(
+            n, Jr, Ji, m, int(bool(balance)), Ar, Ai, m, ipiv, perm, iperm, scale, iscale, Hr, Hi)
+    A = (Ar[:n, :n] + 1j * Ai[:n, :n]).astype(dtype)
+    Ht = (Hr[:n, :n] + 1j * Hi[:n, :n]).astype(dtype)
+    return ZReduction(np.asfortranarray(A), ipiv[: max(n - 2, 0)].copy(), perm[:n].copy(), iperm[:n].copy(),
+                      scale[:n].copy(), iscale[:n].copy(), np.asfortranarray(Ht))
+
+
+def zbalance(J, dtype=np.complex128):
+    rt = _real_of(dtype)
+    J = np.asarray(J)
+    n = J.shape[0]
+    Ar, Ai = _f(J.real, rt).copy(order="F"), _f(J.imag, rt).copy(order="F")
+    d = np.zeros(max(n, 1), dtype=rt)
+    isc = np.zeros(max(n, 1), dtype=rt)
+    if n:
+        getattr(lib(), "lhlo_zbalance" + _sfx(rt))(n, Ar, Ai, n, d, isc)
+    return (Ar + 1j * Ai).astype(dtype), d[:n], isc[:n]
+
+
+def zshift(Ht, sigma, tau):
+    """lhl_shift! of a complex H (Ht complex, n x n)."""
+    dtype = np.dtype(Ht.dtype)
+    rt = _real_of(dtype)
+    n = Ht.shape[0]
+    m = max(n, 1)
+    sigma, tau = complex(sigma), complex(tau)
+    Gr = np.zeros((m, m), dtype=rt, order="F")
+    Gi = np.zeros_like(Gr)
+    sw = np.zeros(m, dtype=np.uint8)
+    rr = np.zeros(m, dtype=rt)
+    ri = np.zeros(m, dtype=rt)
+    info = 0
+    if n:
+        info = getattr(lib(), "lhlo_zshift" + _sfx(rt))(
+            n, _f(Ht.real, rt), _f(Ht.imag, rt), sigma.real, sigma.imag, tau.real, tau.imag, Gr, Gi, sw, rr, ri)
+    return Shift(Gr[:n, :n].copy(order="F"), sw[:n].copy(), rr[:n].copy(), int(info),
+                 Gi[:n, :n].copy(order="F"), ri[:n].copy())
+
+
+def zldiv(red: ZReduction, sh: Shift, x):
+    """lhl_ldiv! on a complex reduction: returns (sigma I + tau J)^-1 x."""
+    n = red.n
+    rt = _real_of(red.A.dtype)
+    x = np.asarray(x)
+    xr, xi = _vec(x.real, rt).copy(), _vec(x.imag, rt).copy()
+    if n:
+        getattr(lib(), "lhlo_zldiv" + _sfx(rt))(
+            n, _f(red.A.real, rt), _f(red.A.imag, rt), n, red.perm, red.iperm, _vec(red.scale, rt),
+            _vec(red.iscale, rt), _f(sh.Gt, rt), _f(sh.Gt_im, rt), sh.swap, _vec(sh.rdiag, rt),
+            _vec(sh.rdiag_im, rt), xr, xi)
+    return xr + 1j * xi
+
+
+def _zapply(name, red, x, idx, sc):
+    n = red.n
+    rt = _real_of(red.A.dtype)
+    x = np.asarray(x)
+    xr, xi = _vec(x.real, rt).copy(), _vec(x.imag, rt).copy()
+    if n:
+        getattr(lib(), name + _sfx(rt))(n, _f(red.A.real, rt), _f(red.A.imag, rt), n, idx, _vec(sc, rt), xr, xi)
+    return xr + 1j * xi
+
+
+def zapplyZ(red: ZReduction, x):
+    return _zapply("lhlo_zapplyZ", red, x, red.iperm, red.scale)
+
+
+def zapplyZinv(red: ZReduction, x):
+    return _zapply("lhlo_zapplyZinv", red, x, red.perm, red.iscale)
