@@ -177,6 +177,7 @@ int lhl_destroy(lhl_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     prof_drain(h);
     blas_release(h);
+    uc_release(h);
     h->cache.clear();
     for (DevBuf *b : {&h->A, &h->Ht, &h->J, &h->scale64, &h->iscale64, &h->scaleT, &h->iscaleT, &h->ipiv, &h->perm,
                       &h->iperm, &h->part, &h->cand, &h->ctl, &h->counter, &h->flag, &h->rowA, &h->gcol, &h->ilog, &h->brec, &h->bpb, &h->bact, &h->blc, &h->bu12, &h->grp, &h->Linv, &h->Lfull, &h->Lp1, &h->ybuf, &h->rbuf, &h->gbuf,
@@ -828,6 +829,8 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         b->release();
     if (info) for (int64_t s = 0; s < nshift; s++) info[s] = pinfo[s];
     if (!st.ok()) return fail(h, st);
+    for (int64_t s = 0; s < nshift; s++)
+        if (pinfo[s] < 0) return fail(h, LHL_ERR_HIP, "batched: a split shift LU's window hand-off did not arrive");
     if (unconverged > 0)
         return fail(h, LHL_NOT_CONVERGED, "batched: " + std::to_string(unconverged) + " shift(s) still have a column "
                                           "above tol after " + std::to_string(max_iter) + " refinement rounds");
@@ -1272,6 +1275,7 @@ int lhl_plan_run(lhl_plan *p, const double *sigma, const double *tau, int64_t *i
         if (refine && p->p_sact[s]) unconverged++;
         if (info) info[s] = p->p_info[s];
         if (iters) iters[s] = p->p_iters[s];
+        if (p->p_info[s] < 0) return fail(h, LHL_ERR_HIP, "plan: a split shift LU's window hand-off did not arrive");
     }
     // the next run's graph: the rounds this batch needed (never fewer than a
     // graph that already ran without extra rounds)
@@ -1557,7 +1561,7 @@ const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
     {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34}, {"gemm_impl", 0, 1},
-    {"group_bytes", 1, INT64_MAX},
+    {"group_bytes", 1, INT64_MAX},        {"shift_split", 1, 4},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;
@@ -1588,6 +1592,7 @@ int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value) {
     else if (k == "lsweep_depth") h->tune_lsweep = value == -1 ? -1 : v;
     else if (k == "h2d_chunk_bytes") h->tune_h2d_chunk = value;
     else if (k == "gemm_impl") h->tune_gemm = v;
+    else if (k == "shift_split") h->tune_shift_split = value == -1 ? 0 : v;
     return LHL_OK;
 }
 
@@ -1601,6 +1606,7 @@ int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value) {
     else if (k == "lsweep_depth") *value = h->tune_lsweep;
     else if (k == "h2d_chunk_bytes") *value = h->tune_h2d_chunk;
     else if (k == "gemm_impl") *value = h->tune_gemm;
+    else if (k == "shift_split") *value = h->tune_shift_split > 0 ? h->tune_shift_split : -1;
     else return fail(h, LHL_ERR_ARG, "lhl_tune_get: unknown knob " + k);
     return LHL_OK;
 }

@@ -377,6 +377,10 @@ struct lhl_handle {
     int tune_switch = 0;   // step (thousandths of n) where the row-resident loop takes over
     int tune_lsweep = -1;  // halvings of the L sweeps' triangular GEMMs (-1: default, 2)
     int64_t tune_shift_wide = 0;   // n from which the shift LU's bulk is 960 threads wide (0: default)
+    int tune_shift_split = 0;      // most workgroups per real shift LU (0: default 4, 1: one per shift)
+    // uncached device blocks (the split shift LU's hand-off slots); the last is
+    // in use, the others are kept while the handle lives
+    std::vector<std::pair<void *, size_t>> uc;
     int64_t tune_pool_bytes = 0;   // bound on one chunk's packed U's in a batched solve (0: default)
     int64_t tune_h2d_chunk = -1;   // bytes per staged chunk of a host J's upload (-1: default, 0: runtime copy)
     int tune_gemm = -1;            // the solves' dense products: 0 rocBLAS, 1 the library's MFMA kernels (-1: default)
@@ -461,6 +465,10 @@ Status reduce_group_device(lhl_handle *const *hs, int64_t count, int balance);
 // loop takes over: *kstart receives that step (n - 2 when the panels finished)
 Status blocked_device(lhl_handle *h, int64_t ldb, int nb, int rmin, int kswitch, int *kstart);   // lhl_blocked.hip
 Status balance_device(lhl_handle *h);                      // lhl_reduce.hip
+// uncached device memory of at least `bytes` in h->uc.back() (lhl_shift.hip),
+// and the handle's blocks back to the process-wide list (at destroy)
+Status uc_ensure(lhl_handle *h, size_t bytes);
+void uc_release(lhl_handle *h);
 // a complex J (lhl_cplx.hip): the reduction of the handle's planar J, the
 // shift LUs of its complex H (prm: sigma_re, sigma_im, tau_re, tau_im per
 // shift, in T), and the sweeps y_out <- L^-1 y_in (fwd) or L y_in of its

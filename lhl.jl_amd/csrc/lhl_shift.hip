@@ -20,10 +20,48 @@
 #include "lhl_common.h"
 #include "lhl_cdiv.h"
 
+#include <climits>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 namespace lhl {
+
+// Uncached blocks on each device.  Memory of this kind is never freed once a
+// kernel has read it: a handle keeps every block it was given until it is
+// destroyed (a captured graph may hold an older one), and then the blocks go
+// to this list for later handles on the same device.
+namespace {
+std::mutex uc_mu;
+std::vector<std::pair<int, std::pair<void *, size_t>>> uc_free;   // (device, block)
+}  // namespace
+
+Status uc_ensure(lhl_handle *h, size_t bytes) {
+    if (!h->uc.empty() && h->uc.back().second >= bytes) return {};
+    std::lock_guard<std::mutex> lk(uc_mu);
+    size_t best = uc_free.size();
+    for (size_t i = 0; i < uc_free.size(); i++)
+        if (uc_free[i].first == h->device && uc_free[i].second.second >= bytes &&
+            (best == uc_free.size() || uc_free[i].second.second < uc_free[best].second.second))
+            best = i;
+    if (best < uc_free.size()) {
+        h->uc.push_back(uc_free[best].second);
+        uc_free.erase(uc_free.begin() + (long)best);
+        return {};
+    }
+    const size_t want = std::max<size_t>(bytes, size_t(4) << 20);
+    void *p = nullptr;
+    LHL_HIP_TRY(hipExtMallocWithFlags(&p, want, hipDeviceMallocUncached));
+    h->uc.push_back({p, want});
+    return {};
+}
+
+void uc_release(lhl_handle *h) {
+    std::lock_guard<std::mutex> lk(uc_mu);
+    for (auto &b : h->uc) uc_free.push_back({h->device, b});
+    h->uc.clear();
+}
+
 namespace {
 
 constexpr int kShiftThreads = 1024;  // the most threads per shift (the spare slots per U plane)
@@ -219,6 +257,22 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 // reference's order, as in the one-step kernel above, so the results are
 // bit-identical to it.
 
+// A shift split over G workgroups (G = 1: one workgroup per shift): the
+// window values each owner hands to the others' panel waves.  hand and cnt are
+// uncached device memory (coherent across the XCDs without cache
+// maintenance): per shift, nwin windows of KP values and nwin counts of the
+// values that have landed.
+template <typename T> struct SplitArgs {
+    int G = 1, nsh = 0, nwin = 0;
+    T *hand = nullptr;
+    int *cnt = nullptr;
+};
+// polls of a hand-off count before it is given up (about a second; never
+// reached while every workgroup of the shift is resident, which the launch
+// checks), and the default number of workgroups per shift
+constexpr int kSplitSpin = 1 << 20;
+constexpr int kSplitDefault = 4;
+
 // W-column groups: the scalar (W = 1) or a two-element vector (W = 2)
 template <typename T, int W> struct ColGroup;
 template <typename T> struct ColGroup<T, 1> {
@@ -260,7 +314,8 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                                                           T *__restrict__ Ubase, int64_t ustride,
                                                           T *__restrict__ lbase, T *__restrict__ rdbase,
                                                           int64_t lstride, uint8_t *__restrict__ swbase,
-                                                          int64_t sstride, int *__restrict__ infobase) {
+                                                          int64_t sstride, int *__restrict__ infobase,
+                                                          SplitArgs<T> sp) {
 #pragma clang fp contract(off)
     constexpr int kPanel = KP;   // steps per block = window columns
     static_assert(KP <= 64 && kPanel % R == 0 && kPanel % PD == 0, "a block of steps in whole rotations");
@@ -268,7 +323,25 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     using G = ColGroup<T, W>;
     using T2 = typename G::V;
     constexpr int ES = (int)sizeof(T);
-    const int s = blockIdx.x;
+    // the shift and this workgroup's part of it: with sp.G workgroups per
+    // shift, the shift's G workgroups sit on one XCD (workgroups are dealt to
+    // the 8 XCDs in turn) and workgroup g owns the windows w = g (mod G)
+    const int GS = sp.G;
+    int s = blockIdx.x, g = 0;
+    if (GS > 1) {
+        const int q = (int)(blockIdx.x >> 3);
+        s = (int)(blockIdx.x & 7) + 8 * (q / GS);
+        g = q % GS;
+        if (s >= sp.nsh) return;   // padding (whole workgroup)
+    }
+    // owned index o -> column: window g + GS (o / KP), offset o % KP
+    auto col_of = [&](int o) -> int {
+        if (GS == 1) return o;
+        const int wl = o / KP;
+        return (g + GS * wl) * KP + (o - wl * KP);
+    };
+    auto owns = [&](int w) -> bool { return GS == 1 || w % GS == g; };
+    bool failed = false;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
     T *lv = lbase + (int64_t)s * lstride;
@@ -281,6 +354,12 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     __shared__ T lsh[2][64];   // a block's multipliers
     __shared__ int ssh[2][64]; // a block's swap flags
     __shared__ T plast;        // the pending value of column n-1 after the last step
+    // The panel wave's rows of U go to LDS (tile sets by iteration parity) and
+    // the bulk threads store them one block later: a global store in the
+    // panel wave's stream would make each wait for a prefetched row of H also
+    // wait for the stores issued before it (one in-order counter).
+    __shared__ T usq[2][KP][64];   // window_apply: U[k0 + i, c1 + lane]
+    __shared__ T utr[2][KP][64];   // a block's own triangle: U[kb + i, kb + lane], lane >= i
     const int nsteps = n - 1;
 
     // bulk threads: the column groups j .. j+W-1 with j = W (tb + m NTB) of
@@ -289,7 +368,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     int jb[NP];   // byte offset of column j within a row
 #pragma unroll
     for (int m = 0; m < NP; m++) {
-        const int j0 = W * (tb + m * NTB);
+        const int j0 = col_of(W * (tb + m * NTB));
         jb[m] = j0 * ES;
 #pragma unroll
         for (int e = 0; e < W; e++) {
@@ -297,8 +376,18 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             T v = (!is_panel && j < n) ? tau * Ht[j] : (T)0;
             if (j == 0) v = v + sigma;
             G::set(r[m], e, v);
-            if (!is_panel && j < 2 * kPanel) win[j / kPanel][j % kPanel] = v;   // windows 0 and 1
-            if (!is_panel && j == 0 && n == 1) plast = v;
+        }
+    }
+    // windows 0 and 1 (every workgroup of the shift forms them, as the bulk
+    // forms its columns)
+    if (is_panel) {
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+            const int j = w * kPanel + lane;
+            T v = (lane < kPanel && j < n) ? tau * Ht[j] : (T)0;
+            if (j == 0) v = v + sigma;
+            win[w][lane] = v;
+            if (j == 0 && n == 1) plast = v;
         }
     }
     // rows of H for the bulk steps, R steps ahead in rotation (a group's
@@ -314,7 +403,11 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
 #pragma unroll
         for (int q = 0; q < R; q++) load_rows(1 + q, hs[q], 0);
     }
-    __syncthreads();
+    // Barriers in this kernel order LDS only (windows, multipliers, swap flags,
+    // plast): no wave reads global memory another wave writes, so loads in
+    // flight (the rows of H prefetched across a block boundary) and U stores
+    // stay outstanding through them.
+    lds_barrier();
 
     // ---- panel wave state: the current window (lane L = column c0 + L)
     int info = 0;
@@ -330,7 +423,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // decisions of block k0 on window k0 (p holds it): steps k0 .. k0+nb-1.  A
     // full block (nb = kPanel) is straight-line code, so the waits for the
     // prefetched rows count the stores between them exactly.
-    auto panel_nb = [&](int k0, int nb, auto fullc) __attribute__((always_inline)) {
+    auto panel_nb = [&](int k0, int nb, bool wr, int ip, auto fullc) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(fullc)::value;
         T uvec = 1;
 #pragma unroll
@@ -361,8 +454,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T u = swk ? (first ? g + sigma : g) : p;
                 const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
                 // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
-                const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
-                buf_st_nt(u, ur, (lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
+                if (wr && lane >= i && lane < kPanel) utr[ip][i][lane] = u;
                 lvec = lane == i ? l : lvec;
                 svec = lane == i ? (int)swk : svec;
                 uvec = lane == i ? (swk ? b : a) : uvec;
@@ -374,23 +466,25 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         if (lane < nb) {
             lsh[bp][lane] = lvec;
             ssh[bp][lane] = svec;
-            lv[k0 + lane] = lvec;
-            sw[k0 + lane] = (uint8_t)svec;
-            rd[k0 + lane] = (T)1 / uvec;
+            if (wr) {
+                lv[k0 + lane] = lvec;
+                sw[k0 + lane] = (uint8_t)svec;
+                rd[k0 + lane] = (T)1 / uvec;
+            }
         }
     };
-    auto panel = [&](int k0, int nb) __attribute__((always_inline)) {
-        if (nb == kPanel) panel_nb(k0, nb, std::true_type{});
-        else panel_nb(k0, nb, std::false_type{});
+    // the owner of the window writes its part of U and the block's records
+    auto panel = [&](int k0, int nb, int ip) __attribute__((always_inline)) {
+        const bool wr = owns(k0 / kPanel);
+        if (nb == kPanel) panel_nb(k0, nb, wr, ip, std::true_type{});
+        else panel_nb(k0, nb, wr, ip, std::false_type{});
     };
     // block k0 (full, multipliers in lvec/svec) applied to window c1 = k0+kPanel,
     // whose pending values are in p; the last step's new row has its
     // diagonal (sigma) in column c1
-    auto window_apply = [&](int k0) __attribute__((always_inline)) {
+    auto window_apply = [&](int k0, int ip) __attribute__((always_inline)) {
         const int c1 = k0 + kPanel;
-        const int off = lane < kPanel ? lane * ES : kBufNone;
-        const int ubytes = (n - c1) * ES;
-        int64_t uo = upack_off(n, k0);
+        const bool st = owns(c1 / kPanel) && lane < kPanel;
 #pragma unroll
         for (int i = 0; i < PD; i++) load_w(k0, c1, i, hw[i]);
 #pragma nounroll
@@ -407,30 +501,103 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const T rn = g - l * p;
             const T u = swk ? (first ? g + sigma : g) : p;
             p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
-            const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
-            uo += n - k;
-            buf_st_nt(u, ur, off);
+            if (st) usq[ip][i][lane] = u;
             load_w(k0, c1, i + PD, hw[ii]);
+        }
+    };
+
+    // block k0 applied to the window at column c >= k0 + 2 kPanel (a window two
+    // ahead, handed over one block early): no diagonal and no stores; the
+    // block's multipliers from LDS.  Each element takes the bulk's operations.
+    auto window_apply_far = [&](int k0, int c) __attribute__((always_inline)) {
+        const int bq = (k0 / kPanel) & 1;
+        const T lvv = lsh[bq][lane];
+        const int svv = ssh[bq][lane];
+#pragma unroll
+        for (int i = 0; i < PD; i++) load_w(k0, c, i, hw[i]);
+#pragma nounroll
+        for (int ib = 0; ib < kPanel; ib += PD)
+#pragma unroll
+        for (int ii = 0; ii < PD; ii++) {
+            const int i = ib + ii;
+            const T l = rdlane(lvv, i);
+            const bool swk = __builtin_amdgcn_readlane(svv, i) != 0;
+            const T g = tau * hw[ii];
+            const T rs = p - l * g;
+            const T rn = g - l * p;
+            p = swk ? rs : rn;
+            load_w(k0, c, i + PD, hw[ii]);
+        }
+    };
+
+    // The panel wave's decisions are the kernel's critical path (a chain of
+    // dependent steps); the bulk waves beside it on its SIMD would otherwise
+    // take most of the issue slots.  It runs at the highest wave priority.
+    if (is_panel) __builtin_amdgcn_s_setprio(3);
+
+    // the bulk threads' store of the panel rows made in iteration b - 1 (the
+    // rows of window b: window_apply's square and block b's triangle), by the
+    // owner of window b; b = 0: block 0's triangle (made before the loop)
+    auto flush = [&](int b) __attribute__((always_inline)) {
+        const int cb = b * kPanel;
+        if (cb >= n || !owns(b)) return;
+        const int q = (b + 1) & 1;            // the set of iteration b - 1
+        const int ncol = min(kPanel, n - cb);
+        const int nb = cb < nsteps ? min(kPanel, nsteps - cb) : 0;
+        const int wv = tb >> 6, nwv = NTB / 64;
+        for (int rr = wv; rr < 2 * kPanel; rr += nwv) {
+            if (rr < kPanel) {
+                if (b < 1) continue;
+                const int k = cb - kPanel + rr;   // a row of block b - 1
+                if (lane < ncol) U[upack_off(n, k) + (cb - k) + lane] = usq[q][rr][lane];
+            } else {
+                const int i = rr - kPanel, k = cb + i;
+                if (i < nb && lane >= i && lane < ncol) U[upack_off(n, k) + (lane - i)] = utr[q][i][lane];
+            }
         }
     };
 
     if (is_panel && nsteps > 0) {
         p = win[0][lane];
-        panel(0, min(kPanel, nsteps));
+        panel(0, min(kPanel, nsteps), 1);
         if (nsteps < kPanel && lane == nsteps) plast = p;   // one partial block: column n-1 is in window 0
     }
-    __syncthreads();
+    lds_barrier();
     for (int k0 = 0; k0 < nsteps; k0 += kPanel) {
         const int bp = (k0 / kPanel) & 1;
         const int c1 = k0 + kPanel;     // window b+1
         const int c2 = c1 + kPanel;     // the bulk's first column
+        if (!is_panel) flush(k0 / kPanel);
         if (is_panel) {
             if (c1 < n) {   // block b is full
-                p = win[bp ^ 1][lane];
-                window_apply(k0);
+                const int w1 = c1 / kPanel;
+                if (w1 < 2 || owns(w1)) {
+                    p = win[bp ^ 1][lane];
+                } else {
+                    // window b+1 of another workgroup: its state after blocks
+                    // <= b-2 (window 2: its initial values; later ones from the
+                    // owner, published at the end of its bulk pass of block b-2,
+                    // a block before it is needed), then block b-1 applied here
+                    if (w1 == 2) {
+                        const int j = c1 + lane;
+                        p = (lane < kPanel && j < n) ? tau * Ht[j] : (T)0;
+                    } else {
+                        const int nval = min(kPanel, n - c1);
+                        int *cnt = sp.cnt + (int64_t)s * sp.nwin + w1;
+                        for (int it = 0;
+                             !failed && __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nval; it++) {
+                            if (it >= kSplitSpin) failed = true;
+                            __builtin_amdgcn_s_sleep(1);
+                        }
+                        const volatile T *src = sp.hand + ((int64_t)s * sp.nwin + w1) * kPanel;
+                        p = (!failed && lane < nval) ? src[lane] : (T)0;
+                    }
+                    window_apply_far(k0 - kPanel, c1);
+                }
+                window_apply(k0, bp);
                 if (c1 < nsteps) {
                     const int nb1 = min(kPanel, nsteps - c1);
-                    panel(c1, nb1);
+                    panel(c1, nb1, bp);
                     // a partial last block: column n-1 is in window b+1 (a full one
                     // leaves it to the next window's pass)
                     if (c1 + nb1 == nsteps && nb1 < kPanel && lane == nb1) plast = p;
@@ -449,7 +616,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 int boff[NP];
 #pragma unroll
                 for (int m = MS; m < NP; m++) {
-                    const int j0 = W * (tb + m * NTB);
+                    const int j0 = col_of(W * (tb + m * NTB));
                     boff[m] = j0 >= c2 ? (j0 - c2) * ES : kBufNone;
                 }
                 const T lvv = lsh[bp][lane];
@@ -488,28 +655,64 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                         load_rows(k + R + 1, hs[q], MS);
                     }
                 }
-                // window b+2 = columns c2 .. c2+63 for the panel wave
+                // window b+2 = columns c2 .. c2+63 for this workgroup's panel
+                // wave, and with several workgroups per shift window b+3 for the
+                // others' (uncached slots, then a count of the values written
+                // once they have landed: they apply block b+1 to it themselves)
+                if (owns(c2 / kPanel)) {
 #pragma unroll
-                for (int m = MS; m < NP; m++)
+                    for (int m = MS; m < NP; m++)
 #pragma unroll
-                    for (int e = 0; e < W; e++) {
-                        const int j = W * (tb + m * NTB) + e;
-                        if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = G::get(r[m], e);
+                        for (int e = 0; e < W; e++) {
+                            const int j = col_of(W * (tb + m * NTB)) + e;
+                            if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = G::get(r[m], e);
+                        }
+                }
+                const int c3 = c2 + kPanel;
+                if (GS > 1 && c3 < n && owns(c3 / kPanel)) {
+                    int put = 0;
+                    volatile T *dst = sp.hand + ((int64_t)s * sp.nwin + c3 / kPanel) * kPanel;
+#pragma unroll
+                    for (int m = MS; m < NP; m++)
+#pragma unroll
+                        for (int e = 0; e < W; e++) {
+                            const int j = col_of(W * (tb + m * NTB)) + e;
+                            if (j >= c3 && j < c3 + kPanel && j < n) {
+                                dst[j - c3] = G::get(r[m], e);
+                                put++;
+                            }
+                        }
+                    if (put > 0) {
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        __builtin_amdgcn_s_waitcnt(0);   // the values have landed
+                        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                        __hip_atomic_fetch_add(sp.cnt + (int64_t)s * sp.nwin + c3 / kPanel, put, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                     }
+                }
             };
-            dispatch_const<0, NP>(c2 / (W * NTB), bulk);
+            // the first group chunk holding a column >= c2: owned windows of
+            // index >= ceil((b + 2 - g) / G)
+            const int wl0 = GS == 1 ? c2 / kPanel : max(0, (c2 / kPanel - g + GS - 1) / GS);
+            dispatch_const<0, NP>(wl0 * kPanel / (W * NTB), bulk);
         }
-        __syncthreads();
+        lds_barrier();
     }
+    // the last iteration's panel rows
+    if (!is_panel && nsteps > 0) flush((nsteps + kPanel - 1) / kPanel);
     // the last diagonal: U[n-1, n-1] = p[n-1]
-    if (tid == 0) {
+    if (tid == 0 && g == 0) {
         const T pl = plast;
         U[upack_off(n, n - 1)] = pl;
         rd[n - 1] = (T)1 / pl;
         sw[n - 1] = 0;
         if (pl == (T)0 && info == 0) info = n;
-        infobase[s] = info;
+        if (GS == 1) infobase[s] = info;
+        else atomicMin(infobase + s, info);   // the reset left INT_MAX
     }
+    // a hand-off that did not arrive (a bound, never reached when every
+    // workgroup of the shift is resident): info -1, reported as an error
+    if (GS > 1 && lane == 0 && is_panel && failed) atomicMin(infobase + s, -1);
 }
 
 // ------------------------------------------------------------- complex shifts
@@ -681,18 +884,87 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
 constexpr int kPanelBulk = 448;
 constexpr int kPanelBulkWide = 960;
 template <typename T> constexpr int panel_w() { return sizeof(T) == 8 ? 2 : 1; }
-template <typename T, int NP, int NTB = kPanelBulk>
-Status launch_panel(lhl_handle *h, int64_t nshift, const T *Ht, int64_t ldh, int n, const ShiftParam<T> *prm,
-                    T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
+template <typename T> constexpr int panel_kp() { return sizeof(T) == 8 ? 64 : 63; }
+template <typename T, int NP, int NTB> constexpr auto panel_fn() {
     constexpr bool F64 = sizeof(T) == 8;
-    auto fn = k_shift_panel<T, NP, NTB, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
-    LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(64 + NTB), 0, Ht, ldh, n, prm, U, us, l, rd, ls,
-               sw, ss, info);
+    return k_shift_panel<T, NP, NTB, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
+}
+
+// Workgroups of one panel-kernel shape resident on the whole device at once
+// (queried once per shape; 0 if the query fails, which turns the split off).
+template <typename T, int NP, int NTB> int panel_capacity() {
+    static const int cap = [] {
+        int nb = 0, dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, panel_fn<T, NP, NTB>(), 64 + NTB, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return nb * cus;
+    }();
+    return cap;
+}
+
+template <typename T, int NP, int NTB>
+Status launch_panel(lhl_handle *h, int64_t nshift, const T *Ht, int64_t ldh, int n, const ShiftParam<T> *prm,
+                    T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info,
+                    const SplitArgs<T> &sp) {
+    const unsigned grid = sp.G > 1 ? 8u * (unsigned)((nshift + 7) / 8) * (unsigned)sp.G : (unsigned)nshift;
+    LHL_LAUNCH(h, "shift", (panel_fn<T, NP, NTB>()), dim3(grid), dim3(64 + NTB), 0, Ht, ldh, n, prm, U, us, l, rd,
+               ls, sw, ss, info, sp);
     return {};
 }
 
-// Real shifts: the panel kernel up to n = 13440, its column groups per bulk
-// thread and its bulk width chosen from n.
+// f(NP, NTB) for the panel kernel's shape that holds `ncols` columns per
+// workgroup: 448 bulk threads while they reach, 960 beyond (or from the
+// tuning knob's n on, `wide`)
+template <typename T, typename F> Status with_panel_shape(int ncols, bool wide, F &&f) {
+    using std::integral_constant;
+    constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
+    constexpr int BW = panel_w<T>() * kPanelBulkWide;
+    if constexpr (sizeof(T) == 4) {
+        if (wide) {
+            if (ncols <= 10 * BW) return f(integral_constant<int, 10>{}, integral_constant<int, kPanelBulkWide>{});
+            return f(integral_constant<int, 14>{}, integral_constant<int, kPanelBulkWide>{});
+        }
+        if (ncols <= B) return f(integral_constant<int, 1>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 2 * B) return f(integral_constant<int, 2>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 4 * B) return f(integral_constant<int, 4>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 6 * B) return f(integral_constant<int, 6>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 10 * B) return f(integral_constant<int, 10>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 14 * B) return f(integral_constant<int, 14>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 19 * B) return f(integral_constant<int, 19>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 10 * BW) return f(integral_constant<int, 10>{}, integral_constant<int, kPanelBulkWide>{});
+        return f(integral_constant<int, 14>{}, integral_constant<int, kPanelBulkWide>{});
+    } else {
+        if (wide) {
+            if (ncols <= 3 * BW) return f(integral_constant<int, 3>{}, integral_constant<int, kPanelBulkWide>{});
+            if (ncols <= 5 * BW) return f(integral_constant<int, 5>{}, integral_constant<int, kPanelBulkWide>{});
+            return f(integral_constant<int, 7>{}, integral_constant<int, kPanelBulkWide>{});
+        }
+        if (ncols <= B) return f(integral_constant<int, 1>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 2 * B) return f(integral_constant<int, 2>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 3 * B) return f(integral_constant<int, 3>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 5 * B) return f(integral_constant<int, 5>{}, integral_constant<int, kPanelBulk>{});
+        if (ncols <= 3 * BW) return f(integral_constant<int, 3>{}, integral_constant<int, kPanelBulkWide>{});
+        if (ncols <= 5 * BW) return f(integral_constant<int, 5>{}, integral_constant<int, kPanelBulkWide>{});
+        return f(integral_constant<int, 7>{}, integral_constant<int, kPanelBulkWide>{});
+    }
+}
+
+// counts of landed window values to 0, info to INT_MAX (the split kernel
+// lowers it with atomicMin)
+__global__ void k_split_reset(int *__restrict__ cnt, int64_t ncnt, int *__restrict__ info, int64_t nshift) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncnt) cnt[i] = 0;
+    if (i < nshift) info[i] = INT_MAX;
+}
+
+// Real shifts: the panel kernel up to n = 13440, split over G workgroups per
+// shift when they can all be resident (G from 4 down; the knob shift_split
+// caps it, 1 turns it off), its column groups per bulk thread and its bulk
+// width chosen from the columns each workgroup holds.
 // Otherwise (and complex shifts) the one-step kernels: 512 threads up to
 // n = 8192, 1024 beyond.
 template <typename T>
@@ -702,49 +974,45 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     static_assert(sizeof(ShiftParam<T>) == 4 * sizeof(T), "ShiftParam is four T: (sr, si, tr, ti)");
     if (h->cplx)   // a complex H (a complex J's reduction)
         return zshift_launch<T>(h, nshift, reinterpret_cast<const T *>(prm), U, us, l, rd, ls, sw, ss, info);
-    constexpr int B = panel_w<T>() * kPanelBulk;   // columns per group chunk
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int BW = panel_w<T>() * kPanelBulkWide;
+    constexpr int KP = panel_kp<T>();
     // the tuning knob: the 960-thread bulk from n = tune_shift_wide on
     const bool wide = h->tune_shift_wide > 0 && n >= h->tune_shift_wide;
-    if (!cplx && wide && n <= (F32 ? 14 : 7) * BW) {
-        if constexpr (F32) {
-            if (n <= 10 * BW)
-                LHL_TRY((launch_panel<T, 10, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 14, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        } else {
-            if (n <= 3 * BW)
-                LHL_TRY((launch_panel<T, 3, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 5 * BW)
-                LHL_TRY((launch_panel<T, 5, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 7, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        }
-        LHL_HIP_TRY(hipGetLastError());
-        return {};
-    }
     if (!cplx && n <= (F32 ? 14 : 7) * BW) {
-        if constexpr (F32) {
-            if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 4 * B) LHL_TRY((launch_panel<T, 4>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 6 * B) LHL_TRY((launch_panel<T, 6>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 10 * B) LHL_TRY((launch_panel<T, 10>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 14 * B) LHL_TRY((launch_panel<T, 14>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 19 * B) LHL_TRY((launch_panel<T, 19>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 10 * BW)
-                LHL_TRY((launch_panel<T, 10, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 14, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-        } else {
-            if (n <= B) LHL_TRY((launch_panel<T, 1>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 2 * B) LHL_TRY((launch_panel<T, 2>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 3 * B) LHL_TRY((launch_panel<T, 3>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 5 * B) LHL_TRY((launch_panel<T, 5>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 3 * BW)
-                LHL_TRY((launch_panel<T, 3, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else if (n <= 5 * BW)
-                LHL_TRY((launch_panel<T, 5, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-            else LHL_TRY((launch_panel<T, 7, kPanelBulkWide>(h, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+        SplitArgs<T> sp;
+        sp.nsh = (int)nshift;
+        sp.nwin = (n + KP - 1) / KP;
+        const int gmax = h->tune_shift_split > 0 ? h->tune_shift_split : kSplitDefault;
+        for (int gc = std::min(gmax, 4); gc > 1; gc /= 2) {
+            if (sp.nwin < 2 * gc + 2) continue;   // too few windows to share
+            const int ncols = KP * ((sp.nwin + gc - 1) / gc);
+            int cap = 0;
+            (void)with_panel_shape<T>(ncols, wide, [&](auto np, auto ntb) -> Status {
+                cap = panel_capacity<T, decltype(np)::value, decltype(ntb)::value>();
+                return {};
+            });
+            if (8 * ((nshift + 7) / 8) * gc > cap) continue;
+            const size_t hb = ((sizeof(T) * (size_t)nshift * sp.nwin * KP + 255) / 256) * 256;
+            if (!uc_ensure(h, hb + sizeof(int) * (size_t)nshift * sp.nwin).ok()) {
+                (void)hipGetLastError();
+                break;
+            }
+            sp.G = gc;
+            sp.hand = static_cast<T *>(h->uc.back().first);
+            sp.cnt = reinterpret_cast<int *>(static_cast<char *>(h->uc.back().first) + hb);
+            break;
         }
+        if (sp.G > 1) {
+            const int64_t nc = nshift * sp.nwin;
+            LHL_LAUNCH(h, "shift", k_split_reset, dim3((unsigned)((std::max<int64_t>(nc, nshift) + 255) / 256)),
+                       dim3(256), 0, sp.cnt, nc, info, nshift);
+        }
+        const int ncols = sp.G > 1 ? KP * ((sp.nwin + sp.G - 1) / sp.G) : n;
+        LHL_TRY(with_panel_shape<T>(ncols, wide, [&](auto np, auto ntb) -> Status {
+            return launch_panel<T, decltype(np)::value, decltype(ntb)::value>(h, nshift, Ht, ldh, n, prm, U, us, l,
+                                                                               rd, ls, sw, ss, info, sp);
+        }));
         LHL_HIP_TRY(hipGetLastError());
         return {};
     }
@@ -876,6 +1144,7 @@ Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t /*nshift*/, cons
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         params.release();
         if (e != hipSuccess) return {LHL_ERR_HIP, std::string("lhl_shift: ") + hipGetErrorString(e)};
+        if (info < 0) return {LHL_ERR_HIP, "lhl_shift: a split shift LU's window hand-off did not arrive"};
     }
     sh->info = info;
     sh->sigma[0] = sig[0]; sh->sigma[1] = sig[1];

@@ -50,6 +50,50 @@ def test_wide_shift_bulk_bitwise(lhl_hip, oracle, monkeypatch, dtype):
             assert np.array_equal(ws.shift.rdiag, ref.rdiag)
 
 
+# the split shift LU: G workgroups per shift, windows of 63 (FP32) / 64 (FP64)
+# columns dealt round-robin; sizes on both sides of window boundaries and of
+# the owned-column shapes, one shift at a time and batched
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_split_shift_bitwise(lhl_hip, oracle, monkeypatch, dtype):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    kp = 64 if dtype == np.float64 else 63
+    for n in (6 * kp, 6 * kp + 1, 10 * kp - 1, 10 * kp, 10 * kp + 1, 2500, 4096, 6000):
+        J = randn(n + 3, n, dtype=dtype)
+        ws = lhl_hip.LHLWorkspace(n, dtype)
+        lhl_hip.lhl_(ws, J)
+        Ht = ws.Ht
+        for split in (1, 2, 4):
+            say(f"case: split shift {np.dtype(dtype).name} n={n} split={split}")
+            lhl_hip.tune_set(ws, "shift_split", split)
+            for s, t in ((1, -0.1), (0.5, -3)):
+                lhl_hip.lhl_shift_(ws, s, t)
+                ref = oracle.shift(Ht, s, t)
+                assert ws.info == ref.info
+                assert np.array_equal(ws.shift.swap, ref.swap.astype(bool))
+                assert np.array_equal(ws.shift.U, ref.U())
+                assert np.array_equal(ws.shift.multipliers, ref.mult())
+                assert np.array_equal(ws.shift.rdiag, ref.rdiag)
+
+
+def test_split_shift_batched_equals_unsplit(lhl_hip, monkeypatch):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    n, S, R = 2048, 24, 3
+    say(f"case: split shift batched n={n} S={S}")
+    J = stiff(5, n)
+    gam = np.logspace(-4, -1, S)
+    offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+    B = np.asfortranarray(np.random.default_rng(9).standard_normal((n, S * R)))
+    out = {}
+    for split in (1, 4):
+        ws = lhl_hip.LHLWorkspace(n, np.float32)
+        lhl_hip.tune_set(ws, "shift_split", split)
+        lhl_hip.lhl_(ws, J)
+        out[split] = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs, precision="refine64", tol=1e-12,
+                                                 max_iter=10)
+    assert np.array_equal(out[1][0], out[4][0])
+    assert np.array_equal(np.asarray(out[1][1]), np.asarray(out[4][1]))
+
+
 def test_reduce_knobs_keep_pivots(lhl_hip, oracle, monkeypatch):
     monkeypatch.setenv("LHL_TUNE", "0")
     n = 600
