@@ -591,16 +591,20 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
         }
     };
     VT uA[MB], iA[UC ? MB : 1], uB[MB], iB[UC ? MB : 1];
+    // the diagonal block inverse of block bb, one block ahead: the next
+    // block's is loaded while this block's update runs (its latency was
+    // exposed at every block start when loaded there)
+    T pre[NPRE];
+    auto load_pre = [&](int bb, T(&dst)[NPRE]) __attribute__((always_inline)) {
+        const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)bb * kT * kT * UP;
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) dst[u] = Ui[min(tid + NT * u, kT * kT * UP - 1)];
+    };
+    load_pre(nbu - 1, pre);
     load_at(nbu - 1, 0, uA, iA);
     for (int b = nbu - 1; b >= 0; b--) {
         const int rb = b * kT, nbk = min(kT, n - rb);
         const int gb = rb / V;                 // the block's first group; groups < gb are above it
-        T pre[NPRE];
-        {
-            const T *Ui = Uinv + (int64_t)g.shift * uistride + (int64_t)b * kT * kT * UP;
-#pragma unroll
-            for (int u = 0; u < NPRE; u++) pre[u] = Ui[min(tid + NT * u, kT * kT * UP - 1)];
-        }
         // Step 3 below (groups above the block: y_i -= U[i, rb:rb+nbk] x_b) is a
         // sequence of batches t (slot t/NB, columns MB (t%NB) .. +MB-1); the
         // loads of batch t+1 are in flight while batch t's FMAs run, and the
@@ -630,6 +634,7 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
             const int pl = e / (kT * kT), ik = e % (kT * kT);
             if (e < kT * kT * UP) UiS[pl][ik / kT][ik % kT] = pre[u];
         }
+        load_pre(max(b - 1, 0), pre);   // block b-1's (block 0: a repeat, unused)
         lds_barrier();   // LDS only: the early U loads stay in flight
         // 2. x_b = Uinv_b y_b
         if (tid < kT * kNC) {

@@ -259,9 +259,9 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 
 // A shift split over G workgroups (G = 1: one workgroup per shift): the
 // window values each owner hands to the others' panel waves.  hand and cnt are
-// uncached device memory (coherent across the XCDs without cache
-// maintenance): per shift, nwin windows of KP values and nwin counts of the
-// values that have landed.
+// uncached device memory, every access to them a device-scope atomic (coherent
+// across the XCDs wherever the workgroups run): per shift, nwin windows of KP
+// values and nwin counts of the values that have landed.
 template <typename T> struct SplitArgs {
     int G = 1, nsh = 0, nwin = 0;
     T *hand = nullptr;
@@ -354,12 +354,18 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     __shared__ T lsh[2][64];   // a block's multipliers
     __shared__ int ssh[2][64]; // a block's swap flags
     __shared__ T plast;        // the pending value of column n-1 after the last step
-    // The panel wave's rows of U go to LDS (tile sets by iteration parity) and
-    // the bulk threads store them one block later: a global store in the
-    // panel wave's stream would make each wait for a prefetched row of H also
-    // wait for the stores issued before it (one in-order counter).
-    __shared__ T usq[2][KP][64];   // window_apply: U[k0 + i, c1 + lane]
-    __shared__ T utr[2][KP][64];   // a block's own triangle: U[kb + i, kb + lane], lane >= i
+    // The panel wave reads the rows of H for its steps from LDS tiles that the
+    // bulk threads stage one block ahead, straight from global memory into LDS
+    // (no registers), so the decision chain never waits on global memory (with
+    // rows prefetched into registers it ran at the load latency over the
+    // prefetch depth).  Per iteration parity: A (window_apply: H[k0+1+i,
+    // c1+lane]), B (the next block's decisions: H[c1+1+i, c1+lane]) and, where
+    // windows are handed over two blocks ahead (FP32), C (block b-1 on window
+    // b+1: H[k0-KP+1+i, c1+lane]).  FP64 hands windows over one block ahead
+    // (its three tile pairs would not fit in LDS).
+    constexpr bool FAR = sizeof(T) == 4;
+    constexpr int NTL = FAR ? 3 : 2;
+    __shared__ __align__(16) T htile[2][NTL][KP][64];
     const int nsteps = n - 1;
 
     // bulk threads: the column groups j .. j+W-1 with j = W (tb + m NTB) of
@@ -415,19 +421,19 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     T lvec = 0;                    // the last panel block's multipliers (lane i: step i)
     int svec = 0;                  // and swap flags
     T hw[PD];
-    // H[k0+i+1, cb+lane] for the steps i of a block, PD ahead
-    auto load_w = [&](int k0, int cb, int i, T &hd) __attribute__((always_inline)) {
-        const BufRsrc rs = buf_rsrc(Ht + (int64_t)min(k0 + i + 1, n - 1) * ldh, n * ES);
-        hd = buf_ld<T>(rs, (cb + lane) * ES);
+    // row i of a staged tile (H[r0+i, c+lane]), PD ahead
+    auto load_t = [&](const T *tl, int i, T &hd) __attribute__((always_inline)) {
+        hd = tl[min(i, kPanel - 1) * 64 + lane];
     };
     // decisions of block k0 on window k0 (p holds it): steps k0 .. k0+nb-1.  A
     // full block (nb = kPanel) is straight-line code, so the waits for the
     // prefetched rows count the stores between them exactly.
-    auto panel_nb = [&](int k0, int nb, bool wr, int ip, auto fullc) __attribute__((always_inline)) {
+    auto panel_nb = [&](int k0, int nb, bool wr, int set, auto fullc) __attribute__((always_inline)) {
         constexpr bool FULL = decltype(fullc)::value;
+        const T *tl = &htile[set][1][0][0];
         T uvec = 1;
 #pragma unroll
-        for (int i = 0; i < PD; i++) load_w(k0, k0, i, hw[i]);
+        for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
 #pragma nounroll
         for (int ib = 0; ib < kPanel; ib += PD)
 #pragma unroll
@@ -454,13 +460,14 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T u = swk ? (first ? g + sigma : g) : p;
                 const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
                 // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
-                if (wr && lane >= i && lane < kPanel) utr[ip][i][lane] = u;
+                const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
+                buf_st_nt(u, ur, (wr && lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
                 lvec = lane == i ? l : lvec;
                 svec = lane == i ? (int)swk : svec;
                 uvec = lane == i ? (swk ? b : a) : uvec;
                 p = pn;   // lanes <= i are finished: their value is no longer read
             }
-            load_w(k0, k0, i + PD, hw[ii]);   // past the block: a clamped row, unused
+            load_t(tl, i + PD, hw[ii]);   // past the block: a clamped row, unused
         }
         const int bp = (k0 / kPanel) & 1;
         if (lane < nb) {
@@ -474,19 +481,22 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         }
     };
     // the owner of the window writes its part of U and the block's records
-    auto panel = [&](int k0, int nb, int ip) __attribute__((always_inline)) {
+    auto panel = [&](int k0, int nb, int set) __attribute__((always_inline)) {
         const bool wr = owns(k0 / kPanel);
-        if (nb == kPanel) panel_nb(k0, nb, wr, ip, std::true_type{});
-        else panel_nb(k0, nb, wr, ip, std::false_type{});
+        if (nb == kPanel) panel_nb(k0, nb, wr, set, std::true_type{});
+        else panel_nb(k0, nb, wr, set, std::false_type{});
     };
     // block k0 (full, multipliers in lvec/svec) applied to window c1 = k0+kPanel,
     // whose pending values are in p; the last step's new row has its
     // diagonal (sigma) in column c1
-    auto window_apply = [&](int k0, int ip) __attribute__((always_inline)) {
+    auto window_apply = [&](int k0, int set) __attribute__((always_inline)) {
         const int c1 = k0 + kPanel;
-        const bool st = owns(c1 / kPanel) && lane < kPanel;
+        const int off = (owns(c1 / kPanel) && lane < kPanel) ? lane * ES : kBufNone;
+        const int ubytes = (n - c1) * ES;
+        int64_t uo = upack_off(n, k0);
+        const T *tl = &htile[set][0][0][0];
 #pragma unroll
-        for (int i = 0; i < PD; i++) load_w(k0, c1, i, hw[i]);
+        for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
 #pragma nounroll
         for (int ib = 0; ib < kPanel; ib += PD)
 #pragma unroll
@@ -501,20 +511,23 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const T rn = g - l * p;
             const T u = swk ? (first ? g + sigma : g) : p;
             p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
-            if (st) usq[ip][i][lane] = u;
-            load_w(k0, c1, i + PD, hw[ii]);
+            const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
+            uo += n - k;
+            buf_st_nt(u, ur, off);
+            load_t(tl, i + PD, hw[ii]);
         }
     };
 
     // block k0 applied to the window at column c >= k0 + 2 kPanel (a window two
     // ahead, handed over one block early): no diagonal and no stores; the
     // block's multipliers from LDS.  Each element takes the bulk's operations.
-    auto window_apply_far = [&](int k0, int c) __attribute__((always_inline)) {
+    auto window_apply_far = [&](int k0, int set) __attribute__((always_inline)) {
         const int bq = (k0 / kPanel) & 1;
         const T lvv = lsh[bq][lane];
         const int svv = ssh[bq][lane];
+        const T *tl = &htile[set][NTL - 1][0][0];
 #pragma unroll
-        for (int i = 0; i < PD; i++) load_w(k0, c, i, hw[i]);
+        for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
 #pragma nounroll
         for (int ib = 0; ib < kPanel; ib += PD)
 #pragma unroll
@@ -526,7 +539,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const T rs = p - l * g;
             const T rn = g - l * p;
             p = swk ? rs : rn;
-            load_w(k0, c, i + PD, hw[ii]);
+            load_t(tl, i + PD, hw[ii]);
         }
     };
 
@@ -535,27 +548,45 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // take most of the issue slots.  It runs at the highest wave priority.
     if (is_panel) __builtin_amdgcn_s_setprio(3);
 
-    // the bulk threads' store of the panel rows made in iteration b - 1 (the
-    // rows of window b: window_apply's square and block b's triangle), by the
-    // owner of window b; b = 0: block 0's triangle (made before the loop)
-    auto flush = [&](int b) __attribute__((always_inline)) {
-        const int cb = b * kPanel;
-        if (cb >= n || !owns(b)) return;
-        const int q = (b + 1) & 1;            // the set of iteration b - 1
-        const int ncol = min(kPanel, n - cb);
-        const int nb = cb < nsteps ? min(kPanel, nsteps - cb) : 0;
-        const int wv = tb >> 6, nwv = NTB / 64;
-        for (int rr = wv; rr < 2 * kPanel; rr += nwv) {
-            if (rr < kPanel) {
-                if (b < 1) continue;
-                const int k = cb - kPanel + rr;   // a row of block b - 1
-                if (lane < ncol) U[upack_off(n, k) + (cb - k) + lane] = usq[q][rr][lane];
-            } else {
-                const int i = rr - kPanel, k = cb + i;
-                if (i < nb && lane >= i && lane < ncol) U[upack_off(n, k) + (lane - i)] = utr[q][i][lane];
+    // bulk threads: stage the tiles of iteration b (into set b & 1), one row
+    // per load instruction per wave (FP64: two dword halves of a row)
+    typedef __attribute__((address_space(1))) void gvoid_t;
+    typedef __attribute__((address_space(3))) void lvoid_t;
+    const int wv = tb >> 6, nwv = NTB / 64;
+    auto stage_row = [&](int set, int t, int r0, int c, int i) __attribute__((always_inline)) {
+        const T *src = Ht + (int64_t)min(r0 + i, n - 1) * ldh;
+        char *dst = reinterpret_cast<char *>(&htile[set][t][i][0]);
+        if constexpr (sizeof(T) == 4) {
+            __builtin_amdgcn_global_load_lds((gvoid_t *)(src + min(c + lane, n - 1)), (lvoid_t *)dst, 4, 0, 0);
+        } else {
+#pragma unroll
+            for (int part = 0; part < 2; part++) {
+                const int d = part * 64 + lane;   // dword d of the row
+                const char *a = reinterpret_cast<const char *>(src + min(c + (d >> 1), n - 1)) + 4 * (d & 1);
+                __builtin_amdgcn_global_load_lds((gvoid_t *)a, (lvoid_t *)(dst + 256 * part), 4, 0, 0);
             }
         }
     };
+    auto stage_iter = [&](int b) __attribute__((always_inline)) {
+        const int k0 = b * kPanel, c1 = k0 + kPanel;
+        if (k0 >= nsteps || c1 >= n) return;   // no window work in iteration b
+        const int set = b & 1;
+        const int ntl = (FAR && GS > 1 && b >= 1) ? 3 : 2;
+        for (int rr = wv; rr < ntl * kPanel; rr += nwv) {
+            const int t = rr / kPanel, i = rr - t * kPanel;
+            const int r0 = t == 0 ? k0 + 1 : (t == 1 ? c1 + 1 : k0 - kPanel + 1);
+            stage_row(set, t, r0, c1, i);
+        }
+    };
+
+    // block 0's rows (set 1, tile B) and iteration 0's tiles
+    if (!is_panel) {
+        if (nsteps > 0)
+            for (int rr = wv; rr < kPanel; rr += nwv) stage_row(1, 1, 1, 0, rr);
+        stage_iter(0);
+        __builtin_amdgcn_s_waitcnt(0);   // the staged rows have landed in LDS
+    }
+    lds_barrier();
 
     if (is_panel && nsteps > 0) {
         p = win[0][lane];
@@ -567,12 +598,26 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         const int bp = (k0 / kPanel) & 1;
         const int c1 = k0 + kPanel;     // window b+1
         const int c2 = c1 + kPanel;     // the bulk's first column
-        if (!is_panel) flush(k0 / kPanel);
+        if (!is_panel) stage_iter(k0 / kPanel + 1);   // the next iteration's tiles
         if (is_panel) {
             if (c1 < n) {   // block b is full
                 const int w1 = c1 / kPanel;
                 if (w1 < 2 || owns(w1)) {
                     p = win[bp ^ 1][lane];
+                } else if constexpr (!FAR) {
+                    // window b+1 of another workgroup after blocks <= b-1, from
+                    // its owner (published at the end of its bulk pass of block b-1)
+                    const int nval = min(kPanel, n - c1);
+                    int *cnt = sp.cnt + (int64_t)s * sp.nwin + w1;
+                    for (int it = 0; !failed && __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nval;
+                         it++) {
+                        if (it >= kSplitSpin) failed = true;
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    T *src = sp.hand + ((int64_t)s * sp.nwin + w1) * kPanel;
+                    p = (!failed && lane < nval)
+                            ? __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : (T)0;
                 } else {
                     // window b+1 of another workgroup: its state after blocks
                     // <= b-2 (window 2: its initial values; later ones from the
@@ -589,10 +634,12 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                             if (it >= kSplitSpin) failed = true;
                             __builtin_amdgcn_s_sleep(1);
                         }
-                        const volatile T *src = sp.hand + ((int64_t)s * sp.nwin + w1) * kPanel;
-                        p = (!failed && lane < nval) ? src[lane] : (T)0;
+                        T *src = sp.hand + ((int64_t)s * sp.nwin + w1) * kPanel;
+                        p = (!failed && lane < nval)
+                                ? __hip_atomic_load(src + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : (T)0;
                     }
-                    window_apply_far(k0 - kPanel, c1);
+                    window_apply_far(k0 - kPanel, bp);
                 }
                 window_apply(k0, bp);
                 if (c1 < nsteps) {
@@ -668,17 +715,19 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                             if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = G::get(r[m], e);
                         }
                 }
-                const int c3 = c2 + kPanel;
+                // the window handed over: b+3 (two blocks ahead, FP32) or b+2
+                const int c3 = FAR ? c2 + kPanel : c2;
                 if (GS > 1 && c3 < n && owns(c3 / kPanel)) {
                     int put = 0;
-                    volatile T *dst = sp.hand + ((int64_t)s * sp.nwin + c3 / kPanel) * kPanel;
+                    T *dst = sp.hand + ((int64_t)s * sp.nwin + c3 / kPanel) * kPanel;
 #pragma unroll
                     for (int m = MS; m < NP; m++)
 #pragma unroll
                         for (int e = 0; e < W; e++) {
                             const int j = col_of(W * (tb + m * NTB)) + e;
                             if (j >= c3 && j < c3 + kPanel && j < n) {
-                                dst[j - c3] = G::get(r[m], e);
+                                __hip_atomic_store(dst + (j - c3), G::get(r[m], e), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
                                 put++;
                             }
                         }
@@ -696,10 +745,10 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const int wl0 = GS == 1 ? c2 / kPanel : max(0, (c2 / kPanel - g + GS - 1) / GS);
             dispatch_const<0, NP>(wl0 * kPanel / (W * NTB), bulk);
         }
+        if (!is_panel) __builtin_amdgcn_s_waitcnt(0);   // the staged rows have landed in LDS
         lds_barrier();
     }
-    // the last iteration's panel rows
-    if (!is_panel && nsteps > 0) flush((nsteps + kPanel - 1) / kPanel);
+
     // the last diagonal: U[n-1, n-1] = p[n-1]
     if (tid == 0 && g == 0) {
         const T pl = plast;
