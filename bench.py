@@ -14,8 +14,11 @@ column has ||b - W x||_inf <= 1e-12 ||b||_inf.
 
 J and the right-hand sides are resident in HBM before the timed region; the
 timed region brackets K steps with a device synchronization on both sides.
-The roofline object describes the kernel with the largest share of the step,
-timed live with HIP events on the library's stream; the CPU baseline is the
+The roofline object describes the hand-written kernel with the largest share
+of the step, timed live with HIP events on the library's stream (the rocBLAS
+GEMM phases, trsm, trmm and gemm_jx, are listed beside it: each is several
+library kernels, and the first pass's forward L sweep runs on a second stream
+beside the shift LUs, so its event span includes the time it waits for CUs); the CPU baseline is the
 C restatement of the reference algorithm (oracle/) on this host's cores, on a
 bounded sample.
 """
@@ -53,6 +56,8 @@ def log(*a):
 
 KERNELS = ["shift", "uinv", "ut_pack", "gather", "trsm", "fwd_chain", "utrsm", "trmm", "scatter", "gemm_jx",
            "resid_combine", "colnorms", "convert"]
+# phases made of rocBLAS GEMMs (several library kernels each)
+LIBRARY_PHASES = {"trsm", "trmm", "gemm_jx"}
 PEAK = {"hbm": (HBM_PEAK_GBS, "GB/s"), "mfma32": (FP32_PEAK_TFLOPS, "TFLOP/s"),
         "mfma64": (FP64_PEAK_TFLOPS, "TFLOP/s")}
 
@@ -564,7 +569,8 @@ def main():
                 kernels[prec][k] = r
     log("bench: fp32 kernels " + ", ".join(f"{k} {v['launches'] // steps}x {v['ms_per_step']:.2f} ms"
                                           for k, v in kernels["fp32"].items()))
-    dom = max(kernels["fp32"], key=lambda k: kernels["fp32"][k]["ms_per_step"])
+    own = [k for k in kernels["fp32"] if k not in LIBRARY_PHASES]
+    dom = max(own, key=lambda k: kernels["fp32"][k]["ms_per_step"])
     roof = dict(kernels["fp32"][dom])
 
     # ---- CPU baseline: the C restatement of the reference algorithm (oracle/),
