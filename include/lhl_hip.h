@@ -261,6 +261,10 @@ int lhl_get_shift_info(lhl_handle *h, lhl_shift_state *sh, int64_t *info, double
  *                      plan's masked refinement rounds always use them)
  * group_bytes          lhl_reduce_group: row-major bytes per sub-group   256 MiB
  *                      (each size class reads its first handle's knob)
+ * shift_split          most workgroups per real shift LU (1: one)       4
+ * utrsm_split          most workgroups per 16 columns of a real shift's 8
+ *                      U back substitution (1: the one-workgroup-per-
+ *                      4-columns kernel)
  * A value of -1 restores the default; lhl_tune_get reports -1 for a knob at
  * its default.  Results do not depend on the knobs beyond rounding (the
  * pivots of a reduction match for every panel setting). */

@@ -865,7 +865,7 @@ struct lhl_plan {
     double *X = nullptr;
     int64_t ldb = 0, ldx = 0, ld = 0, plane = 0;
     DevBuf bst, xst, rst, norms, colshift, shiftpar, cbd, cact, sact, iters, groups, ybuf, rbuf, jx, jxi;
-    int64_t ngroups = 0;
+    int64_t ngroups = 0, nwide = 0;
     ShiftPool pool;
     void *pin = nullptr;                  // pinned: shifts | column shifts | info | sact | iters
     double *p_shift = nullptr, *p_col = nullptr;
@@ -933,6 +933,7 @@ SolveArgs plan_solve_args(lhl_plan *p) {
     a.sact = p->sact.as<int>();
     a.groups = p->groups.p;
     a.ngroups = p->ngroups;
+    a.nwide = p->nwide;
     a.ybuf = &p->ybuf;
     a.rbuf = &p->rbuf;
     (void)h;
@@ -1165,7 +1166,7 @@ int lhl_plan_create(lhl_handle *h, lhl_plan **out, int64_t nshift, int shifts_co
         hipError_t e = hipMemcpy(p->cbd.p, p->cb.data(), sizeof(int64_t) * (size_t)(nshift + 1), hipMemcpyHostToDevice);
         if (e != hipSuccess) st = Status{LHL_ERR_HIP, hipGetErrorString(e)};
     }
-    if (st.ok()) st = groups_device(h, nshift, p->cb.data(), planes, p->groups, &p->ngroups);
+    if (st.ok()) st = groups_device(h, nshift, p->cb.data(), planes, p->groups, &p->ngroups, &p->nwide);
     if (st.ok()) st = ensure_stream2(h);
     if (st.ok()) {
         const size_t bytes = 32 * (size_t)nshift + 32 * (size_t)nc + 3 * sizeof(int) * (size_t)nshift + 256;
@@ -1561,7 +1562,7 @@ const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
     {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34}, {"gemm_impl", 0, 1},
-    {"group_bytes", 1, INT64_MAX},        {"shift_split", 1, 4},
+    {"group_bytes", 1, INT64_MAX},        {"shift_split", 1, 4},          {"utrsm_split", 1, 8},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;
@@ -1593,6 +1594,7 @@ int lhl_tune_set(lhl_handle *h, const char *knob, int64_t value) {
     else if (k == "h2d_chunk_bytes") h->tune_h2d_chunk = value;
     else if (k == "gemm_impl") h->tune_gemm = v;
     else if (k == "shift_split") h->tune_shift_split = value == -1 ? 0 : v;
+    else if (k == "utrsm_split") h->tune_utrsm_split = value == -1 ? 0 : v;
     return LHL_OK;
 }
 
@@ -1607,6 +1609,7 @@ int lhl_tune_get(lhl_handle *h, const char *knob, int64_t *value) {
     else if (k == "h2d_chunk_bytes") *value = h->tune_h2d_chunk;
     else if (k == "gemm_impl") *value = h->tune_gemm;
     else if (k == "shift_split") *value = h->tune_shift_split > 0 ? h->tune_shift_split : -1;
+    else if (k == "utrsm_split") *value = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : -1;
     else return fail(h, LHL_ERR_ARG, "lhl_tune_get: unknown knob " + k);
     return LHL_OK;
 }

@@ -725,6 +725,233 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
     }
 }
 
+constexpr int kRsCols = 16;            // internal columns per group of the row-split back substitution
+constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wait
+
+// ---- back substitution split by rows (real shifts) ---------------------------
+// x <- U_s^-1 y for up to 16 internal columns of one shift on G workgroups.
+// The 64-row blocks are dealt round-robin: block B belongs to workgroup
+// B mod G, which keeps its rows' right-hand sides in registers as MFMA
+// accumulators (wave v, slot j holds block G (v + 8 j) + w).  Going up from
+// the last block, x_b is computed by the owner of block b from its rows once
+// x_{b+1} is known, and handed to the other workgroups of the group through
+// a slot in uncached device memory and a counter (device-scope atomics, a
+// full memory wait before the count; the wait for it is bounded).  Every
+// workgroup subtracts U[rows, block b] x_b from its own rows above block b:
+// each workgroup reads only its rows of U, so U streams from HBM once per
+// group.  The owner of block b-1 takes that block's rows out of the bulk:
+// from LDS, 64 threads per 16-row tile run y_{b-1} -= U[b-1, b] x_b and
+// x_{b-1} = Uinv_{b-1} y_{b-1} on the matrix cores right after x_b arrives
+// (the two tiles were loaded into registers G blocks earlier), publish
+// x_{b-1}, and then run their bulk.  LDS holds -x (the MFMA's B operand, so
+// the accumulators subtract).
+template <typename T> constexpr int rs_kb(int ng) { return sizeof(T) == 4 ? (ng >= 4 ? 2 : ng == 2 ? 4 : 8) : (ng >= 2 ? 2 : 4); }
+
+template <typename T, int G, int NG>
+__global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ UTbase, int64_t ustride,
+                                                  const T *__restrict__ Uinv, int64_t uistride,
+                                                  const UGroup *__restrict__ groups, int g0, const T *__restrict__ Yin,
+                                                  T *__restrict__ Yf, int64_t ldy, const int *__restrict__ sact,
+                                                  T *__restrict__ hand, int *__restrict__ cnt, int64_t hslot) {
+    constexpr int NW = 8, C = kRsCols, V = ut_vec<T>();
+    constexpr int KB = rs_kb<T>(NG);   // k-steps (4 columns each) per batch of U loads
+    constexpr int NBT = 16 / KB;       // batches per 64-column block
+    constexpr int PT = kT * kT / 512;  // tile elements per thread
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    const int gl = blockIdx.x / G, w = blockIdx.x % G;
+    const UGroup gr = groups[g0 + gl];
+    if (gr.nq <= 0) return;
+    if (sact && !sact[gr.shift]) return;   // a converged shift (every workgroup of the group)
+    const int nbu = (n + kT - 1) / kT;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const T *Ut = UTbase + (int64_t)gr.shift * ustride;
+    const T *Ui = Uinv + (int64_t)gr.shift * uistride;
+    T *hx = hand + (int64_t)gl * hslot;
+    int *hc = cnt + gl;
+    __shared__ T Xs[2][kT][C];        // -x_b, -x_{b-1}
+    __shared__ T Ys[kT][C + 1];       // the rows of the block whose x is computed here next
+    __shared__ T Us[kT][kT + 1];      // Us[m][r] = U(64 cb + r, 64 (cb + 1) + m)
+    __shared__ T Is[kT][kT + 1];      // Is[k][i] = Uinv_cb(i, k)
+
+    // the owned rows' right-hand sides, in the accumulator layout
+    acc_t yv[NG][4];
+#pragma unroll
+    for (int j = 0; j < NG; j++) {
+        const int B = G * (wv + NW * j) + w;
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = kT * B + 16 * rt + M::row(lane, v);
+                const T y = Yin[min(row, n - 1) + (int64_t)(gr.q0 + min(li, gr.nq - 1)) * ldy];
+                yv[j][rt][v] = (B < nbu && row < n && li < gr.nq) ? y : (T)0;
+            }
+    }
+    // the two tiles of the next block this workgroup solves, one block ahead
+    T pu[PT], pi[PT];
+    auto preload = [&](int cb) __attribute__((always_inline)) {
+        if (cb + 1 < nbu) {
+#pragma unroll
+            for (int u = 0; u < PT; u++) {
+                const int e = tid + 512 * u, m = e >> 6, r = e & 63;
+                const int col = min(kT * (cb + 1) + m, n - 1);
+                pu[u] = Ut[(int)ucol_off<V>(col) + kT * cb + r];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < PT; u++) pu[u] = 0;   // the last block: no update tile
+        }
+#pragma unroll
+        for (int u = 0; u < PT; u++) pi[u] = Ui[(int64_t)cb * kT * kT + tid + 512 * u];
+    };
+    {
+        const int cb0 = nbu - 1 - ((nbu - 1 - w) % G + G) % G;   // the last block of this workgroup
+        if (cb0 >= 0) preload(cb0);
+    }
+    bool failed = false;
+    for (int b = nbu; b >= 1; b--) {
+        const bool crit = ((b - 1 - w) % G) == 0;   // x_{b-1} is computed here
+        const bool have = b < nbu;                  // x_b exists
+        const bool mine = have && ((b - w) % G) == 0;
+        const int bl = crit ? b - 1 : b;            // bulk rows: owned blocks below index bl
+        const int first = bl - w - G * wv;          // > 0: slot 0 active
+        const int na = (have && first > 0) ? min(NG, (first + G * NW - 1) / (G * NW)) : 0;
+        auto iter = [&](auto nac) __attribute__((always_inline)) {
+            constexpr int NA = decltype(nac)::value;
+            constexpr int NAL = NA > 0 ? NA : 1;
+            T a0[KB][NAL][4], a1[KB][NAL][4];
+            // batch t of block b's U values for the active slots
+            auto load_batch = [&](int t, T(&a)[KB][NAL][4]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int kk = 0; kk < KB; kk++) {
+                    const int col = min(kT * b + 4 * (t * KB + kk) + lk, n - 1);
+                    const T *pc = Ut + (int)ucol_off<V>(col) + li;
+#pragma unroll
+                    for (int j = 0; j < NA; j++)
+#pragma unroll
+                        for (int rt = 0; rt < 4; rt++) a[kk][j][rt] = pc[kT * (G * (wv + NW * j) + w) + 16 * rt];
+                }
+            };
+            auto mma = [&](int t, const T(&a)[KB][NAL][4]) __attribute__((always_inline)) {
+#pragma unroll
+                for (int kk = 0; kk < KB; kk++) {
+                    const T bx = Xs[b & 1][4 * (t * KB + kk) + lk][li];
+#pragma unroll
+                    for (int j = 0; j < NA; j++)
+#pragma unroll
+                        for (int rt = 0; rt < 4; rt++) yv[j][rt] = M::mac(a[kk][j][rt], bx, yv[j][rt]);
+                }
+            };
+            if constexpr (NA > 0) load_batch(0, a0);
+            // 1. x_b into LDS (from its owner, unless computed here)
+            if (have && !mine && (crit || w < bl)) {   // (a workgroup with no rows left skips it)
+                const int need = 4 * (nbu - b);   // four publishing waves per block
+                for (int it = 0; !failed && __hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need;
+                     it++) {
+                    if (it >= kSplitSpinRs) failed = true;
+                    __builtin_amdgcn_s_sleep(1);
+                }
+#pragma unroll
+                for (int u = 0; u < kT * C / 512; u++) {
+                    const int e = tid + 512 * u;
+                    const T v = __hip_atomic_load(hx + (int64_t)b * kT * C + e, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+                    Xs[b & 1][e / C][e % C] = failed ? __builtin_nan("") : -v;
+                }
+            }
+            // 2. block b-1's rows and tiles into LDS
+            if (crit) {
+                const int lb = (b - 1 - w) / G;
+                if (wv == lb % NW) {
+                    const int os = lb / NW;
+#pragma unroll
+                    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            T y = 0;
+#pragma unroll
+                            for (int j = 0; j < NG; j++) y = j == os ? yv[j][rt][v] : y;
+                            Ys[16 * rt + M::row(lane, v)][li] = y;
+                        }
+                }
+#pragma unroll
+                for (int u = 0; u < PT; u++) {
+                    const int e = tid + 512 * u;
+                    Us[e >> 6][e & 63] = pu[u];
+                    Is[e >> 6][e & 63] = pi[u];
+                }
+            }
+            lds_barrier();
+            // 3. x_{b-1} (waves 0-3, one 16-row tile each), published
+            if (crit) {
+                const int rt = wv;
+                if (have) {
+                    if (wv < 4) {
+                        acc_t acc;
+#pragma unroll
+                        for (int v = 0; v < 4; v++) acc[v] = Ys[16 * rt + M::row(lane, v)][li];
+#pragma unroll
+                        for (int ks = 0; ks < 16; ks++)
+                            acc = M::mac(Us[4 * ks + lk][16 * rt + li], Xs[b & 1][4 * ks + lk][li], acc);
+#pragma unroll
+                        for (int v = 0; v < 4; v++) Ys[16 * rt + M::row(lane, v)][li] = acc[v];
+                    }
+                    lds_barrier();
+                }
+                if (wv < 4) {
+                    acc_t acc = M::zero();
+#pragma unroll
+                    for (int ks = 0; ks < 16; ks++) acc = M::mac(Is[4 * ks + lk][16 * rt + li], Ys[4 * ks + lk][li], acc);
+                    T *dst = hx + (int64_t)(b - 1) * kT * C;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int r = 16 * rt + M::row(lane, v), row = kT * (b - 1) + r;
+                        const T x = row < n ? acc[v] : (T)0;
+                        Xs[(b - 1) & 1][r][li] = -x;
+                        __hip_atomic_store(dst + r * C + li, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (row < n && li < gr.nq) Yf[row + (int64_t)(gr.q0 + li) * ldy] = x;
+                    }
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    __builtin_amdgcn_s_waitcnt(0);   // the values have landed
+                    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                    if (lane == 0) __hip_atomic_fetch_add(hc, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (b - 1 - G >= 0) preload(b - 1 - G);
+            }
+            // 4. the bulk: y -= U[rows, block b] x_b for the active slots
+            if constexpr (NA > 0) {
+#pragma unroll
+                for (int t = 0; t < NBT; t += 2) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (t + 1 < NBT) load_batch(t + 1, a1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma(t, a0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (t + 2 < NBT) load_batch(t + 2, a0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (t + 1 < NBT) mma(t + 1, a1);
+                }
+            }
+        };
+        if constexpr (NG >= 4) {
+            if (na == 4) iter(std::integral_constant<int, 4>{});
+            else if (na == 3) iter(std::integral_constant<int, 3>{});
+            else if (na == 2) iter(std::integral_constant<int, 2>{});
+            else if (na == 1) iter(std::integral_constant<int, 1>{});
+            else iter(std::integral_constant<int, 0>{});
+        } else if constexpr (NG >= 2) {
+            if (na == 2) iter(std::integral_constant<int, 2>{});
+            else if (na == 1) iter(std::integral_constant<int, 1>{});
+            else iter(std::integral_constant<int, 0>{});
+        } else {
+            if (na == 1) iter(std::integral_constant<int, 1>{});
+            else iter(std::integral_constant<int, 0>{});
+        }
+    }
+}
+
 // Forward interchange chain of the Hessenberg solve (src:2806-2816) as a scan.
 // Per column, with c the carried value (c = y[0] at the start), step k reads
 // b = y[k+1] and writes y[k] = (s_k ? b : c), then c <- (s_k ? c : b) - l_k y[k];
@@ -1118,6 +1345,110 @@ std::vector<UGroup> build_groups(int64_t nshift, const int64_t *cb, int planes) 
     return ug;
 }
 
+// Groups of the row-split back substitution: up to kRsCols internal columns
+// of one shift each, in shift order (its workgroups read disjoint rows of U,
+// so no XCD placement).
+std::vector<UGroup> build_wgroups(int64_t nshift, const int64_t *cb, int planes) {
+    std::vector<UGroup> wg;
+    for (int64_t s = 0; s < nshift; s++)
+        for (int64_t q = cb[s] * planes; q < cb[s + 1] * planes; q += kRsCols)
+            wg.push_back(UGroup{(int)s, (int)q, (int)std::min<int64_t>(kRsCols, cb[s + 1] * planes - q), 0});
+    return wg;
+}
+
+__global__ void k_rs_reset(int *__restrict__ cnt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = 0;
+}
+
+// Workgroups of k_utrsm_rs<T, G, NG> resident on the whole device at once
+// (0 if the query fails, which leaves the row split off).
+template <typename T, int G, int NG> int rs_capacity() {
+    static const int cap = [] {
+        int nb = 0, dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_utrsm_rs<T, G, NG>, 512, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return nb * cus;
+    }();
+    return cap;
+}
+
+// f(G, NG) with the pair as integral constants (G in {2, 4, 8}, NG in {1, 2, 4})
+template <typename T, typename F> Status with_rs_shape(int G, int NG, F &&f) {
+    using std::integral_constant;
+    auto ng = [&](auto g) -> Status {
+        if (NG == 1) return f(g, integral_constant<int, 1>{});
+        if (NG == 2) return f(g, integral_constant<int, 2>{});
+        if constexpr (sizeof(T) == 4) return f(g, integral_constant<int, 4>{});
+        return {LHL_ERR_ARG, "utrsm: unsupported row-split shape"};
+    };
+    if (G == 2) return ng(integral_constant<int, 2>{});
+    if (G == 4) return ng(integral_constant<int, 4>{});
+    return ng(integral_constant<int, 8>{});
+}
+
+// The row-split back substitution of real shifts (n <= 8192, 128 rows and
+// up): G workgroups per group of 16 columns, G as large as the knob allows
+// with at least two blocks per workgroup, few enough slots per thread, and
+// the whole launch resident at once (else fewer groups per launch).  Sets
+// *used to false, launching nothing, where it does not apply.
+template <typename T>
+Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int64_t nwide, const T *Yin, T *Yf,
+                int64_t ldy, int64_t uis, bool *used) {
+    *used = false;
+    const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
+    const int nbu = (n + kT - 1) / kT;
+    if (gmax < 2 || nwide <= 0 || a.shift_complex || !a.UT || n > 8192) return {};
+    constexpr int NGMAX = sizeof(T) == 4 ? 4 : 2;
+    int G = 0, NG = 0, cap = 0;
+    for (int gc : {8, 4, 2}) {
+        if (gc > gmax || nbu < 2 * gc) continue;
+        int ng = (nbu + 8 * gc - 1) / (8 * gc);
+        if (ng == 3) ng = 4;
+        if (ng > NGMAX) continue;
+        int cc = 0;
+        LHL_TRY(with_rs_shape<T>(gc, ng, [&](auto g, auto s) -> Status {
+            cc = rs_capacity<T, decltype(g)::value, decltype(s)::value>();
+            return {};
+        }));
+        if (cc < gc) continue;
+        const bool fits = nwide * gc <= cc;
+        if (G == 0 || fits) {
+            G = gc;
+            NG = ng;
+            cap = cc;
+        }
+        if (fits) break;
+    }
+    if (G == 0) return {};
+    const int64_t chunk = std::min<int64_t>(nwide, cap / G);
+    const int64_t hslot = (int64_t)nbu * kT * kRsCols;
+    const size_t hb = ((sizeof(T) * (size_t)(chunk * hslot) + 255) / 256) * 256;
+    if (!uc_ensure(h, hb + sizeof(int) * (size_t)chunk).ok()) {
+        (void)hipGetLastError();
+        return {};
+    }
+    T *hand = static_cast<T *>(h->uc.back().first);
+    int *cnt = reinterpret_cast<int *>(static_cast<char *>(h->uc.back().first) + hb);
+    for (int64_t g0 = 0; g0 < nwide; g0 += chunk) {
+        const int cg = (int)std::min<int64_t>(chunk, nwide - g0);
+        LHL_LAUNCH(h, "utrsm_reset", k_rs_reset, dim3((unsigned)((cg + 255) / 256)), dim3(256), 0, cnt, cg);
+        LHL_TRY(with_rs_shape<T>(G, NG, [&](auto g, auto s) -> Status {
+            constexpr int GC = decltype(g)::value, NGC = decltype(s)::value;
+            LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
+                       (const T *)a.UT, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact, hand,
+                       cnt, hslot);
+            return {};
+        }));
+    }
+    *used = true;
+    return {};
+}
+
 template <typename T, typename TX>
 Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int n = (int)h->n;
@@ -1153,9 +1484,11 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     // the back substitution's column groups (within one shift)
     const UGroup *dg;
     size_t ngroups;
+    int64_t nwide = 0;
     if (a.groups) {   // uploaded once by the caller (a plan)
         dg = (const UGroup *)a.groups;
         ngroups = (size_t)a.ngroups;
+        nwide = a.nwide;
     } else {
         std::vector<int64_t> cb((size_t)a.nshift + 1);
         if (a.col_begin) {
@@ -1164,7 +1497,13 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
             cb[0] = 0;
             cb[1] = a.ncol;
         }
-        const std::vector<UGroup> ug = build_groups(a.nshift, cb.data(), planes);
+        std::vector<UGroup> ug = build_groups(a.nshift, cb.data(), planes);
+        ngroups = ug.size();
+        if (!a.shift_complex) {
+            const std::vector<UGroup> wgv = build_wgroups(a.nshift, cb.data(), planes);
+            nwide = (int64_t)wgv.size();
+            ug.insert(ug.end(), wgv.begin(), wgv.end());
+        }
         LHL_TRY(h->gbuf.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
         // ordered on the handle's stream: an earlier solve's back substitution
         // may still be reading the table
@@ -1173,7 +1512,6 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         std::memcpy(hug, ug.data(), ugb);
         LHL_HIP_TRY(hipMemcpyAsync(h->gbuf.p, hug, ugb, hipMemcpyHostToDevice, h->stream));
         dg = h->gbuf.as<UGroup>();
-        ngroups = ug.size();
     }
 
     const int N1 = n - 1;
@@ -1191,7 +1529,12 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = a.uistride ? a.uistride : (int64_t)nbu * kT * kT * (a.shift_complex ? 2 : 1);
     const dim3 gu((unsigned)ngroups);
-    if (a.UT && n <= 512 * 16) {
+    // real shifts: rows split over several workgroups per 16 columns where it applies
+    bool rs_used = false;
+    LHL_TRY(utrsm_rs<T>(h, a, n, dg + ngroups, nwide, (const T *)Yo, Yw, ldy, uis, &rs_used));
+    if (rs_used) {
+        // launched
+    } else if (a.UT && n <= 512 * 16) {
         // U in packed columns, right-hand sides in registers (512 threads:
         // measured faster than 1024 on this shape)
         constexpr int NTR = 512;
@@ -1425,11 +1768,15 @@ Status linv_device(lhl_handle *h) {
     return h->dtype == LHL_F64 ? linv_typed<double>(h) : linv_typed<float>(h);
 }
 
-Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups) {
-    const std::vector<UGroup> ug = build_groups(nshift, cb, planes);
+Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups,
+                     int64_t *nwide) {
+    std::vector<UGroup> ug = build_groups(nshift, cb, planes);
+    const std::vector<UGroup> wg = build_wgroups(nshift, cb, planes);
+    *ngroups = (int64_t)ug.size();
+    *nwide = (int64_t)wg.size();
+    ug.insert(ug.end(), wg.begin(), wg.end());
     LHL_TRY(out.ensure(sizeof(UGroup) * std::max<size_t>(ug.size(), 1)));
     if (!ug.empty()) LHL_HIP_TRY(hipMemcpy(out.p, ug.data(), sizeof(UGroup) * ug.size(), hipMemcpyHostToDevice));
-    *ngroups = (int64_t)ug.size();
     (void)h;
     return {};
 }

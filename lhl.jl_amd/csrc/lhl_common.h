@@ -378,6 +378,8 @@ struct lhl_handle {
     int tune_lsweep = -1;  // halvings of the L sweeps' triangular GEMMs (-1: default, 2)
     int64_t tune_shift_wide = 0;   // n from which the shift LU's bulk is 960 threads wide (0: default)
     int tune_shift_split = 0;      // most workgroups per real shift LU (0: default 4, 1: one per shift)
+    int tune_utrsm_split = 0;      // most workgroups per 16 columns of a real shift's U solve (0: default 8,
+                                   // 1: the kernel with one workgroup per 4 columns)
     // uncached device blocks (the split shift LU's hand-off slots); the last is
     // in use, the others are kept while the handle lives
     std::vector<std::pair<void *, size_t>> uc;
@@ -531,13 +533,17 @@ struct SolveArgs {
     // build_groups lays them out) and the two work buffers
     const void *groups = nullptr;
     int64_t ngroups = 0;
+    // and the row-split back substitution's table (nwide entries of up to 16
+    // internal columns of one shift, after the ngroups entries)
+    int64_t nwide = 0;
     DevBuf *ybuf = nullptr, *rbuf = nullptr;
 };
 Status solve_device(lhl_handle *h, const SolveArgs &a);    // lhl_batch.hip
-// the back substitution's group table for nshift shifts with column offsets
+// the back substitution's group tables for nshift shifts with column offsets
 // cb (host, nshift + 1) and planes per column, uploaded into `out`
-// (synchronously); *ngroups receives its length
-Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups);
+// (synchronously): *ngroups entries of up to 4 columns, then *nwide of up to 16
+Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int planes, DevBuf &out, int64_t *ngroups,
+                     int64_t *nwide);
 // after a shift LU: the inverses of the 64x64 diagonal blocks of nshift packed
 // U's (per-shift stride returned in *uistride) and U in packed columns (UT,
 // ustride apart)

@@ -42,7 +42,7 @@ __all__ = [
 
 TUNING_KNOBS = ("reduce_blocked_min", "reduce_nb", "reduce_rmin", "reduce_switch", "lsweep_depth",
                 "shift_wide_min", "shift_pool_bytes", "h2d_chunk_bytes", "gemm_impl", "group_bytes",
-                "shift_split")
+                "shift_split", "utrsm_split")
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "liblhl_hip.so")

@@ -94,6 +94,38 @@ def test_split_shift_batched_equals_unsplit(lhl_hip, monkeypatch):
     assert np.array_equal(np.asarray(out[1][1]), np.asarray(out[4][1]))
 
 
+# the row-split U back substitution (G workgroups per 16 columns of a shift)
+# against the one-workgroup-per-4-columns kernel and dense FP64 solves: a
+# partial last block, fewer and more than 16 columns per shift, G = 2, 4, 8,
+# and a batch of more groups than fit on the device at once (several launches)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_utrsm_split_solves(lhl_hip, monkeypatch, dtype):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    rtol = 1e-11 if dtype == np.float64 else 5e-3
+    for n, S, R in ((200, 3, 16), (700, 2, 5), (1000, 2, 17), (2049, 2, 33), (4096, 2, 16), (512, 300, 16)):
+        J = np.asfortranarray(randn(n + 7, n) / np.sqrt(n))
+        gam = np.linspace(0.05, 0.5, S)
+        offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+        B = np.asfortranarray(np.random.default_rng(n).standard_normal((n, S * R)))
+        ws = lhl_hip.LHLWorkspace(n, dtype)
+        lhl_hip.lhl_(ws, J)
+        X = {}
+        for split in (1, -1):
+            say(f"case: utrsm split {np.dtype(dtype).name} n={n} S={S} R={R} split={split}")
+            lhl_hip.tune_set(ws, "utrsm_split", split)
+            X[split], info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs)
+            assert np.all(np.asarray(info) == 0)
+        for s in sorted({0, S // 2, S - 1}):
+            c = slice(offs[s], offs[s + 1])
+            ref = np.linalg.solve(np.eye(n) - gam[s] * J, B[:, c])
+            den = np.abs(ref).max()
+            err = {split: np.abs(X[split][:, c] - ref).max() / den for split in (1, -1)}
+            # FP64: both at the dense solve; FP32: the split kernel no further
+            # from it than the other (the FP32 factors set the error)
+            assert err[1] <= rtol, (n, S, R, s, err)
+            assert err[-1] <= (rtol if dtype == np.float64 else 3 * err[1] + 1e-6), (n, S, R, s, err)
+
+
 def test_reduce_knobs_keep_pivots(lhl_hip, oracle, monkeypatch):
     monkeypatch.setenv("LHL_TUNE", "0")
     n = 600
