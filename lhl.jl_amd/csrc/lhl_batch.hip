@@ -744,8 +744,14 @@ constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wai
 // x_{b-1} = Uinv_{b-1} y_{b-1} on the matrix cores right after x_b arrives
 // (the two tiles were loaded into registers G blocks earlier), publish
 // x_{b-1}, and then run their bulk.  LDS holds -x (the MFMA's B operand, so
-// the accumulators subtract).
-template <typename T> constexpr int rs_kb(int ng) { return sizeof(T) == 4 ? (ng >= 4 ? 2 : ng == 2 ? 4 : 8) : (ng >= 2 ? 2 : 4); }
+// the accumulators subtract).  U values come in batches of KB k-steps through
+// four register sets, three batches in flight while one is multiplied; the
+// pipeline runs on across blocks (U does not depend on x), so a block starts
+// with its first three batches already loaded.
+// k-steps (4 columns each) per batch of U loads: four batch register sets of
+// 16 NG KB values per lane (64 VGPRs in either precision)
+template <typename T> constexpr int rs_kb(int ng) { return sizeof(T) == 4 ? (ng >= 2 ? 2 : 4) : (ng >= 2 ? 1 : 2); }
+constexpr int kRsSets = 4;
 
 template <typename T, int G, int NG>
 __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ UTbase, int64_t ustride,
@@ -754,8 +760,10 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                                                   T *__restrict__ Yf, int64_t ldy, const int *__restrict__ sact,
                                                   T *__restrict__ hand, int *__restrict__ cnt, int64_t hslot) {
     constexpr int NW = 8, C = kRsCols, V = ut_vec<T>();
-    constexpr int KB = rs_kb<T>(NG);   // k-steps (4 columns each) per batch of U loads
+    constexpr int KB = rs_kb<T>(NG);   // k-steps per batch
     constexpr int NBT = 16 / KB;       // batches per 64-column block
+    constexpr int NS = kRsSets;        // batch register sets: NS - 1 batches in flight
+    static_assert(NBT % NS == 0, "the sets rotate whole blocks");
     constexpr int PT = kT * kT / 512;  // tile elements per thread
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
@@ -775,7 +783,12 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     __shared__ T Us[kT][kT + 1];      // Us[m][r] = U(64 cb + r, 64 (cb + 1) + m)
     __shared__ T Is[kT][kT + 1];      // Is[k][i] = Uinv_cb(i, k)
 
-    // the owned rows' right-hand sides, in the accumulator layout
+    // The owned rows' right-hand sides as MFMA accumulators, rows interleaved:
+    // tile q of slot j holds the block's rows 4 i + q (i = 0..15), so the A
+    // operand of lane (i, k) is 4 consecutive rows of one column of U, one
+    // 16-byte load for the four tiles.  Accumulator v of tile q in this lane:
+    // row 4 M::row(lane, v) + q of the block, column li.
+    auto rowq = [&](int q, int v) { return 4 * M::row(lane, v) + q; };
     acc_t yv[NG][4];
 #pragma unroll
     for (int j = 0; j < NG; j++) {
@@ -784,7 +797,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         for (int rt = 0; rt < 4; rt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = kT * B + 16 * rt + M::row(lane, v);
+                const int row = kT * B + rowq(rt, v);
                 const T y = Yin[min(row, n - 1) + (int64_t)(gr.q0 + min(li, gr.nq - 1)) * ldy];
                 yv[j][rt][v] = (B < nbu && row < n && li < gr.nq) ? y : (T)0;
             }
@@ -810,41 +823,78 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         const int cb0 = nbu - 1 - ((nbu - 1 - w) % G + G) % G;   // the last block of this workgroup
         if (cb0 >= 0) preload(cb0);
     }
+    // Batch register sets of CAP values: with NA slots active a batch holds
+    // kb(NA) = KB NG / NA k-steps (more columns when fewer slots are active,
+    // so as many bytes stay in flight), value (kk NA + j) 4 + q.  Batch t of
+    // block bb for slots j < NA; rows past the matrix (slots beyond the last
+    // block) are clamped below n.
+    constexpr int CAP = KB * NG * 4;
+    T A[NS][CAP];
+    auto load_batch = [&](auto nac, int bb, int t, T(&a)[CAP]) __attribute__((always_inline)) {
+        constexpr int NA = decltype(nac)::value;
+        constexpr int KBA = NA > 0 ? KB * NG / NA : 1;
+#pragma unroll
+        for (int kk = 0; kk < KBA; kk++) {
+            const int col = min(kT * bb + 4 * (t * KBA + kk) + lk, n - 1);
+            const T *pc = Ut + (int)ucol_off<V>(col);
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                // rows 4 li .. 4 li + 3 of the slot's block (a multiple of 4
+                // from the column start: 16-byte aligned), clamped below n
+                const int r0 = min(kT * (G * (wv + NW * j) + w) + 4 * li, (n - 1) & ~3);
+                const V4<T> rows = ld4u(pc + r0);
+#pragma unroll
+                for (int rt = 0; rt < 4; rt++) a[(kk * NA + j) * 4 + rt] = rows[rt];
+            }
+        }
+    };
+    // active slots of this wave in the bulk of block b: owned blocks below
+    // b (below b - 1 where block b-1's x is computed here)
+    auto nact = [&](int b) -> int {
+        if (b >= nbu) return 0;
+        const int bl = ((b - 1 - w) % G) == 0 ? b - 1 : b;
+        const int first = bl - w - G * wv;
+        return first > 0 ? min(NG, (first + G * NW - 1) / (G * NW)) : 0;
+    };
+    auto dispatch = [&](int na, auto f) __attribute__((always_inline)) {
+        if constexpr (NG >= 2) {
+            if (na == 2) f(std::integral_constant<int, 2>{});
+            else if (na == 1) f(std::integral_constant<int, 1>{});
+            else f(std::integral_constant<int, 0>{});
+        } else {
+            if (na == 1) f(std::integral_constant<int, 1>{});
+            else f(std::integral_constant<int, 0>{});
+        }
+    };
+    if (nbu >= 2) {   // the first bulk block's first batches
+        dispatch(nact(nbu - 1), [&](auto nac) __attribute__((always_inline)) {
+#pragma unroll
+            for (int t = 0; t < NS - 1; t++) load_batch(nac, nbu - 1, t, A[t]);
+        });
+    }
     bool failed = false;
     for (int b = nbu; b >= 1; b--) {
         const bool crit = ((b - 1 - w) % G) == 0;   // x_{b-1} is computed here
         const bool have = b < nbu;                  // x_b exists
         const bool mine = have && ((b - w) % G) == 0;
         const int bl = crit ? b - 1 : b;            // bulk rows: owned blocks below index bl
-        const int first = bl - w - G * wv;          // > 0: slot 0 active
-        const int na = (have && first > 0) ? min(NG, (first + G * NW - 1) / (G * NW)) : 0;
-        auto iter = [&](auto nac) __attribute__((always_inline)) {
-            constexpr int NA = decltype(nac)::value;
-            constexpr int NAL = NA > 0 ? NA : 1;
-            T a0[KB][NAL][4], a1[KB][NAL][4];
-            // batch t of block b's U values for the active slots
-            auto load_batch = [&](int t, T(&a)[KB][NAL][4]) __attribute__((always_inline)) {
+        const int na = nact(b);
+        // na drops by at most one from block to block (owned blocks of a wave are 8 G apart)
+        const int nax = na > 0 && b > 1 ? nact(b - 1) : 0;   // the next block's
+        auto iter = [&](auto nac, auto nnc) __attribute__((always_inline)) {
+            constexpr int NA = decltype(nac)::value, NAX = decltype(nnc)::value;
+            constexpr int KBA = NA > 0 ? KB * NG / NA : 1;
+            constexpr int NBA = 16 / KBA;   // batches of block b
+            auto mma = [&](int t, const T(&a)[CAP]) __attribute__((always_inline)) {
 #pragma unroll
-                for (int kk = 0; kk < KB; kk++) {
-                    const int col = min(kT * b + 4 * (t * KB + kk) + lk, n - 1);
-                    const T *pc = Ut + (int)ucol_off<V>(col) + li;
-#pragma unroll
-                    for (int j = 0; j < NA; j++)
-#pragma unroll
-                        for (int rt = 0; rt < 4; rt++) a[kk][j][rt] = pc[kT * (G * (wv + NW * j) + w) + 16 * rt];
-                }
-            };
-            auto mma = [&](int t, const T(&a)[KB][NAL][4]) __attribute__((always_inline)) {
-#pragma unroll
-                for (int kk = 0; kk < KB; kk++) {
-                    const T bx = Xs[b & 1][4 * (t * KB + kk) + lk][li];
+                for (int kk = 0; kk < KBA; kk++) {
+                    const T bx = Xs[b & 1][4 * (t * KBA + kk) + lk][li];
 #pragma unroll
                     for (int j = 0; j < NA; j++)
 #pragma unroll
-                        for (int rt = 0; rt < 4; rt++) yv[j][rt] = M::mac(a[kk][j][rt], bx, yv[j][rt]);
+                        for (int rt = 0; rt < 4; rt++) yv[j][rt] = M::mac(a[(kk * NA + j) * 4 + rt], bx, yv[j][rt]);
                 }
             };
-            if constexpr (NA > 0) load_batch(0, a0);
             // 1. x_b into LDS (from its owner, unless computed here)
             if (have && !mine && (crit || w < bl)) {   // (a workgroup with no rows left skips it)
                 const int need = 4 * (nbu - b);   // four publishing waves per block
@@ -873,7 +923,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                             T y = 0;
 #pragma unroll
                             for (int j = 0; j < NG; j++) y = j == os ? yv[j][rt][v] : y;
-                            Ys[16 * rt + M::row(lane, v)][li] = y;
+                            Ys[rowq(rt, v)][li] = y;
                         }
                 }
 #pragma unroll
@@ -920,34 +970,40 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                 }
                 if (b - 1 - G >= 0) preload(b - 1 - G);
             }
-            // 4. the bulk: y -= U[rows, block b] x_b for the active slots
+            // 4. the bulk: y -= U[rows, block b] x_b for the active slots, the
+            //    loads running NS - 1 batches ahead into block b-1 (its active
+            //    slots are among this block's; block 0 has no successor: a
+            //    repeat of block 1, unused)
             if constexpr (NA > 0) {
+                static_assert(NBA % NS == 0, "the sets rotate whole blocks");
 #pragma unroll
-                for (int t = 0; t < NBT; t += 2) {
+                for (int t = 0; t < NBA; t++) {
+                    constexpr int L = NS - 1;
                     __builtin_amdgcn_sched_barrier(0);
-                    if (t + 1 < NBT) load_batch(t + 1, a1);
+                    if (t + L < NBA) load_batch(nac, b, t + L, A[(t + L) % NS]);
+                    else if constexpr (NAX > 0) load_batch(nnc, b - 1, t + L - NBA, A[(t + L) % NS]);
                     __builtin_amdgcn_sched_barrier(0);
-                    mma(t, a0);
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (t + 2 < NBT) load_batch(t + 2, a0);
-                    __builtin_amdgcn_sched_barrier(0);
-                    if (t + 1 < NBT) mma(t + 1, a1);
+                    mma(t, A[t % NS]);
                 }
             }
         };
-        if constexpr (NG >= 4) {
-            if (na == 4) iter(std::integral_constant<int, 4>{});
-            else if (na == 3) iter(std::integral_constant<int, 3>{});
-            else if (na == 2) iter(std::integral_constant<int, 2>{});
-            else if (na == 1) iter(std::integral_constant<int, 1>{});
-            else iter(std::integral_constant<int, 0>{});
-        } else if constexpr (NG >= 2) {
-            if (na == 2) iter(std::integral_constant<int, 2>{});
-            else if (na == 1) iter(std::integral_constant<int, 1>{});
-            else iter(std::integral_constant<int, 0>{});
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        if constexpr (NG >= 2) {
+            if (na == 2) {
+                if (nax == 2) iter(I2{}, I2{});
+                else iter(I2{}, I1{});
+            } else if (na == 1) {
+                if (nax == 1) iter(I1{}, I1{});
+                else iter(I1{}, I0{});
+            } else {
+                iter(I0{}, I0{});
+            }
         } else {
-            if (na == 1) iter(std::integral_constant<int, 1>{});
-            else iter(std::integral_constant<int, 0>{});
+            if (na == 1 && nax == 1) iter(I1{}, I1{});
+            else if (na == 1) iter(I1{}, I0{});
+            else iter(I0{}, I0{});
         }
     }
 }
@@ -1377,14 +1433,12 @@ template <typename T, int G, int NG> int rs_capacity() {
     return cap;
 }
 
-// f(G, NG) with the pair as integral constants (G in {2, 4, 8}, NG in {1, 2, 4})
+// f(G, NG) with the pair as integral constants (G in {2, 4, 8}, NG in {1, 2})
 template <typename T, typename F> Status with_rs_shape(int G, int NG, F &&f) {
     using std::integral_constant;
     auto ng = [&](auto g) -> Status {
         if (NG == 1) return f(g, integral_constant<int, 1>{});
-        if (NG == 2) return f(g, integral_constant<int, 2>{});
-        if constexpr (sizeof(T) == 4) return f(g, integral_constant<int, 4>{});
-        return {LHL_ERR_ARG, "utrsm: unsupported row-split shape"};
+        return f(g, integral_constant<int, 2>{});
     };
     if (G == 2) return ng(integral_constant<int, 2>{});
     if (G == 4) return ng(integral_constant<int, 4>{});
@@ -1393,7 +1447,7 @@ template <typename T, typename F> Status with_rs_shape(int G, int NG, F &&f) {
 
 // The row-split back substitution of real shifts (n <= 8192, 128 rows and
 // up): G workgroups per group of 16 columns, G as large as the knob allows
-// with at least two blocks per workgroup, few enough slots per thread, and
+// with at least two blocks per workgroup, at most two slots per thread, and
 // the whole launch resident at once (else fewer groups per launch).  Sets
 // *used to false, launching nothing, where it does not apply.
 template <typename T>
@@ -1403,13 +1457,11 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
     const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
     const int nbu = (n + kT - 1) / kT;
     if (gmax < 2 || nwide <= 0 || a.shift_complex || !a.UT || n > 8192) return {};
-    constexpr int NGMAX = sizeof(T) == 4 ? 4 : 2;
     int G = 0, NG = 0, cap = 0;
     for (int gc : {8, 4, 2}) {
         if (gc > gmax || nbu < 2 * gc) continue;
-        int ng = (nbu + 8 * gc - 1) / (8 * gc);
-        if (ng == 3) ng = 4;
-        if (ng > NGMAX) continue;
+        const int ng = (nbu + 8 * gc - 1) / (8 * gc);   // slots per thread (at most 2)
+        if (ng > 2) continue;
         int cc = 0;
         LHL_TRY(with_rs_shape<T>(gc, ng, [&](auto g, auto s) -> Status {
             cc = rs_capacity<T, decltype(g)::value, decltype(s)::value>();
