@@ -719,7 +719,8 @@ int lhl_shift_solve_batched(lhl_handle *h, int64_t nshift, const double *sigma, 
         a.rdiag = pool.rdiag.p;
         a.Uinv = pool.Uinv.p;
         a.uistride = (((n + 63) / 64) * 64) * 64 * planes;
-        a.UT = pool.UT.p;
+        a.UT = pool.ut ? nullptr : pool.UT.p;
+        a.ut = pool.ut;
         a.swp = pool.swp.as<uint8_t>();
         const int64_t lplane = ((n + 63) / 64) * 64;
         a.ustride = pool.ustride;
@@ -924,7 +925,8 @@ SolveArgs plan_solve_args(lhl_plan *p) {
     a.rdiag = p->pool.rdiag.p;
     a.Uinv = p->pool.Uinv.p;
     a.uistride = (((n + 63) / 64) * 64) * 64 * p->planes;
-    a.UT = p->pool.UT.p;
+    a.UT = p->pool.ut ? nullptr : p->pool.UT.p;
+    a.ut = p->pool.ut;
     a.swp = p->pool.swp.as<uint8_t>();
     a.ustride = p->pool.ustride;
     a.lstride = lplane * p->planes;

@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(64 * 8) k_linv_full(int n, const T *__restrict
 template <typename T, int UC>
 __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase, int64_t ustride,
                                              const T *__restrict__ rdbase, int64_t lstride, T *__restrict__ Uinv,
-                                             int64_t uistride) {
+                                             int64_t uistride, int ut) {
 #pragma clang fp contract(off)
     constexpr int UP = UC ? 2 : 1;
     const int b = blockIdx.x, s = blockIdx.y;
@@ -203,7 +203,9 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
     for (int t = threadIdx.x; t < kT * kT; t += 64) {
         const int c = t % kT, r = t / kT;
         const bool ok = r < nb && c < nb && c > r;
-        const int64_t o = upack_off(n, ok ? r0 + r : 0) - (ok ? r0 + r : 0) + r0 + c;
+        const int rr = ok ? r0 + r : 0, cc = ok ? r0 + c : 0;
+        constexpr int UV = utile_gw<T>();
+        const int64_t o = ut ? utile_row<UV>((n + 63) / 64, rr) + utile_col<UV>(cc) : upack_off(n, rr) - rr + cc;
         Us[0][r][c] = ok ? U[o] : (T)0;
         if constexpr (UC) Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
     }
@@ -728,7 +730,7 @@ __global__ void __launch_bounds__(NT) k_utrsm_t(int n, const T *__restrict__ UTb
 constexpr int kRsCols = 16;            // internal columns per group of the row-split back substitution
 constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wait
 
-// ---- back substitution split by rows (real shifts) ---------------------------
+// ---- back substitution split by rows (real shifts, U in 64x64 tiles) --------
 // x <- U_s^-1 y for up to 16 internal columns of one shift on G workgroups.
 // The 64-row blocks are dealt round-robin: block B belongs to workgroup
 // B mod G, which keeps its rows' right-hand sides in registers as MFMA
@@ -737,33 +739,29 @@ constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wai
 // x_{b+1} is known, and handed to the other workgroups of the group through
 // a slot in uncached device memory and a counter (device-scope atomics, a
 // full memory wait before the count; the wait for it is bounded).  Every
-// workgroup subtracts U[rows, block b] x_b from its own rows above block b:
-// each workgroup reads only its rows of U, so U streams from HBM once per
-// group.  The owner of block b-1 takes that block's rows out of the bulk:
-// from LDS, 64 threads per 16-row tile run y_{b-1} -= U[b-1, b] x_b and
-// x_{b-1} = Uinv_{b-1} y_{b-1} on the matrix cores right after x_b arrives
-// (the two tiles were loaded into registers G blocks earlier), publish
-// x_{b-1}, and then run their bulk.  LDS holds -x (the MFMA's B operand, so
-// the accumulators subtract).  U values come in batches of KB k-steps through
-// four register sets, three batches in flight while one is multiplied; the
-// pipeline runs on across blocks (U does not depend on x), so a block starts
-// with its first three batches already loaded.
-// k-steps (4 columns each) per batch of U loads: four batch register sets of
-// 16 NG KB values per lane (64 VGPRs in either precision)
-template <typename T> constexpr int rs_kb(int ng) { return sizeof(T) == 4 ? (ng >= 2 ? 2 : 4) : (ng >= 2 ? 1 : 2); }
-constexpr int kRsSets = 4;
+// workgroup subtracts U[rows, block b] x_b from its own rows above block b,
+// reading only its tiles of U (each a contiguous 64x64 piece), so U streams
+// from HBM once per group.  The owner of block b-1 takes that block's rows
+// out of the bulk: from LDS, 64 threads per 16-row tile run
+// y_{b-1} -= U[b-1, b] x_b and x_{b-1} = Uinv_{b-1} y_{b-1} on the matrix
+// cores right after x_b arrives (the two tiles were loaded into registers G
+// blocks earlier), publish x_{b-1}, and then run their bulk.  LDS holds -x
+// (the MFMA's B operand, so the accumulators subtract).
+// The bulk's unit of work is a lane's row and four consecutive columns of a
+// tile (16 or 32 bytes) feeding four MFMAs, one per column (their k index runs
+// over the lanes' column quadruples); a ring of RG units keeps RG - 1 in
+// flight, running on across blocks (U does not depend on x), so a block starts
+// with its first loads done.
+template <typename T, int NG> constexpr int rs_ring() { return sizeof(T) == 4 ? 16 : (NG >= 2 ? 4 : 8); }
 
 template <typename T, int G, int NG>
-__global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ UTbase, int64_t ustride,
+__global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ Ubase, int64_t ustride,
                                                   const T *__restrict__ Uinv, int64_t uistride,
                                                   const UGroup *__restrict__ groups, int g0, const T *__restrict__ Yin,
                                                   T *__restrict__ Yf, int64_t ldy, const int *__restrict__ sact,
                                                   T *__restrict__ hand, int *__restrict__ cnt, int64_t hslot) {
-    constexpr int NW = 8, C = kRsCols, V = ut_vec<T>();
-    constexpr int KB = rs_kb<T>(NG);   // k-steps per batch
-    constexpr int NBT = 16 / KB;       // batches per 64-column block
-    constexpr int NS = kRsSets;        // batch register sets: NS - 1 batches in flight
-    static_assert(NBT % NS == 0, "the sets rotate whole blocks");
+    constexpr int NW = 8, C = kRsCols;
+    constexpr int RG = rs_ring<T, NG>();
     constexpr int PT = kT * kT / 512;  // tile elements per thread
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
@@ -774,8 +772,15 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     const int nbu = (n + kT - 1) / kT;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const T *Ut = UTbase + (int64_t)gr.shift * ustride;
+    const T *Ut = Ubase + (int64_t)gr.shift * ustride;
     const T *Ui = Uinv + (int64_t)gr.shift * uistride;
+    auto tile = [&](int I, int J) -> const T * {
+        return Ut + ((int64_t)I * nbu - ((int64_t)I * (I - 1)) / 2 + (J - I)) * (kT * kT);
+    };
+    // element e of a tile: granule column e / (64 UV), row (e / UV) % 64
+    constexpr int UV = utile_gw<T>();
+    auto trow = [](int e) { return (e / UV) & 63; };
+    auto tcol = [](int e) { return (e / (64 * UV)) * UV + (e % UV); };
     T *hx = hand + (int64_t)gl * hslot;
     int *hc = cnt + gl;
     __shared__ T Xs[2][kT][C];        // -x_b, -x_{b-1}
@@ -783,12 +788,8 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     __shared__ T Us[kT][kT + 1];      // Us[m][r] = U(64 cb + r, 64 (cb + 1) + m)
     __shared__ T Is[kT][kT + 1];      // Is[k][i] = Uinv_cb(i, k)
 
-    // The owned rows' right-hand sides as MFMA accumulators, rows interleaved:
-    // tile q of slot j holds the block's rows 4 i + q (i = 0..15), so the A
-    // operand of lane (i, k) is 4 consecutive rows of one column of U, one
-    // 16-byte load for the four tiles.  Accumulator v of tile q in this lane:
-    // row 4 M::row(lane, v) + q of the block, column li.
-    auto rowq = [&](int q, int v) { return 4 * M::row(lane, v) + q; };
+    // the owned rows' right-hand sides, in the accumulator layout: tile rt of
+    // slot j, accumulator v holds row 16 rt + M::row(lane, v) of the block, column li
     acc_t yv[NG][4];
 #pragma unroll
     for (int j = 0; j < NG; j++) {
@@ -797,21 +798,19 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         for (int rt = 0; rt < 4; rt++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = kT * B + rowq(rt, v);
+                const int row = kT * B + 16 * rt + M::row(lane, v);
                 const T y = Yin[min(row, n - 1) + (int64_t)(gr.q0 + min(li, gr.nq - 1)) * ldy];
                 yv[j][rt][v] = (B < nbu && row < n && li < gr.nq) ? y : (T)0;
             }
     }
     // the two tiles of the next block this workgroup solves, one block ahead
+    // (columns past n, in the last block's tile, read as zero)
     T pu[PT], pi[PT];
     auto preload = [&](int cb) __attribute__((always_inline)) {
         if (cb + 1 < nbu) {
+            const T *tp = tile(cb, cb + 1);
 #pragma unroll
-            for (int u = 0; u < PT; u++) {
-                const int e = tid + 512 * u, m = e >> 6, r = e & 63;
-                const int col = min(kT * (cb + 1) + m, n - 1);
-                pu[u] = Ut[(int)ucol_off<V>(col) + kT * cb + r];
-            }
+            for (int u = 0; u < PT; u++) pu[u] = tp[tid + 512 * u];   // masked when written to LDS
         } else {
 #pragma unroll
             for (int u = 0; u < PT; u++) pu[u] = 0;   // the last block: no update tile
@@ -823,30 +822,19 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         const int cb0 = nbu - 1 - ((nbu - 1 - w) % G + G) % G;   // the last block of this workgroup
         if (cb0 >= 0) preload(cb0);
     }
-    // Batch register sets of CAP values: with NA slots active a batch holds
-    // kb(NA) = KB NG / NA k-steps (more columns when fewer slots are active,
-    // so as many bytes stay in flight), value (kk NA + j) 4 + q.  Batch t of
-    // block bb for slots j < NA; rows past the matrix (slots beyond the last
-    // block) are clamped below n.
-    constexpr int CAP = KB * NG * 4;
-    T A[NS][CAP];
-    auto load_batch = [&](auto nac, int bb, int t, T(&a)[CAP]) __attribute__((always_inline)) {
+    // Unit u of block bb with NA slots active: column quadruple g16 = u / (4 NA)
+    // (columns 16 g16 + 4 lk .. + 3), slot j = (u / 4) % NA, row tile rt = u % 4.
+    T A[RG][4];
+    auto load_unit = [&](auto nac, int bb, int u, T(&a)[4]) __attribute__((always_inline)) {
         constexpr int NA = decltype(nac)::value;
-        constexpr int KBA = NA > 0 ? KB * NG / NA : 1;
+        const int g16 = u / (4 * NA), j = (u / 4) % NA, rt = u % 4;
+        const int B = min(G * (wv + NW * j) + w, bb);   // below bb for an active slot
+        const int row = 16 * rt + li, m0 = 16 * g16 + 4 * lk;   // this lane: row, columns m0 .. m0 + 3
+        const T *tp = tile(B, bb);
+        static_assert(UV % 4 == 0, "a lane's four columns lie in one granule");
+        const V4<T> v = ld4u(tp + ((m0 / UV) * kT + row) * UV + (m0 % UV));
 #pragma unroll
-        for (int kk = 0; kk < KBA; kk++) {
-            const int col = min(kT * bb + 4 * (t * KBA + kk) + lk, n - 1);
-            const T *pc = Ut + (int)ucol_off<V>(col);
-#pragma unroll
-            for (int j = 0; j < NA; j++) {
-                // rows 4 li .. 4 li + 3 of the slot's block (a multiple of 4
-                // from the column start: 16-byte aligned), clamped below n
-                const int r0 = min(kT * (G * (wv + NW * j) + w) + 4 * li, (n - 1) & ~3);
-                const V4<T> rows = ld4u(pc + r0);
-#pragma unroll
-                for (int rt = 0; rt < 4; rt++) a[(kk * NA + j) * 4 + rt] = rows[rt];
-            }
-        }
+        for (int q = 0; q < 4; q++) a[q] = v[q];   // columns past n are masked where used
     };
     // active slots of this wave in the bulk of block b: owned blocks below
     // b (below b - 1 where block b-1's x is computed here)
@@ -856,21 +844,14 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         const int first = bl - w - G * wv;
         return first > 0 ? min(NG, (first + G * NW - 1) / (G * NW)) : 0;
     };
-    auto dispatch = [&](int na, auto f) __attribute__((always_inline)) {
-        if constexpr (NG >= 2) {
-            if (na == 2) f(std::integral_constant<int, 2>{});
-            else if (na == 1) f(std::integral_constant<int, 1>{});
-            else f(std::integral_constant<int, 0>{});
-        } else {
-            if (na == 1) f(std::integral_constant<int, 1>{});
-            else f(std::integral_constant<int, 0>{});
-        }
-    };
-    if (nbu >= 2) {   // the first bulk block's first batches
-        dispatch(nact(nbu - 1), [&](auto nac) __attribute__((always_inline)) {
+    if (nbu >= 2) {   // the first bulk block's first units
+        const int na0 = nact(nbu - 1);
+        auto pro = [&](auto nac) __attribute__((always_inline)) {
 #pragma unroll
-            for (int t = 0; t < NS - 1; t++) load_batch(nac, nbu - 1, t, A[t]);
-        });
+            for (int u = 0; u < RG - 1; u++) load_unit(nac, nbu - 1, u, A[u]);
+        };
+        if (na0 >= 2 && NG >= 2) pro(std::integral_constant<int, NG >= 2 ? 2 : 1>{});
+        else if (na0 >= 1) pro(std::integral_constant<int, 1>{});
     }
     bool failed = false;
     for (int b = nbu; b >= 1; b--) {
@@ -883,18 +864,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
         const int nax = na > 0 && b > 1 ? nact(b - 1) : 0;   // the next block's
         auto iter = [&](auto nac, auto nnc) __attribute__((always_inline)) {
             constexpr int NA = decltype(nac)::value, NAX = decltype(nnc)::value;
-            constexpr int KBA = NA > 0 ? KB * NG / NA : 1;
-            constexpr int NBA = 16 / KBA;   // batches of block b
-            auto mma = [&](int t, const T(&a)[CAP]) __attribute__((always_inline)) {
-#pragma unroll
-                for (int kk = 0; kk < KBA; kk++) {
-                    const T bx = Xs[b & 1][4 * (t * KBA + kk) + lk][li];
-#pragma unroll
-                    for (int j = 0; j < NA; j++)
-#pragma unroll
-                        for (int rt = 0; rt < 4; rt++) yv[j][rt] = M::mac(a[(kk * NA + j) * 4 + rt], bx, yv[j][rt]);
-                }
-            };
+            constexpr int NU = 16 * NA;   // units of block b
             // 1. x_b into LDS (from its owner, unless computed here)
             if (have && !mine && (crit || w < bl)) {   // (a workgroup with no rows left skips it)
                 const int need = 4 * (nbu - b);   // four publishing waves per block
@@ -923,13 +893,13 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                             T y = 0;
 #pragma unroll
                             for (int j = 0; j < NG; j++) y = j == os ? yv[j][rt][v] : y;
-                            Ys[rowq(rt, v)][li] = y;
+                            Ys[16 * rt + M::row(lane, v)][li] = y;
                         }
                 }
 #pragma unroll
                 for (int u = 0; u < PT; u++) {
-                    const int e = tid + 512 * u;
-                    Us[e >> 6][e & 63] = pu[u];
+                    const int e = tid + 512 * u;   // columns past n (the last block's tile) as zero
+                    Us[tcol(e)][trow(e)] = kT * b + tcol(e) < n ? pu[u] : (T)0;
                     Is[e >> 6][e & 63] = pi[u];
                 }
             }
@@ -971,25 +941,33 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                 if (b - 1 - G >= 0) preload(b - 1 - G);
             }
             // 4. the bulk: y -= U[rows, block b] x_b for the active slots, the
-            //    loads running NS - 1 batches ahead into block b-1 (its active
-            //    slots are among this block's; block 0 has no successor: a
-            //    repeat of block 1, unused)
+            //    loads running RG - 1 units ahead into block b-1 (its active
+            //    slots are among this block's)
             if constexpr (NA > 0) {
-                static_assert(NBA % NS == 0, "the sets rotate whole blocks");
+                static_assert(NU % RG == 0 && (NAX == 0 || (16 * NAX) % RG == 0), "the ring turns whole blocks");
+                T bx[4];
 #pragma unroll
-                for (int t = 0; t < NBA; t++) {
-                    constexpr int L = NS - 1;
+                for (int u = 0; u < NU; u++) {
+                    constexpr int L = RG - 1;
                     __builtin_amdgcn_sched_barrier(0);
-                    if (t + L < NBA) load_batch(nac, b, t + L, A[(t + L) % NS]);
-                    else if constexpr (NAX > 0) load_batch(nnc, b - 1, t + L - NBA, A[(t + L) % NS]);
+                    if (u + L < NU) load_unit(nac, b, u + L, A[(u + L) % RG]);
+                    else if constexpr (NAX > 0) load_unit(nnc, b - 1, u + L - NU, A[(u + L) % RG]);
                     __builtin_amdgcn_sched_barrier(0);
-                    mma(t, A[t % NS]);
+                    const int g16 = u / (4 * NA), j = (u / 4) % NA, rt = u % 4;
+                    const int lim = n - (kT * b + 16 * g16 + 4 * lk);   // columns past n (the last block): zero
+                    if (u % (4 * NA) == 0) {
+#pragma unroll
+                        for (int q = 0; q < 4; q++) bx[q] = Xs[b & 1][16 * g16 + 4 * lk + q][li];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        yv[j][rt] = M::mac(q < lim ? A[u % RG][q] : (T)0, bx[q], yv[j][rt]);
                 }
             }
         };
         using I0 = std::integral_constant<int, 0>;
         using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
+        using I2 = std::integral_constant<int, NG >= 2 ? 2 : 1>;
         if constexpr (NG >= 2) {
             if (na == 2) {
                 if (nax == 2) iter(I2{}, I2{});
@@ -1433,6 +1411,14 @@ template <typename T, int G, int NG> int rs_capacity() {
     return cap;
 }
 
+// Whether some G <= gmax in {8, 4, 2} gives every workgroup two blocks or more
+// and at most two slots per thread (before any capacity check).
+static bool rs_shape_ok(int nbu, int gmax) {
+    for (int gc : {8, 4, 2})
+        if (gc <= gmax && nbu >= 2 * gc && (nbu + 8 * gc - 1) / (8 * gc) <= 2) return true;
+    return false;
+}
+
 // f(G, NG) with the pair as integral constants (G in {2, 4, 8}, NG in {1, 2})
 template <typename T, typename F> Status with_rs_shape(int G, int NG, F &&f) {
     using std::integral_constant;
@@ -1445,8 +1431,8 @@ template <typename T, typename F> Status with_rs_shape(int G, int NG, F &&f) {
     return ng(integral_constant<int, 8>{});
 }
 
-// The row-split back substitution of real shifts (n <= 8192, 128 rows and
-// up): G workgroups per group of 16 columns, G as large as the knob allows
+// The row-split back substitution of real shifts whose U is in tiles
+// (n <= 8192, 256 rows and up): G workgroups per group of 16 columns, G as large as the knob allows
 // with at least two blocks per workgroup, at most two slots per thread, and
 // the whole launch resident at once (else fewer groups per launch).  Sets
 // *used to false, launching nothing, where it does not apply.
@@ -1456,7 +1442,7 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
     *used = false;
     const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
     const int nbu = (n + kT - 1) / kT;
-    if (gmax < 2 || nwide <= 0 || a.shift_complex || !a.UT || n > 8192) return {};
+    if (gmax < 2 || nwide <= 0 || a.shift_complex || !a.ut || n > 8192) return {};
     int G = 0, NG = 0, cap = 0;
     for (int gc : {8, 4, 2}) {
         if (gc > gmax || nbu < 2 * gc) continue;
@@ -1492,7 +1478,7 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
         LHL_TRY(with_rs_shape<T>(G, NG, [&](auto g, auto s) -> Status {
             constexpr int GC = decltype(g)::value, NGC = decltype(s)::value;
             LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
-                       (const T *)a.UT, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact, hand,
+                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact, hand,
                        cnt, hslot);
             return {};
         }));
@@ -1586,6 +1572,8 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     LHL_TRY(utrsm_rs<T>(h, a, n, dg + ngroups, nwide, (const T *)Yo, Yw, ldy, uis, &rs_used));
     if (rs_used) {
         // launched
+    } else if (a.ut) {
+        return {LHL_ERR_HIP, "solve: U is in tiles but the row-split U solve could not run (hand-off memory)"};
     } else if (a.UT && n <= 512 * 16) {
         // U in packed columns, right-hand sides in registers (512 threads:
         // measured faster than 1024 on this shape)
@@ -1647,7 +1635,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
 
 template <typename T>
 Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride, int64_t nshift,
-                  int cplx, DevBuf &Uinv, int64_t *uistride, DevBuf &UT) {
+                  int cplx, DevBuf &Uinv, int64_t *uistride, DevBuf &UT, int ut) {
     const int n = (int)h->n;
     const int nbu = (n + kT - 1) / kT;
     const int64_t uis = (int64_t)nbu * kT * kT * (cplx ? 2 : 1);
@@ -1656,12 +1644,13 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     *uistride = uis;
     if (cplx)
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
-                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
+                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis, 0);
     else
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
-                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis);
-    // U in packed columns for the back substitution (n <= 8192)
-    if (n <= 512 * 16) {
+                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis, ut);
+    // U in packed columns for the other back substitutions (n <= 8192; U in
+    // tiles is read as it is)
+    if (!ut && n <= 512 * 16) {
         LHL_TRY(UT.ensure(sizeof(T) * (size_t)ustride * (size_t)nshift));
         const int planes = cplx ? 2 : 1;
         LHL_LAUNCH(h, "ut_pack", k_ut_pack<T>, dim3((unsigned)(nbu * (nbu + 1) / 2), (unsigned)(nshift * planes)),
@@ -1806,12 +1795,18 @@ Status conv_device(lhl_handle *h, int it, int max_iter, double tol, int64_t nshi
     return {};
 }
 
+bool pool_tiles(const lhl_handle *h, int64_t n, int is_complex) {
+    if (is_complex || h->cplx || n > 8192) return false;
+    const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
+    return rs_shape_ok((int)((n + kT - 1) / kT), gmax);
+}
+
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
-                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT) {
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT, int ut) {
     if (h->n == 0 || nshift == 0) return {};
     return h->dtype == LHL_F64
-               ? uinv_typed<double>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT)
-               : uinv_typed<float>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT);
+               ? uinv_typed<double>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT, ut)
+               : uinv_typed<float>(h, U, ustride, rdiag, lstride, nshift, is_complex, Uinv, uistride, UT, ut);
 }
 
 Status linv_device(lhl_handle *h) {

@@ -126,6 +126,14 @@ template <> __device__ __forceinline__ Vec2<float> buf_ld2<float>(BufRsrc r, int
 template <> __device__ __forceinline__ Vec2<double> buf_ld2<double>(BufRsrc r, int off) {
     return __builtin_bit_cast(Vec2<double>, __builtin_amdgcn_raw_buffer_load_b128(r, buf_opaque(off), 0, 0));
 }
+__device__ __forceinline__ void buf_st2(Vec2<float> v, BufRsrc r, int off) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 0);
+}
+__device__ __forceinline__ void buf_st2(Vec2<double> v, BufRsrc r, int off) {
+    using U4 = unsigned __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, v), r, buf_opaque(off), 0, 0);
+}
 __device__ __forceinline__ void buf_st2_nt(Vec2<float> v, BufRsrc r, int off) {
     using U2 = unsigned __attribute__((ext_vector_type(2)));
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, buf_opaque(off), 0, 2);
@@ -215,6 +223,29 @@ template <typename T> constexpr int ut_vec() { return sizeof(T) == 4 ? 4 : 2; }
 template <int V> __host__ __device__ inline int64_t ucol_off(int64_t j) {
     const int64_t a = j / V, b = j % V;
     return V * ((V * a * (a + 1)) / 2 + b * (a + 1));
+}
+
+// U of a real shift in 64x64 tiles (the batched solves' pool, n <= 8192):
+// the upper triangle's tiles (I, J), J >= I, one after another, tile row I
+// holding J = I .. nbu-1, each tile row-major.  The shift LU's row stores stay
+// whole 256-byte runs per tile, and the U solve reads a tile (one 64-column
+// block of 64 rows) as one contiguous piece.  Element (k, j), j >= k, sits at
+// utile_row<V>(nbu, k) + utile_col<V>(j) with V = utile_gw<T>() = 64; the
+// formulas also describe tiles of V-column granules stored granule column by
+// granule column ((64 g + r) V), which read better in the U solve and wrote
+// worse in the shift LU (FP32 with 16-byte granules: 118.5 k against 121.4 k
+// solves/s on the 4096 x 64 shifts x 16 batch).
+template <typename T> constexpr int utile_gw() { return 64; }
+template <int V> __host__ __device__ inline int64_t utile_row(int64_t nbu, int64_t k) {
+    const int64_t I = k >> 6;
+    return (I * nbu - (I * (I - 1)) / 2 - I) * 4096 + (k & 63) * V;
+}
+template <int V> __host__ __device__ inline int64_t utile_col(int64_t j) {
+    return ((j >> 6) << 12) + ((j & 63) / V) * (64 * V) + (j & (V - 1));
+}
+inline int64_t utile_plane(int64_t n) {
+    const int64_t nbu = (n + 63) / 64;
+    return nbu * (nbu + 1) / 2 * 4096;
 }
 
 // Device blocks kept by a handle between calls (per-call buffers of the
@@ -490,6 +521,7 @@ Status shift_device(lhl_handle *h, lhl_shift_state *sh, int64_t nshift, const do
 struct ShiftPool {
     int64_t n = 0, count = 0;
     int planes = 1;
+    int ut = 0;            // U in 64x64 tiles (utile_row / utile_col) instead of packed rows
     int64_t ustride = 0;   // elements between consecutive U's (per plane stride = ustride/planes)
     DevBuf U, l, rdiag, swp, dinfo, params, Uinv, UT;
 };
@@ -503,6 +535,7 @@ struct SolveArgs {
     const void *U, *l, *rdiag;
     const void *Uinv = nullptr;   // inverses of U's 64x64 diagonal blocks (per shift, uistride apart)
     const void *UT = nullptr;     // U in packed columns, ustride apart (nullptr: solve from the packed rows)
+    int ut = 0;                   // U is in 64x64 tiles (read by the row-split U solve; UT unused)
     int64_t uistride = 0;
     const uint8_t *swp;
     int64_t ustride = 0;          // per shift, elements (both planes)
@@ -548,7 +581,12 @@ Status groups_device(lhl_handle *h, int64_t nshift, const int64_t *cb, int plane
 // U's (per-shift stride returned in *uistride) and U in packed columns (UT,
 // ustride apart)
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,
-                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT);   // lhl_batch.hip
+                   int64_t nshift, int is_complex, DevBuf &Uinv, int64_t *uistride, DevBuf &UT,
+                   int ut = 0);   // lhl_batch.hip (ut: U in 64x64 tiles, no packed columns made)
+// whether a batched solve's shift pool keeps U in 64x64 tiles: real shifts
+// with 256 <= n <= 8192, where the shift LU writes them and the row-split U
+// solve reads them (the knob utrsm_split = 1 keeps packed rows)
+bool pool_tiles(const lhl_handle *h, int64_t n, int is_complex);   // lhl_batch.hip
 // the inverses of L's 64x64 diagonal blocks, formed once per reduction
 Status linv_device(lhl_handle *h);                          // lhl_batch.hip
 Status applyz_device(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t ldx, int64_t xplane,

@@ -251,9 +251,9 @@ __global__ void __launch_bounds__(NT) k_shift_real(const T *__restrict__ Ht, int
 // its loads and stores moves W elements: the bulk is bound by its memory
 // operations (loads and stores share one in-order counter, so the stores'
 // completion limits the prefetch).  With KP even, the first bulk column, a
-// multiple of KP, never splits a pair.  U is written with non-temporal stores
-// (it is read back only after the kernel, and a batch of shifts' U far exceeds
-// the caches).  Each element sees the reference's operations in the
+// multiple of KP, never splits a pair.  U is written with ordinary stores
+// (measured as fast as non-temporal ones with packed rows and tiles alike).
+// Each element sees the reference's operations in the
 // reference's order, as in the one-step kernel above, so the results are
 // bit-identical to it.
 
@@ -266,6 +266,7 @@ template <typename T> struct SplitArgs {
     int G = 1, nsh = 0, nwin = 0;
     T *hand = nullptr;
     int *cnt = nullptr;
+    int ut = 0;   // U in 64x64 tiles (utile_row / utile_col) instead of packed rows
 };
 // polls of a hand-off count before it is given up (about a second; never
 // reached while every workgroup of the shift is resident, which the launch
@@ -278,14 +279,14 @@ template <typename T, int W> struct ColGroup;
 template <typename T> struct ColGroup<T, 1> {
     using V = T;
     static __device__ __forceinline__ V ld(BufRsrc r, int off) { return buf_ld<T>(r, off); }
-    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st_nt(v, r, off); }
+    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st(v, r, off); }
     static __device__ __forceinline__ T get(V v, int) { return v; }
     static __device__ __forceinline__ void set(V &v, int, T x) { v = x; }
 };
 template <typename T> struct ColGroup<T, 2> {
     using V = Vec2<T>;
     static __device__ __forceinline__ V ld(BufRsrc r, int off) { return buf_ld2<T>(r, off); }
-    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st2_nt(v, r, off); }
+    static __device__ __forceinline__ void st(V v, BufRsrc r, int off) { buf_st2(v, r, off); }
     static __device__ __forceinline__ T get(V v, int e) { return v[e]; }
     static __device__ __forceinline__ void set(V &v, int e, T x) { v[e] = x; }
 };
@@ -341,6 +342,11 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         return (g + GS * wl) * KP + (o - wl * KP);
     };
     auto owns = [&](int w) -> bool { return GS == 1 || w % GS == g; };
+    // U[k, j] (j >= k) at urow(k) + ucol(j): packed rows, or 64x64 tiles
+    const int nbu = (n + 63) / 64;
+    constexpr int UV = utile_gw<T>();   // columns per granule of the tiles
+    auto urow = [&](int k) -> int64_t { return sp.ut ? utile_row<UV>(nbu, k) : upack_off(n, k) - k; };
+    auto ucol = [&](int j) -> int { return sp.ut ? (int)utile_col<UV>(j) : j; };
     bool failed = false;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
@@ -432,6 +438,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         constexpr bool FULL = decltype(fullc)::value;
         const T *tl = &htile[set][1][0][0];
         T uvec = 1;
+        const int ucl = ucol(k0 + lane), uclast = ucol(n - 1);
 #pragma unroll
         for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
 #pragma nounroll
@@ -459,9 +466,11 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T rn = g - l * p;       // no swap: new row minus l * pending row
                 const T u = swk ? (first ? g + sigma : g) : p;
                 const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
-                // U[k, c] for window columns c >= k (lane i: the diagonal, a or b)
-                const BufRsrc ur = buf_rsrc(U + upack_off(n, k), (n - k) * ES);
-                buf_st_nt(u, ur, (wr && lane >= i && lane < kPanel) ? (lane - i) * ES : kBufNone);
+                // U[k, c] for window columns c = k0 + lane >= k (lane i: the
+                // diagonal, a or b); columns past n - 1 fall past the row
+                const int uk = ucol(k);
+                const BufRsrc ur = buf_rsrc(U + urow(k) + uk, (uclast - uk + 1) * ES);
+                buf_st(u, ur, (wr && lane >= i && lane < kPanel) ? (ucl - uk) * ES : kBufNone);
                 lvec = lane == i ? l : lvec;
                 svec = lane == i ? (int)swk : svec;
                 uvec = lane == i ? (swk ? b : a) : uvec;
@@ -491,9 +500,9 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // diagonal (sigma) in column c1
     auto window_apply = [&](int k0, int set) __attribute__((always_inline)) {
         const int c1 = k0 + kPanel;
-        const int off = (owns(c1 / kPanel) && lane < kPanel) ? lane * ES : kBufNone;
-        const int ubytes = (n - c1) * ES;
-        int64_t uo = upack_off(n, k0);
+        const int uc1 = ucol(c1);
+        const int off = (owns(c1 / kPanel) && lane < kPanel) ? (ucol(c1 + lane) - uc1) * ES : kBufNone;
+        const int ubytes = (ucol(n - 1) - uc1 + 1) * ES;
         const T *tl = &htile[set][0][0][0];
 #pragma unroll
         for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
@@ -511,9 +520,8 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const T rn = g - l * p;
             const T u = swk ? (first ? g + sigma : g) : p;
             p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
-            const BufRsrc ur = buf_rsrc(U + uo + (c1 - k), ubytes);
-            uo += n - k;
-            buf_st_nt(u, ur, off);
+            const BufRsrc ur = buf_rsrc(U + urow(k) + uc1, ubytes);
+            buf_st(u, ur, off);
             load_t(tl, i + PD, hw[ii]);
         }
     };
@@ -658,18 +666,19 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             // (exact waits for the prefetched rows)
             auto bulk = [&](auto msc) __attribute__((always_inline)) {
                 constexpr int MS = decltype(msc)::value;
-                // U[k, j] at byte (j - c2) ES from U[k, c2]; groups below c2 store
-                // nothing, and a group's elements at columns >= n fall past the row
+                // U[k, j] at byte (ucol(j) - ucol(c2)) ES from U[k, c2]; groups
+                // below c2 store nothing, and a group's elements at columns >= n
+                // fall past the row
+                const int uc2 = ucol(c2);
                 int boff[NP];
 #pragma unroll
                 for (int m = MS; m < NP; m++) {
                     const int j0 = col_of(W * (tb + m * NTB));
-                    boff[m] = j0 >= c2 ? (j0 - c2) * ES : kBufNone;
+                    boff[m] = j0 >= c2 ? (ucol(j0) - uc2) * ES : kBufNone;
                 }
                 const T lvv = lsh[bp][lane];
                 const int svv = ssh[bp][lane];
-                const int ubytes = (n - c2) * ES;
-                int64_t uo = upack_off(n, k0);
+                const int ubytes = (ucol(n - 1) - uc2 + 1) * ES;
                 for (int i0 = 0; i0 < kPanel; i0 += R) {
 #pragma unroll
                     for (int q = 0; q < R; q++) {
@@ -677,8 +686,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                         const int k = k0 + i;
                         const T l = rdlane(lvv, i);
                         const int sk = __builtin_amdgcn_readlane(svv, i);
-                        const BufRsrc ur = buf_rsrc(U + uo + (c2 - k), ubytes);
-                        uo += n - k;
+                        const BufRsrc ur = buf_rsrc(U + urow(k) + uc2, ubytes);
                         // values only in the two branches, then the stores: every
                         // path issues the same memory operations
                         T2 u[NP];
@@ -752,7 +760,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // the last diagonal: U[n-1, n-1] = p[n-1]
     if (tid == 0 && g == 0) {
         const T pl = plast;
-        U[upack_off(n, n - 1)] = pl;
+        U[urow(n - 1) + ucol(n - 1)] = pl;
         rd[n - 1] = (T)1 / pl;
         sw[n - 1] = 0;
         if (pl == (T)0 && info == 0) info = n;
@@ -1019,8 +1027,10 @@ __global__ void k_split_reset(int *__restrict__ cnt, int64_t ncnt, int *__restri
 template <typename T>
 Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
-                      int64_t ss, int *info) {
+                      int64_t ss, int *info, int ut = 0) {
     static_assert(sizeof(ShiftParam<T>) == 4 * sizeof(T), "ShiftParam is four T: (sr, si, tr, ti)");
+    if (ut && (cplx || h->cplx || n > 8192))
+        return {LHL_ERR_ARG, "lhl_shift: U in tiles is written by the real panel kernel only"};
     if (h->cplx)   // a complex H (a complex J's reduction)
         return zshift_launch<T>(h, nshift, reinterpret_cast<const T *>(prm), U, us, l, rd, ls, sw, ss, info);
     constexpr bool F32 = sizeof(T) == 4;
@@ -1030,6 +1040,7 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     const bool wide = h->tune_shift_wide > 0 && n >= h->tune_shift_wide;
     if (!cplx && n <= (F32 ? 14 : 7) * BW) {
         SplitArgs<T> sp;
+        sp.ut = ut;
         sp.nsh = (int)nshift;
         sp.nwin = (n + KP - 1) / KP;
         const int gmax = h->tune_shift_split > 0 ? h->tune_shift_split : kSplitDefault;
@@ -1092,7 +1103,7 @@ __global__ void k_shift_params(int64_t nshift, const double *__restrict__ src, S
 
 template <typename T>
 Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, const double *tau, DevBuf &params,
-                  T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info) {
+                  T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss, int *info, int ut = 0) {
     if (h->dev_shiftpar) {
         // the shifts are on the device already (a plan's step)
         LHL_TRY(params.ensure(sizeof(ShiftParam<T>) * (size_t)nshift));
@@ -1100,7 +1111,7 @@ Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, co
                    h->dev_shiftpar, params.as<ShiftParam<T>>());
         ProfWork pw(h, nshift, 0);
         return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
-                                 params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info);
+                                 params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info, ut);
     }
     std::vector<ShiftParam<T>> hp((size_t)nshift);
     for (int64_t s = 0; s < nshift; s++) {
@@ -1118,7 +1129,7 @@ Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, co
                                h->stream));
     ProfWork pw(h, nshift, 0);
     return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
-                             params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info);
+                             params.as<ShiftParam<T>>(), U, us, l, rd, ls, sw, ss, info, ut);
 }
 
 }  // namespace
@@ -1130,8 +1141,9 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
     pool.n = n;
     pool.count = nshift;
     pool.planes = is_complex ? 2 : 1;
+    pool.ut = pool_tiles(h, n, is_complex) ? 1 : 0;
     const int64_t uplane = upack_plane(n);
-    pool.ustride = uplane * pool.planes;
+    pool.ustride = pool.ut ? utile_plane(n) : uplane * pool.planes;
     const int64_t lplane = ((n + 63) / 64) * 64;
     LHL_TRY(pool.U.ensure(es * (size_t)pool.ustride * (size_t)nshift));
     LHL_TRY(pool.l.ensure(es * (size_t)lplane * pool.planes * (size_t)nshift));
@@ -1141,14 +1153,14 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
     if (h->dtype == LHL_F64)
         LHL_TRY(shift_into<double>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<double>(), pool.ustride,
                                    pool.l.as<double>(), pool.rdiag.as<double>(), lplane * pool.planes,
-                                   pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>()));
+                                   pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>(), pool.ut));
     else
         LHL_TRY(shift_into<float>(h, is_complex, nshift, sig, tau, pool.params, pool.U.as<float>(), pool.ustride,
                                   pool.l.as<float>(), pool.rdiag.as<float>(), lplane * pool.planes,
-                                  pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>()));
+                                  pool.swp.as<uint8_t>(), lplane, pool.dinfo.as<int>(), pool.ut));
     int64_t uis = 0;
     return uinv_device(h, pool.U.p, pool.ustride, pool.rdiag.p, lplane * pool.planes, nshift, is_complex, pool.Uinv,
-                       &uis, pool.UT);
+                       &uis, pool.UT, pool.ut);
 }
 
 // One shift state (ws.shift or an LHLShift): storage is the pool layout with
