@@ -268,8 +268,8 @@ def algorithmic(kernel, n, S, ncol, es):
 # which reads the FP64 J (134.2 MB) once and writes its row-major copy once:
 # FETCH_SIZE counts 0.501 and WRITE_SIZE 1.000 of those bytes.
 MEASURED_TRAFFIC_N4096_S64X16_F32 = {
-    "shift": 2.8262e9, "uinv": 0.1337e9, "ut_pack": 5.2995e9, "gather": 0.1922e9,
-    "fwd_chain": 0.1201e9, "utrsm": 2.2292e9, "scatter": 0.1157e9,
+    "shift": 2.725e9, "uinv": 0.118e9, "gather": 0.192e9,
+    "fwd_chain": 0.120e9, "utrsm": 2.461e9, "scatter": 0.116e9,
 }
 
 

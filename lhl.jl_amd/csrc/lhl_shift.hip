@@ -1065,7 +1065,7 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
         }
         if (sp.G > 1) {
             const int64_t nc = nshift * sp.nwin;
-            LHL_LAUNCH(h, "shift", k_split_reset, dim3((unsigned)((std::max<int64_t>(nc, nshift) + 255) / 256)),
+            LHL_LAUNCH(h, "shift_reset", k_split_reset, dim3((unsigned)((std::max<int64_t>(nc, nshift) + 255) / 256)),
                        dim3(256), 0, sp.cnt, nc, info, nshift);
         }
         const int ncols = sp.G > 1 ? KP * ((sp.nwin + sp.G - 1) / sp.G) : n;
@@ -1107,7 +1107,7 @@ Status shift_into(lhl_handle *h, int cplx, int64_t nshift, const double *sig, co
     if (h->dev_shiftpar) {
         // the shifts are on the device already (a plan's step)
         LHL_TRY(params.ensure(sizeof(ShiftParam<T>) * (size_t)nshift));
-        LHL_LAUNCH(h, "shift", k_shift_params<T>, dim3((unsigned)((nshift + 255) / 256)), dim3(256), 0, nshift,
+        LHL_LAUNCH(h, "shift_params", k_shift_params<T>, dim3((unsigned)((nshift + 255) / 256)), dim3(256), 0, nshift,
                    h->dev_shiftpar, params.as<ShiftParam<T>>());
         ProfWork pw(h, nshift, 0);
         return launch_shift_n<T>(h, cplx, nshift, (const T *)h->Ht.as<T>(), h->lda, (int)h->n,
