@@ -1564,7 +1564,7 @@ const Knob kKnobs[] = {
     {"reduce_blocked_min", 3, INT64_MAX}, {"reduce_nb", 8, 16},          {"reduce_rmin", 1, 64},
     {"reduce_switch", 0, 1000},           {"lsweep_depth", 0, 4},         {"shift_wide_min", 1, 1 << 20},
     {"shift_pool_bytes", 1, INT64_MAX},   {"h2d_chunk_bytes", 0, (int64_t)1 << 34}, {"gemm_impl", 0, 1},
-    {"group_bytes", 1, INT64_MAX},        {"shift_split", 1, 4},          {"utrsm_split", 1, 8},
+    {"group_bytes", 1, INT64_MAX},        {"shift_split", 1, 8},          {"utrsm_split", 1, 8},
 };
 int64_t *knob_slot(lhl_handle *h, const std::string &k) {
     if (k == "reduce_blocked_min") return &h->tune_blocked_min;

@@ -200,14 +200,31 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
     const int64_t up = ustride / 2, lp = lstride / 2;
     __shared__ T Us[UP][kT][kT + 1];
     __shared__ T ds[UP][kT];
-    for (int t = threadIdx.x; t < kT * kT; t += 64) {
-        const int c = t % kT, r = t / kT;
-        const bool ok = r < nb && c < nb && c > r;
-        const int rr = ok ? r0 + r : 0, cc = ok ? r0 + c : 0;
+    // the block's strict upper triangle: lane c loads column c of every row,
+    // all loads issued before the first is used (a select on each value right
+    // after its load would wait for it: 64 round trips)
+    {
+        const int c = threadIdx.x;
         constexpr int UV = utile_gw<T>();
-        const int64_t o = ut ? utile_row<UV>((n + 63) / 64, rr) + utile_col<UV>(cc) : upack_off(n, rr) - rr + cc;
-        Us[0][r][c] = ok ? U[o] : (T)0;
-        if constexpr (UC) Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
+        if constexpr (UC) {   // complex: one row at a time (registers)
+            for (int r = 0; r < kT; r++) {
+                const bool ok = r < nb && c < nb && c > r;
+                const int64_t o = upack_off(n, ok ? r0 + r : 0) - (ok ? r0 + r : 0) + (ok ? r0 + c : 0);
+                Us[0][r][c] = ok ? U[o] : (T)0;
+                Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
+            }
+        } else {
+            T v[kT];
+#pragma unroll
+            for (int r = 0; r < kT; r++) {
+                const bool ok = r < nb && c < nb && c > r;
+                const int rr = ok ? r0 + r : 0, cc = ok ? r0 + c : 0;
+                const int64_t o = ut ? utile_row<UV>((n + 63) / 64, rr) + utile_col<UV>(cc) : upack_off(n, rr) - rr + cc;
+                v[r] = U[o];
+            }
+#pragma unroll
+            for (int r = 0; r < kT; r++) Us[0][r][c] = (r < nb && c < nb && c > r) ? v[r] : (T)0;
+        }
     }
     {
         const int t = threadIdx.x;
@@ -219,16 +236,21 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
     T xr[kT], xi[kT];
 #pragma unroll
     for (int i = kT - 1; i >= 0; i--) {
-        T ar = (i == c) ? (T)1 : (T)0, ai = 0;
+        // real shifts: four partial sums over j mod 4 (a quarter of the
+        // dependent chain); complex: one (registers)
+        constexpr int NPS = UC ? 1 : 4;
+        T pr[4] = {(i == c) ? (T)1 : (T)0, 0, 0, 0}, pim[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int j = i + 1; j < kT; j++) {
-            ar = fma_t(-Us[0][i][j], xr[j], ar);
+            const int q = (j - i - 1) % NPS;
+            pr[q] = fma_t(-Us[0][i][j], xr[j], pr[q]);
             if constexpr (UC) {
-                ar = fma_t(Us[UP - 1][i][j], xi[j], ar);
-                ai = fma_t(-Us[0][i][j], xi[j], ai);
-                ai = fma_t(-Us[UP - 1][i][j], xr[j], ai);
+                pr[q] = fma_t(Us[UP - 1][i][j], xi[j], pr[q]);
+                pim[q] = fma_t(-Us[0][i][j], xi[j], pim[q]);
+                pim[q] = fma_t(-Us[UP - 1][i][j], xr[j], pim[q]);
             }
         }
+        const T ar = (pr[0] + pr[1]) + (pr[2] + pr[3]), ai = (pim[0] + pim[1]) + (pim[2] + pim[3]);
         if constexpr (UC) {
             const T dr = ds[0][i], di = ds[UP - 1][i];
             xr[i] = (i <= c) ? ar * dr - ai * di : (T)0;

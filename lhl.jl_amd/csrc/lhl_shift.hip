@@ -1044,7 +1044,7 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
         sp.nsh = (int)nshift;
         sp.nwin = (n + KP - 1) / KP;
         const int gmax = h->tune_shift_split > 0 ? h->tune_shift_split : kSplitDefault;
-        for (int gc = std::min(gmax, 4); gc > 1; gc /= 2) {
+        for (int gc = std::min(gmax, 8); gc > 1; gc /= 2) {
             if (sp.nwin < 2 * gc + 2) continue;   // too few windows to share
             const int ncols = KP * ((sp.nwin + gc - 1) / gc);
             int cap = 0;

@@ -62,7 +62,7 @@ def test_split_shift_bitwise(lhl_hip, oracle, monkeypatch, dtype):
         ws = lhl_hip.LHLWorkspace(n, dtype)
         lhl_hip.lhl_(ws, J)
         Ht = ws.Ht
-        for split in (1, 2, 4):
+        for split in (1, 2, 4, 8):
             say(f"case: split shift {np.dtype(dtype).name} n={n} split={split}")
             lhl_hip.tune_set(ws, "shift_split", split)
             for s, t in ((1, -0.1), (0.5, -3)):
