@@ -209,7 +209,8 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
         if constexpr (UC) {   // complex: one row at a time (registers)
             for (int r = 0; r < kT; r++) {
                 const bool ok = r < nb && c < nb && c > r;
-                const int64_t o = upack_off(n, ok ? r0 + r : 0) - (ok ? r0 + r : 0) + (ok ? r0 + c : 0);
+                const int rr = ok ? r0 + r : 0, cc = ok ? r0 + c : 0;
+                const int64_t o = ut ? utile_row<UV>((n + 63) / 64, rr) + utile_col<UV>(cc) : upack_off(n, rr) - rr + cc;
                 Us[0][r][c] = ok ? U[o] : (T)0;
                 Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
             }
@@ -774,17 +775,20 @@ constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wai
 // over the lanes' column quadruples); a ring of RG units keeps RG - 1 in
 // flight, running on across blocks (U does not depend on x), so a block starts
 // with its first loads done.
-template <typename T, int NG> constexpr int rs_ring() { return sizeof(T) == 4 ? 16 : (NG >= 2 ? 4 : 8); }
+template <typename T, int NG, int UC> constexpr int rs_ring() {
+    return UC ? (sizeof(T) == 4 ? 8 : 4) : (sizeof(T) == 4 ? 16 : (NG >= 2 ? 4 : 8));
+}
 
-template <typename T, int G, int NG>
+template <typename T, int G, int NG, int UC>
 __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ Ubase, int64_t ustride,
                                                   const T *__restrict__ Uinv, int64_t uistride,
                                                   const UGroup *__restrict__ groups, int g0, const T *__restrict__ Yin,
                                                   T *__restrict__ Yf, int64_t ldy, const int *__restrict__ sact,
                                                   T *__restrict__ hand, int *__restrict__ cnt, int64_t hslot) {
     constexpr int NW = 8, C = kRsCols;
-    constexpr int RG = rs_ring<T, NG>();
+    constexpr int RG = rs_ring<T, NG, UC>();
     constexpr int PT = kT * kT / 512;  // tile elements per thread
+    constexpr int UP = UC ? 2 : 1;     // planes of U and Uinv (complex shifts: real, imaginary)
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     const int gl = blockIdx.x / G, w = blockIdx.x % G;
@@ -796,6 +800,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     const int li = lane & 15, lk = lane >> 4;
     const T *Ut = Ubase + (int64_t)gr.shift * ustride;
     const T *Ui = Uinv + (int64_t)gr.shift * uistride;
+    const int64_t up = ustride / 2;    // the imaginary plane of U (complex shifts)
     auto tile = [&](int I, int J) -> const T * {
         return Ut + ((int64_t)I * nbu - ((int64_t)I * (I - 1)) / 2 + (J - I)) * (kT * kT);
     };
@@ -807,8 +812,13 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     int *hc = cnt + gl;
     __shared__ T Xs[2][kT][C];        // -x_b, -x_{b-1}
     __shared__ T Ys[kT][C + 1];       // the rows of the block whose x is computed here next
-    __shared__ T Us[kT][kT + 1];      // Us[m][r] = U(64 cb + r, 64 (cb + 1) + m)
-    __shared__ T Is[kT][kT + 1];      // Is[k][i] = Uinv_cb(i, k)
+    __shared__ T Us[UP][kT][kT + 1];  // Us[p][m][r] = U(64 cb + r, 64 (cb + 1) + m), plane p
+    __shared__ T Is[UP][kT][kT + 1];  // Is[p][k][i] = Uinv_cb(i, k), plane p
+    // Complex shifts: internal columns 2c, 2c+1 are one complex column's real
+    // and imaginary parts, so U x = Ur X + Ui X' with X' = (-x_im, x_re) per
+    // column: the imaginary plane's products take their B operand from the
+    // partner column (li ^ 1) with the sign of -X' (rot below).
+    auto rot = [&](T v) -> T { return (li & 1) ? v : -v; };
 
     // the owned rows' right-hand sides, in the accumulator layout: tile rt of
     // slot j, accumulator v holds row 16 rt + M::row(lane, v) of the block, column li
@@ -827,18 +837,24 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     }
     // the two tiles of the next block this workgroup solves, one block ahead
     // (columns past n, in the last block's tile, read as zero)
-    T pu[PT], pi[PT];
+    T pu[UP][PT], pi[UP][PT];
     auto preload = [&](int cb) __attribute__((always_inline)) {
         if (cb + 1 < nbu) {
             const T *tp = tile(cb, cb + 1);
 #pragma unroll
-            for (int u = 0; u < PT; u++) pu[u] = tp[tid + 512 * u];   // masked when written to LDS
+            for (int p = 0; p < UP; p++)
+#pragma unroll
+                for (int u = 0; u < PT; u++) pu[p][u] = tp[p * up + tid + 512 * u];   // masked when written to LDS
         } else {
 #pragma unroll
-            for (int u = 0; u < PT; u++) pu[u] = 0;   // the last block: no update tile
+            for (int p = 0; p < UP; p++)
+#pragma unroll
+                for (int u = 0; u < PT; u++) pu[p][u] = 0;   // the last block: no update tile
         }
 #pragma unroll
-        for (int u = 0; u < PT; u++) pi[u] = Ui[(int64_t)cb * kT * kT + tid + 512 * u];
+        for (int p = 0; p < UP; p++)
+#pragma unroll
+            for (int u = 0; u < PT; u++) pi[p][u] = Ui[((int64_t)cb * UP + p) * kT * kT + tid + 512 * u];
     };
     {
         const int cb0 = nbu - 1 - ((nbu - 1 - w) % G + G) % G;   // the last block of this workgroup
@@ -846,17 +862,20 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
     }
     // Unit u of block bb with NA slots active: column quadruple g16 = u / (4 NA)
     // (columns 16 g16 + 4 lk .. + 3), slot j = (u / 4) % NA, row tile rt = u % 4.
-    T A[RG][4];
-    auto load_unit = [&](auto nac, int bb, int u, T(&a)[4]) __attribute__((always_inline)) {
+    T A[RG][4 * UP];
+    auto load_unit = [&](auto nac, int bb, int u, T(&a)[4 * UP]) __attribute__((always_inline)) {
         constexpr int NA = decltype(nac)::value;
         const int g16 = u / (4 * NA), j = (u / 4) % NA, rt = u % 4;
         const int B = min(G * (wv + NW * j) + w, bb);   // below bb for an active slot
         const int row = 16 * rt + li, m0 = 16 * g16 + 4 * lk;   // this lane: row, columns m0 .. m0 + 3
         const T *tp = tile(B, bb);
         static_assert(UV % 4 == 0, "a lane's four columns lie in one granule");
-        const V4<T> v = ld4u(tp + ((m0 / UV) * kT + row) * UV + (m0 % UV));
 #pragma unroll
-        for (int q = 0; q < 4; q++) a[q] = v[q];   // columns past n are masked where used
+        for (int p = 0; p < UP; p++) {
+            const V4<T> v = ld4u(tp + p * up + ((m0 / UV) * kT + row) * UV + (m0 % UV));
+#pragma unroll
+            for (int q = 0; q < 4; q++) a[4 * p + q] = v[q];   // columns past n are masked where used
+        }
     };
     // active slots of this wave in the bulk of block b: owned blocks below
     // b (below b - 1 where block b-1's x is computed here)
@@ -919,11 +938,13 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                         }
                 }
 #pragma unroll
-                for (int u = 0; u < PT; u++) {
-                    const int e = tid + 512 * u;   // columns past n (the last block's tile) as zero
-                    Us[tcol(e)][trow(e)] = kT * b + tcol(e) < n ? pu[u] : (T)0;
-                    Is[e >> 6][e & 63] = pi[u];
-                }
+                for (int p = 0; p < UP; p++)
+#pragma unroll
+                    for (int u = 0; u < PT; u++) {
+                        const int e = tid + 512 * u;   // columns past n (the last block's tile) as zero
+                        Us[p][tcol(e)][trow(e)] = kT * b + tcol(e) < n ? pu[p][u] : (T)0;
+                        Is[p][e >> 6][e & 63] = pi[p][u];
+                    }
             }
             lds_barrier();
             // 3. x_{b-1} (waves 0-3, one 16-row tile each), published
@@ -935,8 +956,12 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
 #pragma unroll
                         for (int v = 0; v < 4; v++) acc[v] = Ys[16 * rt + M::row(lane, v)][li];
 #pragma unroll
-                        for (int ks = 0; ks < 16; ks++)
-                            acc = M::mac(Us[4 * ks + lk][16 * rt + li], Xs[b & 1][4 * ks + lk][li], acc);
+                        for (int ks = 0; ks < 16; ks++) {
+                            acc = M::mac(Us[0][4 * ks + lk][16 * rt + li], Xs[b & 1][4 * ks + lk][li], acc);
+                            if constexpr (UC)
+                                acc = M::mac(Us[1][4 * ks + lk][16 * rt + li], rot(Xs[b & 1][4 * ks + lk][li ^ 1]),
+                                             acc);
+                        }
 #pragma unroll
                         for (int v = 0; v < 4; v++) Ys[16 * rt + M::row(lane, v)][li] = acc[v];
                     }
@@ -945,7 +970,12 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                 if (wv < 4) {
                     acc_t acc = M::zero();
 #pragma unroll
-                    for (int ks = 0; ks < 16; ks++) acc = M::mac(Is[4 * ks + lk][16 * rt + li], Ys[4 * ks + lk][li], acc);
+                    for (int ks = 0; ks < 16; ks++) {
+                        acc = M::mac(Is[0][4 * ks + lk][16 * rt + li], Ys[4 * ks + lk][li], acc);
+                        // x = Ir Y + Ii Y' with Y' = (-y_im, y_re) = rot of the partner column
+                        if constexpr (UC)
+                            acc = M::mac(Is[1][4 * ks + lk][16 * rt + li], rot(Ys[4 * ks + lk][li ^ 1]), acc);
+                    }
                     T *dst = hx + (int64_t)(b - 1) * kT * C;
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
@@ -967,7 +997,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
             //    slots are among this block's)
             if constexpr (NA > 0) {
                 static_assert(NU % RG == 0 && (NAX == 0 || (16 * NAX) % RG == 0), "the ring turns whole blocks");
-                T bx[4];
+                T bx[4], bq[4];
 #pragma unroll
                 for (int u = 0; u < NU; u++) {
                     constexpr int L = RG - 1;
@@ -979,11 +1009,16 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
                     const int lim = n - (kT * b + 16 * g16 + 4 * lk);   // columns past n (the last block): zero
                     if (u % (4 * NA) == 0) {
 #pragma unroll
-                        for (int q = 0; q < 4; q++) bx[q] = Xs[b & 1][16 * g16 + 4 * lk + q][li];
+                        for (int q = 0; q < 4; q++) {
+                            bx[q] = Xs[b & 1][16 * g16 + 4 * lk + q][li];
+                            if constexpr (UC) bq[q] = rot(Xs[b & 1][16 * g16 + 4 * lk + q][li ^ 1]);
+                        }
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; q++)
+                    for (int q = 0; q < 4; q++) {
                         yv[j][rt] = M::mac(q < lim ? A[u % RG][q] : (T)0, bx[q], yv[j][rt]);
+                        if constexpr (UC) yv[j][rt] = M::mac(q < lim ? A[u % RG][4 + q] : (T)0, bq[q], yv[j][rt]);
+                    }
                 }
             }
         };
@@ -1419,11 +1454,11 @@ __global__ void k_rs_reset(int *__restrict__ cnt, int n) {
 
 // Workgroups of k_utrsm_rs<T, G, NG> resident on the whole device at once
 // (0 if the query fails, which leaves the row split off).
-template <typename T, int G, int NG> int rs_capacity() {
+template <typename T, int G, int NG, int UC> int rs_capacity() {
     static const int cap = [] {
         int nb = 0, dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_utrsm_rs<T, G, NG>, 512, 0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_utrsm_rs<T, G, NG, UC>, 512, 0) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
             (void)hipGetLastError();
             return 0;
@@ -1464,7 +1499,9 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
     *used = false;
     const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
     const int nbu = (n + kT - 1) / kT;
-    if (gmax < 2 || nwide <= 0 || a.shift_complex || !a.ut || n > 8192) return {};
+    if (gmax < 2 || nwide <= 0 || !a.ut || n > 8192) return {};
+    const int uc = a.shift_complex ? 1 : 0;
+    if (uc && sizeof(T) != 4) return {};   // complex tiles are FP32 only (pool_tiles)
     int G = 0, NG = 0, cap = 0;
     for (int gc : {8, 4, 2}) {
         if (gc > gmax || nbu < 2 * gc) continue;
@@ -1472,7 +1509,11 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
         if (ng > 2) continue;
         int cc = 0;
         LHL_TRY(with_rs_shape<T>(gc, ng, [&](auto g, auto s) -> Status {
-            cc = rs_capacity<T, decltype(g)::value, decltype(s)::value>();
+            if constexpr (sizeof(T) == 4)
+                cc = uc ? rs_capacity<T, decltype(g)::value, decltype(s)::value, 1>()
+                        : rs_capacity<T, decltype(g)::value, decltype(s)::value, 0>();
+            else
+                cc = rs_capacity<T, decltype(g)::value, decltype(s)::value, 0>();
             return {};
         }));
         if (cc < gc) continue;
@@ -1499,9 +1540,17 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
         LHL_LAUNCH(h, "utrsm_reset", k_rs_reset, dim3((unsigned)((cg + 255) / 256)), dim3(256), 0, cnt, cg);
         LHL_TRY(with_rs_shape<T>(G, NG, [&](auto g, auto s) -> Status {
             constexpr int GC = decltype(g)::value, NGC = decltype(s)::value;
-            LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
-                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact, hand,
-                       cnt, hslot);
+            if constexpr (sizeof(T) == 4) {
+                if (uc) {
+                    LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC, 1>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
+                               (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact,
+                               hand, cnt, hslot);
+                    return {};
+                }
+            }
+            LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC, 0>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
+                       (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact, hand, cnt,
+                       hslot);
             return {};
         }));
     }
@@ -1559,7 +1608,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         }
         std::vector<UGroup> ug = build_groups(a.nshift, cb.data(), planes);
         ngroups = ug.size();
-        if (!a.shift_complex) {
+        if (a.ut) {
             const std::vector<UGroup> wgv = build_wgroups(a.nshift, cb.data(), planes);
             nwide = (int64_t)wgv.size();
             ug.insert(ug.end(), wgv.begin(), wgv.end());
@@ -1666,7 +1715,7 @@ Status uinv_typed(lhl_handle *h, const void *U, int64_t ustride, const void *rdi
     *uistride = uis;
     if (cplx)
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 1>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
-                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis, 0);
+                   ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis, ut);
     else
         LHL_LAUNCH(h, "uinv", (k_uinv<T, 0>), dim3((unsigned)nbu, (unsigned)nshift), dim3(64), 0, n, (const T *)U,
                    ustride, (const T *)rdiag, lstride, Uinv.as<T>(), uis, ut);
@@ -1818,7 +1867,9 @@ Status conv_device(lhl_handle *h, int it, int max_iter, double tol, int64_t nshi
 }
 
 bool pool_tiles(const lhl_handle *h, int64_t n, int is_complex) {
-    if (is_complex || h->cplx || n > 8192) return false;
+    // complex shifts on a real H: tiles in two planes, FP32 only (the FP64
+    // complex row-split kernel does not fit its registers)
+    if (h->cplx || n > 8192 || (is_complex && h->dtype != LHL_F32)) return false;
     const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
     return rs_shape_ok((int)((n + kT - 1) / kT), gmax);
 }

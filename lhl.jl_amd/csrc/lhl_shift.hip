@@ -783,10 +783,17 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
                                                               T *__restrict__ Ubase, int64_t ustride,
                                                               T *__restrict__ lbase, T *__restrict__ rdbase,
                                                               int64_t lstride, uint8_t *__restrict__ swbase,
-                                                              int64_t sstride, int *__restrict__ infobase) {
+                                                              int64_t sstride, int *__restrict__ infobase, int ut) {
 #pragma clang fp contract(off)
     const int s = blockIdx.x;
     const T sr = prm[s].sr, si = prm[s].si, tr = prm[s].tr, ti = prm[s].ti;
+    // U[k, j] (j >= k) at urow(k) + ucol(j) of each plane: packed rows, or
+    // 64x64 tiles (the batched pool; the spare slots past the tiles hold the
+    // idle threads' stores as with packed rows)
+    const int nbu = (n + 63) / 64;
+    constexpr int UV = utile_gw<T>();
+    auto urow = [&](int k) -> int64_t { return ut ? utile_row<UV>(nbu, k) : upack_off(n, k) - k; };
+    auto ucol = [&](int j) -> int64_t { return ut ? utile_col<UV>(j) : (int64_t)j; };
     const int64_t up = ustride / 2, lp = lstride / 2;
     T *Ur = Ubase + (int64_t)s * ustride, *Ui = Ur + up;
     T *dump = Ur + ustride - NT;      // spare tail of the imaginary plane
@@ -800,6 +807,9 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
     T *subd = reinterpret_cast<T *>(lds_dyn);     // subd[k] = H[k+1, k]
     for (int i = tid; i + 1 < n; i += NT) subd[i] = Ht[i + (int64_t)(i + 1) * ldh];
 
+    int64_t uc[NPER];   // this thread's columns' offsets within a row
+#pragma unroll
+    for (int m = 0; m < NPER; m++) uc[m] = ucol(min(tid + m * NT, n - 1));
     T rr[NPER], ri[NPER];
 #pragma unroll
     for (int m = 0; m < NPER; m++) {
@@ -833,14 +843,14 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
         cdiv_sel(sw_k ? ar : br, sw_k ? ai : bi, sw_k ? br : ar, sw_k ? bi : ai, qr, qi);
         const T lre = zero ? (T)0 : qr, lim = zero ? (T)0 : qi;
         info = (zero && info == 0) ? k + 1 : info;
-        const int64_t ro = upack_off(n, k) - k;
+        const int64_t ro = urow(k);
         const T ukr = sw_k ? br : ar, uki = sw_k ? bi : ai;
         if (tid < kWave) {   // wave 0 alone: row k's diagonal, reciprocal pivot, multiplier, swap flag
             T dr, di;
             cdiv_sel((T)1, (T)0, ukr, uki, dr, di);
             const bool t0 = tid == 0;
-            *(t0 ? Ur + ro + k : dump + tid) = ukr;
-            *(t0 ? Ui + ro + k : dump + tid) = uki;
+            *(t0 ? Ur + ro + ucol(k) : dump + tid) = ukr;
+            *(t0 ? Ui + ro + ucol(k) : dump + tid) = uki;
             *(t0 ? rdr + k : dump + tid) = dr;
             *(t0 ? rdi + k : dump + tid) = di;
             *(t0 ? lr_ + k : dump + tid) = lre;
@@ -871,8 +881,8 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
             const T xi = sw_k ? (first ? s2i : s1i) : (first ? n2i : n1i);
             rr[m] = active ? xr : xr0;
             ri[m] = active ? xi : xi0;
-            *(active ? Ur + ro + j : dump + tid) = ur;
-            *(active ? Ui + ro + j : dump + tid) = ui;
+            *(active ? Ur + ro + uc[m] : dump + tid) = ur;
+            *(active ? Ui + ro + uc[m] : dump + tid) = ui;
             *(first ? &slot[(k + 1) & 1][0] : &ldump[0][tid]) = xr;
             *(first ? &slot[(k + 1) & 1][1] : &ldump[1][tid]) = xi;
         }
@@ -895,7 +905,7 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
             T xr = 0, xi = 0;
 #pragma unroll
             for (int m = 0; m < NPER; m++) if (m == mm) { xr = rr[m]; xi = ri[m]; }
-            const int64_t o = upack_off(n, n - 1);
+            const int64_t o = urow(n - 1) + ucol(n - 1);
             Ur[o] = xr; Ui[o] = xi;
             T qr, qi;
             cdiv((T)1, (T)0, xr, xi, qr, qi);
@@ -910,14 +920,15 @@ __global__ void __launch_bounds__(NT) k_shift_cplx(const T *__restrict__ Ht, int
 template <typename T, int NPER, int NT>
 Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_t ldh, int n,
                     const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw, int64_t ss,
-                    int *info) {
+                    int *info, int ut = 0) {
     const size_t lds = sizeof(T) * (size_t)std::max(n, 1);   // the subdiagonal of H
     if (cplx) {
         auto fn = k_shift_cplx<T, NPER, NT>;
         LHL_TRY(allow_dynamic_lds(fn));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
-                   info);
+                   info, ut);
     } else {
+        if (ut) return {LHL_ERR_ARG, "lhl_shift: U in tiles is written by the panel kernel for real shifts"};
         auto fn = k_shift_real<T, NPER, NT>;
         LHL_TRY(allow_dynamic_lds(fn));
         LHL_LAUNCH(h, "shift", fn, dim3((unsigned)nshift), dim3(NT), lds, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss,
@@ -1029,8 +1040,8 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
                       const ShiftParam<T> *prm, T *U, int64_t us, T *l, T *rd, int64_t ls, uint8_t *sw,
                       int64_t ss, int *info, int ut = 0) {
     static_assert(sizeof(ShiftParam<T>) == 4 * sizeof(T), "ShiftParam is four T: (sr, si, tr, ti)");
-    if (ut && (cplx || h->cplx || n > 8192))
-        return {LHL_ERR_ARG, "lhl_shift: U in tiles is written by the real panel kernel only"};
+    if (ut && (h->cplx || n > 8192))
+        return {LHL_ERR_ARG, "lhl_shift: U in tiles is written for a real H up to n = 8192 only"};
     if (h->cplx)   // a complex H (a complex J's reduction)
         return zshift_launch<T>(h, nshift, reinterpret_cast<const T *>(prm), U, us, l, rd, ls, sw, ss, info);
     constexpr bool F32 = sizeof(T) == 4;
@@ -1078,12 +1089,12 @@ Status launch_shift_n(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int6
     }
     if ((size_t)n * sizeof(T) + 16 * 1024 > lds_limit())
         return {LHL_ERR_ARG, "lhl_shift: n too large for the LDS-resident subdiagonal of the shift kernel"};
-    if (n <= 512) LHL_TRY((launch_shift<T, 1, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-    else if (n <= 1024) LHL_TRY((launch_shift<T, 2, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-    else if (n <= 2048) LHL_TRY((launch_shift<T, 4, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-    else if (n <= 4096) LHL_TRY((launch_shift<T, 8, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-    else if (n <= 8192) LHL_TRY((launch_shift<T, 16, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
-    else if (n <= 16384) LHL_TRY((launch_shift<T, 16, 1024>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info)));
+    if (n <= 512) LHL_TRY((launch_shift<T, 1, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
+    else if (n <= 1024) LHL_TRY((launch_shift<T, 2, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
+    else if (n <= 2048) LHL_TRY((launch_shift<T, 4, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
+    else if (n <= 4096) LHL_TRY((launch_shift<T, 8, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
+    else if (n <= 8192) LHL_TRY((launch_shift<T, 16, 512>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
+    else if (n <= 16384) LHL_TRY((launch_shift<T, 16, 1024>(h, cplx, nshift, Ht, ldh, n, prm, U, us, l, rd, ls, sw, ss, info, ut)));
     else return {LHL_ERR_ARG, "lhl_shift: n above 16384 is not supported by the register-resident shift kernel"};
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -1143,7 +1154,10 @@ Status shift_pool_device(lhl_handle *h, ShiftPool &pool, int64_t nshift, const d
     pool.planes = is_complex ? 2 : 1;
     pool.ut = pool_tiles(h, n, is_complex) ? 1 : 0;
     const int64_t uplane = upack_plane(n);
-    pool.ustride = pool.ut ? utile_plane(n) : uplane * pool.planes;
+    // tiles: complex shifts keep kShiftThreads spare slots past each plane's
+    // tiles (the idle threads' stores of the one-step kernel)
+    const int64_t tplane = ((utile_plane(n) + (is_complex ? kShiftThreads : 0) + 63) / 64) * 64;
+    pool.ustride = (pool.ut ? tplane : uplane) * pool.planes;
     const int64_t lplane = ((n + 63) / 64) * 64;
     LHL_TRY(pool.U.ensure(es * (size_t)pool.ustride * (size_t)nshift));
     LHL_TRY(pool.l.ensure(es * (size_t)lplane * pool.planes * (size_t)nshift));

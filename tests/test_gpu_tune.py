@@ -127,6 +127,36 @@ def test_utrsm_split_solves(lhl_hip, monkeypatch, dtype):
                 assert err[split] <= (rtol if dtype == np.float64 else 3 * err[1] + 1e-6), (n, S, R, s, err)
 
 
+# complex shifts on a real J (FP32 factors: U in two planes of tiles and the
+# row-split U solve; FP64 keeps packed columns): against the one-workgroup-per-
+# 4-columns kernel and dense complex FP64 solves
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_utrsm_split_complex_shifts(lhl_hip, monkeypatch, dtype):
+    monkeypatch.setenv("LHL_TUNE", "0")
+    rtol = 5e-3 if dtype == np.float32 else 1e-11
+    for n, S, R in ((300, 3, 8), (1000, 2, 9), (4096, 2, 8)):
+        J = np.asfortranarray(randn(n + 11, n) / np.sqrt(n))
+        gam = np.linspace(0.05, 0.5, S) * (0.6 + 0.3j)
+        offs = np.arange(0, S * R + 1, R, dtype=np.int64)
+        rng = np.random.default_rng(n + 1)
+        B = np.asfortranarray(rng.standard_normal((n, S * R)) + 1j * rng.standard_normal((n, S * R)))
+        ws = lhl_hip.LHLWorkspace(n, dtype)
+        lhl_hip.lhl_(ws, J)
+        X = {}
+        for split in (1, -1):
+            say(f"case: utrsm split complex shifts {np.dtype(dtype).name} n={n} S={S} R={R} split={split}")
+            lhl_hip.tune_set(ws, "utrsm_split", split)
+            X[split], info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -gam, B, offs)
+            assert np.all(np.asarray(info) == 0)
+        for s in sorted({0, S - 1}):
+            c = slice(offs[s], offs[s + 1])
+            ref = np.linalg.solve(np.eye(n) - gam[s] * J, B[:, c])
+            den = np.abs(ref).max()
+            err = {split: np.abs(X[split][:, c] - ref).max() / den for split in (1, -1)}
+            assert err[1] <= rtol, (n, S, R, s, err)
+            assert err[-1] <= (rtol if dtype == np.float64 else 3 * err[1] + 1e-6), (n, S, R, s, err)
+
+
 def test_reduce_knobs_keep_pivots(lhl_hip, oracle, monkeypatch):
     monkeypatch.setenv("LHL_TUNE", "0")
     n = 600
