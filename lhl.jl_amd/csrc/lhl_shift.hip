@@ -366,10 +366,12 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // rows prefetched into registers it ran at the load latency over the
     // prefetch depth).  Per iteration parity: A (window_apply: H[k0+1+i,
     // c1+lane]), B (the next block's decisions: H[c1+1+i, c1+lane]) and, where
-    // windows are handed over two blocks ahead (FP32), C (block b-1 on window
-    // b+1: H[k0-KP+1+i, c1+lane]).  FP64 hands windows over one block ahead
-    // (its three tile pairs would not fit in LDS).
-    constexpr bool FAR = sizeof(T) == 4;
+    // windows are handed over two blocks ahead (FAR), C (block b-1 on window
+    // b+1: H[k0-KP+1+i, c1+lane]).  Windows are handed over one block ahead:
+    // FAR (two ahead, the panel wave applying block b-1 to the window itself)
+    // measured 2-5 % slower in FP32, and FP64's three tile pairs would not fit
+    // in LDS; it stays a compile-time switch.
+    constexpr bool FAR = false;
     constexpr int NTL = FAR ? 3 : 2;
     __shared__ __align__(16) T htile[2][NTL][KP][64];
     const int nsteps = n - 1;
@@ -723,7 +725,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                             if (j >= c2 && j < c2 + kPanel) win[bp][j - c2] = G::get(r[m], e);
                         }
                 }
-                // the window handed over: b+3 (two blocks ahead, FP32) or b+2
+                // the window handed over: b+2 (b+3 two blocks ahead, FAR)
                 const int c3 = FAR ? c2 + kPanel : c2;
                 if (GS > 1 && c3 < n && owns(c3 / kPanel)) {
                     int put = 0;
