@@ -713,9 +713,10 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                     }
                 }
                 // window b+2 = columns c2 .. c2+63 for this workgroup's panel
-                // wave, and with several workgroups per shift window b+3 for the
-                // others' (uncached slots, then a count of the values written
-                // once they have landed: they apply block b+1 to it themselves)
+                // wave, and with several workgroups per shift the same window
+                // for the others' (uncached slots, then a count of the values
+                // written once they have landed; with FAR, window b+3, to which
+                // they apply block b+1 themselves)
                 if (owns(c2 / kPanel)) {
 #pragma unroll
                     for (int m = MS; m < NP; m++)
