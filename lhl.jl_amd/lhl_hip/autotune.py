@@ -8,8 +8,9 @@ _lhl_shift_fused_min).  Here they are per-handle settings (lhl_tune_set):
   reduce_blocked_min, reduce_nb, reduce_rmin, reduce_switch   lhl(): whether and how the
       delayed-update panels run (width, rows per step workgroup, the step where the
       row-resident loop takes over)
-  lsweep_depth, shift_wide_min                                the batched solve: halvings of
-      the L sweeps' triangular products, the n from which the shift LU's bulk is wide
+  lsweep_depth, shift_wide_min, shift_split, utrsm_split      the batched solve: halvings of
+      the L sweeps' triangular products, the n from which the shift LU's bulk is wide, the
+      workgroups per shift LU, and the row-split U solve's workgroups per 16 columns
 
 Search: coordinate descent from the built-in defaults, one knob at a time over
 its candidate values, keeping a change only when it is faster by more than
@@ -45,6 +46,8 @@ REDUCE_CANDIDATES = {
 SOLVE_CANDIDATES = {
     "lsweep_depth": [0, 1, 2, 3],
     "shift_wide_min": [-1, 1],                     # default threshold, or wide at every size
+    "shift_split": [-1, 2, 8],                     # workgroups per shift LU (default 4)
+    "utrsm_split": [-1, 1, 4],                     # row-split U solve (default up to 8; 1: packed columns)
 }
 
 _cache = None
