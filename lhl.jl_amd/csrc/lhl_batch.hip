@@ -80,19 +80,21 @@ __global__ void __launch_bounds__(64) k_linv(int n, const T *__restrict__ A, int
 // x[perm[p], col] (+)= y_new[p] * d[perm[p]]: the scatter that ends a solve
 // (src:3092-3107), from the swept rows y_new' (rows 1.. of Yn) and row 0 of Y
 // (the sweep leaves y[0] as it is).  Internal column q = col*planes + plane.
+// Run as a gather: thread r takes p = iperm[r], so the read-modify-writes of x
+// are contiguous and only the reads of y are permuted.
 template <typename T, typename TX>
 __global__ void k_scatter(int n, const T *__restrict__ Yn, const T *__restrict__ Y, int64_t ldy, int planes,
-                          const int *__restrict__ perm, const double *__restrict__ scale, TX *__restrict__ x,
+                          const int *__restrict__ iperm, const double *__restrict__ scale, TX *__restrict__ x,
                           int64_t ldx, int64_t xplane, int accumulate, const int *__restrict__ cact) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
-    if (p >= n) return;
+    if (r >= n) return;
     const int col = q / planes, pl = q % planes;
     if (cact && !cact[col]) return;   // a converged column keeps its x
+    const int p = iperm[r];
     const T yn = p == 0 ? Y[(int64_t)q * ldy] : Yn[p + (int64_t)q * ldy];
-    const int dst_row = perm[p];
-    TX *dst = x + dst_row + (int64_t)col * ldx + pl * xplane;
-    const TX v = (TX)yn * (TX)scale[dst_row];
+    TX *dst = x + r + (int64_t)col * ldx + pl * xplane;
+    const TX v = (TX)yn * (TX)scale[r];
     if (accumulate) *dst += v;
     else *dst = v;
 }
@@ -1056,7 +1058,7 @@ __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ U
 // writes is the first of the next block, read once before any write.  Real
 // shifts: one chain per internal column.  Complex shifts: columns (re, im)
 // pair up into one complex chain.
-template <typename T, int UC>
+template <typename T, int UC, int SY>
 __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ lbase, int64_t lstride,
                                                   const uint8_t *__restrict__ swbase, int64_t sstride,
                                                   const UGroup *__restrict__ groups, T *__restrict__ Y, int64_t ldy,
@@ -1075,12 +1077,31 @@ __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ l
     T *yr = Y + (int64_t)(g.q0 + PL * wv) * ldy;
     T *yi = yr + ldy;                            // imaginary column (UC)
     const int k0 = lane * bs, k1 = min(k0 + bs, steps);   // this lane's steps [k0, k1)
+    // SY (real chains): the wave's column staged in LDS (read coalesced, then
+    // each lane walks its own block there; element k at k + k/64, so the lanes'
+    // blocks fall in different banks) and written back coalesced at the end
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    T *ys = reinterpret_cast<T *>(lds_dyn) + (int64_t)wv * (n + (n >> 6) + 1);
+    auto yget = [&](int k) -> T {
+        if constexpr (SY) return ys[k + (k >> 6)];
+        else return yr[k];
+    };
+    auto yset = [&](int k, T v) {
+        if constexpr (SY) ys[k + (k >> 6)] = v;
+        else yr[k] = v;
+    };
+    if constexpr (SY) {
+#pragma unroll 16
+        for (int i = lane; i < n; i += 64) ys[i + (i >> 6)] = yr[i];
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+    }
     // 1. the block's composed map (A, B), and the next block's first element
     T Ar = 1, Ai = 0, Br = 0, Bi = 0;
 #pragma unroll 4
     for (int k = k0; k < k1; k++) {
         const bool s = sw[k] != 0;
-        const T l = lr[k], b = yr[k + 1];
+        const T l = lr[k], b = yget(k + 1);
         if constexpr (UC) {
             const T lm = li[k], bim = yi[k + 1];
             if (s) {   // A unchanged, B -= l b
@@ -1101,7 +1122,7 @@ __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ l
         }
     }
     const int kb = min(k0 + bs, steps);          // y[kb]: read by this lane's last step
-    const T bndr = yr[kb], bndi = UC ? yi[kb] : (T)0;
+    const T bndr = yget(kb), bndi = UC ? yi[kb] : (T)0;
     // 2. inclusive scan of the maps in lane order: (A, B) <- (A A', A B' + B)
     //    with (A', B') the composition of the lanes before
 #pragma unroll
@@ -1120,7 +1141,7 @@ __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ l
         }
     }
     // the start value of this lane's block: the lanes before applied to y[0]
-    const T c0r = yr[0], c0i = UC ? yi[0] : (T)0;
+    const T c0r = yget(0), c0i = UC ? yi[0] : (T)0;
     T eAr = __shfl_up(Ar, 1, 64), eBr = __shfl_up(Br, 1, 64);
     T eAi = UC ? __shfl_up(Ai, 1, 64) : (T)0, eBi = UC ? __shfl_up(Bi, 1, 64) : (T)0;
     if (lane == 0) { eAr = 1; eAi = 0; eBr = 0; eBi = 0; }
@@ -1137,26 +1158,32 @@ __global__ void __launch_bounds__(256) k_fwd_scan(int n, const T *__restrict__ l
         const bool s = sw[k] != 0;
         const T l = lr[k];
         const bool last = k + 1 == kb;
-        const T b = last ? bndr : yr[k + 1];
+        const T b = last ? bndr : yget(k + 1);
         if constexpr (UC) {
             const T lm = li[k];
             const T bim = last ? bndi : yi[k + 1];
             const T xr = s ? b : cr, xi = s ? bim : ci;
-            yr[k] = xr;
+            yset(k, xr);
             yi[k] = xi;
             const T orr = s ? cr : b, oi = s ? ci : bim;
             cr = orr - (l * xr - lm * xi);
             ci = oi - (l * xi + lm * xr);
         } else {
             const T xk = s ? b : cr;
-            yr[k] = xk;
+            yset(k, xk);
             cr = (s ? cr : b) - l * xk;
         }
     }
     // y[n-1]: the carried value after the last step (the lane whose block holds step n-2)
     if (k0 < steps && k1 == steps) {
-        yr[n - 1] = cr;
+        yset(n - 1, cr);
         if constexpr (UC) yi[n - 1] = ci;
+    }
+    if constexpr (SY) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll 16
+        for (int i = lane; i < n; i += 64) yr[i] = ys[i + (i >> 6)];
     }
 }
 
@@ -1626,12 +1653,19 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     const int N1 = n - 1;
     // forward interchange chain in place on Yo: a scan per column
     if (n >= 2) {
-        if (a.shift_complex)
-            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 1>), dim3((unsigned)ngroups), dim3(256), 0, n,
+        // real chains: each wave's column staged in LDS where four fit
+        const size_t ylds = 4 * sizeof(T) * (size_t)(n + (n >> 6) + 1);
+        if (a.shift_complex) {
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 1, 0>), dim3((unsigned)ngroups), dim3(256), 0, n,
                        (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy, a.sact);
-        else
-            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0>), dim3((unsigned)ngroups), dim3(256), 0, n,
+        } else if (ylds <= lds_limit()) {
+            LHL_TRY(allow_dynamic_lds(k_fwd_scan<T, 0, 1>));
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0, 1>), dim3((unsigned)ngroups), dim3(256), ylds, n,
                        (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy, a.sact);
+        } else {
+            LHL_LAUNCH(h, "fwd_chain", (k_fwd_scan<T, 0, 0>), dim3((unsigned)ngroups), dim3(256), 0, n,
+                       (const T *)a.l, a.lstride, a.swp, a.sstride, (const UGroup *)dg, Yo, ldy, a.sact);
+        }
         LHL_HIP_TRY(hipGetLastError());
     }
     // x <- U^-1 y per shift (solution in Yw)
@@ -1698,7 +1732,7 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
                                    Yo + 1, ldy));
     }
     LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), dim3((unsigned)((n + 255) / 256), (unsigned)Q), dim3(256), 0, n,
-               (const T *)Yo, (const T *)Yw, ldy, planes, (const int *)h->perm.as<int>(),
+               (const T *)Yo, (const T *)Yw, ldy, planes, (const int *)h->iperm.as<int>(),
                (const double *)h->scale64.as<double>(), (TX *)a.xout, a.ldx, a.xplane, a.accumulate, a.cact);
     LHL_HIP_TRY(hipGetLastError());
     return {};
@@ -1936,7 +1970,7 @@ Status zapplyz_typed(lhl_handle *h, int inverse, void *x, int64_t ncol, int64_t 
         LHL_LAUNCH(h, "gather", (k_zstage<T, TX>), gq, dim3(256), 0, n, (const TX *)x, ldx, xplane, Yw, ldy, 1);
         LHL_TRY(zsweep_device<T>(h, 0, Yw, Yo, ldy, ncol));
         LHL_LAUNCH(h, "scatter", (k_scatter<T, TX>), gq, dim3(256), 0, n, (const T *)Yo, (const T *)Yo, ldy, 2,
-                   (const int *)h->perm.as<int>(), (const double *)h->scale64.as<double>(), (TX *)x, ldx, xplane, 0,
+                   (const int *)h->iperm.as<int>(), (const double *)h->scale64.as<double>(), (TX *)x, ldx, xplane, 0,
                    (const int *)nullptr);
     }
     LHL_HIP_TRY(hipGetLastError());
