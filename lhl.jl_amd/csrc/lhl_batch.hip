@@ -47,6 +47,31 @@ __global__ void k_gather(int n, const int *__restrict__ perm, const double *__re
     if (i == 0) Y2[(int64_t)q * ldy] = v;
 }
 
+// The same with one workgroup per internal column: the column of x read into
+// LDS in one coalesced pass, then permuted from there (the row-per-thread
+// form reads x at permuted rows straight from memory).
+template <typename T, typename TX>
+__global__ void __launch_bounds__(256) k_gather_col(int n, const int *__restrict__ perm,
+                                                    const double *__restrict__ iscale, const TX *__restrict__ x,
+                                                    int64_t ldx, int64_t xplane, int planes, T *__restrict__ Y,
+                                                    T *__restrict__ Y2, int64_t ldy, const int *__restrict__ cact) {
+    const int q = blockIdx.x;
+    const int col = q / planes, pl = q % planes;
+    if (cact && !cact[col]) return;   // a converged column (whole workgroup)
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    TX *xs = reinterpret_cast<TX *>(lds_dyn);
+    const TX *xc = x + (int64_t)col * ldx + pl * xplane;
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n; i += 256) xs[i] = xc[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int p = perm[i];
+        const T v = (T)(xs[p] * (TX)iscale[p]);
+        Y[i + (int64_t)q * ldy] = v;
+        if (i == 0) Y2[(int64_t)q * ldy] = v;
+    }
+}
+
 // Linv[b] = (L'[64b:64b+64, 64b:64b+64])^-1, column-major 64x64 per block: the
 // explicit inverse of each unit lower triangular diagonal block (the
 // reference forms its nb x nb panel inverse the same way, src:1942-1954).
@@ -1600,10 +1625,18 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
     if (a.phase != 2) {
         // y = P^-1 D^-1 x, then y' <- L'^-1 y' as a product with the explicit
         // inverse formed by lhl() (final rows in Yo)
-        const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
-        LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
-                   (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
-                   ldy, a.cact);
+        const size_t xlds = sizeof(TX) * (size_t)n;
+        if (xlds <= lds_limit() / 2) {   // a column of x in LDS
+            LHL_TRY(allow_dynamic_lds(k_gather_col<T, TX>));
+            LHL_LAUNCH(h, "gather", (k_gather_col<T, TX>), dim3((unsigned)Q), dim3(256), xlds, n,
+                       (const int *)h->perm.as<int>(), (const double *)h->iscale64.as<double>(), (const TX *)a.xin,
+                       a.ldx, a.xplane, planes, Yw, Yo, ldy, a.cact);
+        } else {
+            const dim3 gq((unsigned)((n + 255) / 256), (unsigned)Q);
+            LHL_LAUNCH(h, "gather", (k_gather<T, TX>), gq, dim3(256), 0, n, (const int *)h->perm.as<int>(),
+                       (const double *)h->iscale64.as<double>(), (const TX *)a.xin, a.ldx, a.xplane, planes, Yw, Yo,
+                       ldy, a.cact);
+        }
         if (h->cplx)   // a complex L: the blocked sweep (every column; the gather skipped none of Yo's rows)
             LHL_TRY(zsweep_device<T>(h, 1, Yw, Yo, ldy, a.ncol));
         else if (own_products(h, a.cact, a.full_products))
