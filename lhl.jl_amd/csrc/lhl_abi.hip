@@ -197,6 +197,7 @@ int lhl_destroy(lhl_handle *h) {
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_join) (void)hipEventDestroy(h->ev_join);
     if (h->hpin) (void)hipHostFree(h->hpin);
+    if (h->pin_arena) (void)hipHostFree(h->pin_arena);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return LHL_OK;

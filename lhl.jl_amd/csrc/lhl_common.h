@@ -425,6 +425,10 @@ struct lhl_handle {
     // shifts' parameters): each upload reads its own block, kept until the
     // next public call begins (every call ends by synchronizing its streams)
     std::vector<std::vector<unsigned char>> host_kept;
+    // pinned host arena for host_keep (asynchronous uploads stay asynchronous):
+    // bump-allocated within a call, sized between calls to the last call's use
+    unsigned char *pin_arena = nullptr;
+    size_t pin_cap = 0, pin_used = 0, pin_want = 0;
     lhl::DevBuf Linv;           // inverses of the 64x64 diagonal blocks of L'
     lhl::DevBuf Lfull;          // the explicit inverse of L' (solve), dense unit lower, leading dimension ldlf
     lhl::DevBuf Lp1;            // I + L' as a dense unit lower matrix (solve), leading dimension ldlf
@@ -470,17 +474,42 @@ struct ProfWork {
 
 // A host block of `bytes` bytes that stays valid until the handle's next
 // public call: the source of an asynchronous host-to-device copy.
+// From the handle's pinned arena when it has room (a copy from pinned memory
+// is truly asynchronous; one from pageable memory is staged by the runtime),
+// else pageable; the arena grows between calls to what the last call used.
 inline unsigned char *host_keep(lhl_handle *h, size_t bytes) {
+    const size_t b = ((std::max<size_t>(bytes, 1) + 255) / 256) * 256;
+    if (h->pin_arena && h->pin_used + b <= h->pin_cap) {
+        unsigned char *p = h->pin_arena + h->pin_used;
+        h->pin_used += b;
+        return p;
+    }
+    h->pin_want = std::max(h->pin_want, h->pin_used + b);
     h->host_kept.emplace_back(std::max<size_t>(bytes, 1));
     return h->host_kept.back().data();
 }
 // Drops the blocks of the previous call (which ended synchronized).
 // (The streams are drained first in case an earlier call left by an error.)
 inline void host_keep_reset(lhl_handle *h) {
-    if (h->host_kept.empty()) return;
+    if (h->host_kept.empty() && h->pin_used == 0) return;
     (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     h->host_kept.clear();
+    h->pin_used = 0;
+    if (h->pin_want > h->pin_cap) {   // the arena for the next calls
+        if (h->pin_arena) (void)hipHostFree(h->pin_arena);
+        h->pin_arena = nullptr;
+        h->pin_cap = 0;
+        void *p = nullptr;
+        const size_t want = std::max<size_t>(2 * h->pin_want, size_t(64) << 10);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) == hipSuccess) {
+            h->pin_arena = static_cast<unsigned char *>(p);
+            h->pin_cap = want;
+        } else {
+            (void)hipGetLastError();
+        }
+        h->pin_want = 0;
+    }
 }
 
 #define LHL_LAUNCH(h, name, kern, grid, block, lds, ...)                            \
