@@ -802,9 +802,13 @@ constexpr int kSplitSpinRs = 1 << 20;  // bound on the polls of one hand-off wai
 // over the lanes' column quadruples); a ring of RG units keeps RG - 1 in
 // flight, running on across blocks (U does not depend on x), so a block starts
 // with its first loads done.
+// (FP64 complex: 2, one slot only; the two planes' tiles, preloads and units
+// fill the registers)
 template <typename T, int NG, int UC> constexpr int rs_ring() {
-    return UC ? (sizeof(T) == 4 ? 8 : 4) : (sizeof(T) == 4 ? 16 : (NG >= 2 ? 4 : 8));
+    return UC ? (sizeof(T) == 4 ? 8 : 2) : (sizeof(T) == 4 ? 16 : (NG >= 2 ? 4 : 8));
 }
+// the most slots per thread of a shape: FP64 complex shifts one, else two
+template <typename T> constexpr int rs_max_slots(int uc) { return (uc && sizeof(T) == 8) ? 1 : 2; }
 
 template <typename T, int G, int NG, int UC>
 __global__ void __launch_bounds__(512) k_utrsm_rs(int n, const T *__restrict__ Ubase, int64_t ustride,
@@ -1521,10 +1525,10 @@ template <typename T, int G, int NG, int UC> int rs_capacity() {
 }
 
 // Whether some G <= gmax in {8, 4, 2} gives every workgroup two blocks or more
-// and at most two slots per thread (before any capacity check).
-static bool rs_shape_ok(int nbu, int gmax) {
+// and at most maxng slots per thread (before any capacity check).
+static bool rs_shape_ok(int nbu, int gmax, int maxng) {
     for (int gc : {8, 4, 2})
-        if (gc <= gmax && nbu >= 2 * gc && (nbu + 8 * gc - 1) / (8 * gc) <= 2) return true;
+        if (gc <= gmax && nbu >= 2 * gc && (nbu + 8 * gc - 1) / (8 * gc) <= maxng) return true;
     return false;
 }
 
@@ -1553,19 +1557,18 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
     const int nbu = (n + kT - 1) / kT;
     if (gmax < 2 || nwide <= 0 || !a.ut || n > 8192) return {};
     const int uc = a.shift_complex ? 1 : 0;
-    if (uc && sizeof(T) != 4) return {};   // complex tiles are FP32 only (pool_tiles)
     int G = 0, NG = 0, cap = 0;
     for (int gc : {8, 4, 2}) {
         if (gc > gmax || nbu < 2 * gc) continue;
-        const int ng = (nbu + 8 * gc - 1) / (8 * gc);   // slots per thread (at most 2)
-        if (ng > 2) continue;
+        const int ng = (nbu + 8 * gc - 1) / (8 * gc);   // slots per thread
+        if (ng > rs_max_slots<T>(uc)) continue;
         int cc = 0;
         LHL_TRY(with_rs_shape<T>(gc, ng, [&](auto g, auto s) -> Status {
-            if constexpr (sizeof(T) == 4)
-                cc = uc ? rs_capacity<T, decltype(g)::value, decltype(s)::value, 1>()
-                        : rs_capacity<T, decltype(g)::value, decltype(s)::value, 0>();
+            constexpr int GC = decltype(g)::value, NGC = decltype(s)::value;
+            if constexpr (sizeof(T) == 4 || NGC == 1)
+                cc = uc ? rs_capacity<T, GC, NGC, 1>() : rs_capacity<T, GC, NGC, 0>();
             else
-                cc = rs_capacity<T, decltype(g)::value, decltype(s)::value, 0>();
+                cc = rs_capacity<T, GC, NGC, 0>();
             return {};
         }));
         if (cc < gc) continue;
@@ -1592,7 +1595,7 @@ Status utrsm_rs(lhl_handle *h, const SolveArgs &a, int n, const UGroup *wg, int6
         LHL_LAUNCH(h, "utrsm_reset", k_rs_reset, dim3((unsigned)((cg + 255) / 256)), dim3(256), 0, cnt, cg);
         LHL_TRY(with_rs_shape<T>(G, NG, [&](auto g, auto s) -> Status {
             constexpr int GC = decltype(g)::value, NGC = decltype(s)::value;
-            if constexpr (sizeof(T) == 4) {
+            if constexpr (sizeof(T) == 4 || NGC == 1) {
                 if (uc) {
                     LHL_LAUNCH(h, "utrsm", (k_utrsm_rs<T, GC, NGC, 1>), dim3((unsigned)(cg * GC)), dim3(512), 0, n,
                                (const T *)a.U, a.ustride, (const T *)a.Uinv, uis, wg, (int)g0, Yin, Yf, ldy, a.sact,
@@ -1934,11 +1937,12 @@ Status conv_device(lhl_handle *h, int it, int max_iter, double tol, int64_t nshi
 }
 
 bool pool_tiles(const lhl_handle *h, int64_t n, int is_complex) {
-    // complex shifts on a real H: tiles in two planes, FP32 only (the FP64
-    // complex row-split kernel does not fit its registers)
-    if (h->cplx || n > 8192 || (is_complex && h->dtype != LHL_F32)) return false;
+    // complex shifts on a real H: tiles in two planes (FP64: where one slot
+    // per thread reaches every block, n <= 4096)
+    if (h->cplx || n > 8192) return false;
     const int gmax = h->tune_utrsm_split > 0 ? h->tune_utrsm_split : 8;
-    return rs_shape_ok((int)((n + kT - 1) / kT), gmax);
+    const int maxng = h->dtype == LHL_F64 ? rs_max_slots<double>(is_complex) : rs_max_slots<float>(is_complex);
+    return rs_shape_ok((int)((n + kT - 1) / kT), gmax, maxng);
 }
 
 Status uinv_device(lhl_handle *h, const void *U, int64_t ustride, const void *rdiag, int64_t lstride,

@@ -127,14 +127,14 @@ def test_utrsm_split_solves(lhl_hip, monkeypatch, dtype):
                 assert err[split] <= (rtol if dtype == np.float64 else 3 * err[1] + 1e-6), (n, S, R, s, err)
 
 
-# complex shifts on a real J (FP32 factors: U in two planes of tiles and the
-# row-split U solve; FP64 keeps packed columns): against the one-workgroup-per-
-# 4-columns kernel and dense complex FP64 solves
+# complex shifts on a real J (U in two planes of tiles and the row-split U
+# solve; FP64 up to n = 4096, packed columns beyond, as at 4160): against the
+# one-workgroup-per-4-columns kernel and dense complex FP64 solves
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_utrsm_split_complex_shifts(lhl_hip, monkeypatch, dtype):
     monkeypatch.setenv("LHL_TUNE", "0")
     rtol = 5e-3 if dtype == np.float32 else 1e-11
-    for n, S, R in ((300, 3, 8), (1000, 2, 9), (4096, 2, 8)):
+    for n, S, R in ((300, 3, 8), (1000, 2, 9), (4096, 2, 8), (4160, 1, 3)):
         J = np.asfortranarray(randn(n + 11, n) / np.sqrt(n))
         gam = np.linspace(0.05, 0.5, S) * (0.6 + 0.3j)
         offs = np.arange(0, S * R + 1, R, dtype=np.int64)
