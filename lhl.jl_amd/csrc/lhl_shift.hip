@@ -350,6 +350,13 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     bool failed = false;
     const T sigma = prm[s].sr, tau = prm[s].tr;
     T *U = Ubase + (int64_t)s * ustride;
+    // The panel wave's stores address U[k, c] as one buffer over this shift's
+    // U at byte urow(k) ES + ucol(c) ES: per block, lane L holds row k0 + L's
+    // offset (read from lane i at step i) and its column's; lanes that store
+    // nothing carry kBufNone, which stays out of range after the row offset is
+    // added (U of one shift is below 2 GiB: n <= 13440).
+    const BufRsrc uall = buf_rsrc(U, (int)min(ustride * (int64_t)ES, (int64_t)0x7fffffff));
+    auto urow_b = [&](int k) -> int { return (int)(urow(min(k, n - 1)) * ES); };
     T *lv = lbase + (int64_t)s * lstride;
     T *rd = rdbase + (int64_t)s * lstride;
     uint8_t *sw = swbase + (int64_t)s * sstride;
@@ -440,7 +447,10 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         constexpr bool FULL = decltype(fullc)::value;
         const T *tl = &htile[set][1][0][0];
         T uvec = 1;
-        const int ucl = ucol(k0 + lane), uclast = ucol(n - 1);
+        // lane L: row k0 + L and column k0 + L; lane i leaves the stores after
+        // step i (the diagonal), so every step stores columns c >= k
+        const int rowb = urow_b(k0 + lane);
+        int colb = (wr && lane < kPanel && k0 + lane <= n - 1) ? ucol(k0 + lane) * ES : kBufNone;
 #pragma unroll
         for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
 #pragma nounroll
@@ -449,7 +459,6 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
         for (int ii = 0; ii < PD; ii++) {
             const int i = ib + ii;
             if (FULL || i < nb) {
-                const int k = k0 + i;
                 const T h = hw[ii];
                 const T a = rdlane(p, i);
                 const T b = tau * rdlane(h, i);
@@ -459,7 +468,6 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T num = swk ? a : b, den = swk ? b : a;
                 const T q = num / den;
                 const T l = zero ? (T)0 : q;
-                info = (zero && info == 0) ? k + 1 : info;
                 // the elimination with the reference's operation order (src:2550-2572)
                 const bool first = lane == i + 1;
                 const T g = tau * h;
@@ -469,10 +477,9 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T u = swk ? (first ? g + sigma : g) : p;
                 const T pn = swk ? (first ? rs - ls : rs) : (first ? rn + sigma : rn);
                 // U[k, c] for window columns c = k0 + lane >= k (lane i: the
-                // diagonal, a or b); columns past n - 1 fall past the row
-                const int uk = ucol(k);
-                const BufRsrc ur = buf_rsrc(U + urow(k) + uk, (uclast - uk + 1) * ES);
-                buf_st(u, ur, (wr && lane >= i && lane < kPanel) ? (ucl - uk) * ES : kBufNone);
+                // diagonal, a or b); columns past n - 1 store nothing
+                buf_st(u, uall, colb + __builtin_amdgcn_readlane(rowb, i));
+                colb = lane == i ? kBufNone : colb;
                 lvec = lane == i ? l : lvec;
                 svec = lane == i ? (int)swk : svec;
                 uvec = lane == i ? (swk ? b : a) : uvec;
@@ -480,6 +487,10 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             }
             load_t(tl, i + PD, hw[ii]);   // past the block: a clamped row, unused
         }
+        // a zero pivot at step k (no swap and a == 0) is exactly a zero in
+        // uvec (a swap's pivot b has |b| > |a| >= 0): the first one is info
+        const unsigned long long zm = __ballot(lane < nb && uvec == (T)0);
+        if (info == 0 && zm != 0ull) info = k0 + __builtin_ctzll(zm) + 1;
         const int bp = (k0 / kPanel) & 1;
         if (lane < nb) {
             lsh[bp][lane] = lvec;
@@ -502,9 +513,9 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
     // diagonal (sigma) in column c1
     auto window_apply = [&](int k0, int set) __attribute__((always_inline)) {
         const int c1 = k0 + kPanel;
-        const int uc1 = ucol(c1);
-        const int off = (owns(c1 / kPanel) && lane < kPanel) ? (ucol(c1 + lane) - uc1) * ES : kBufNone;
-        const int ubytes = (ucol(n - 1) - uc1 + 1) * ES;
+        const int rowb = urow_b(k0 + lane);
+        const int colb =
+            (owns(c1 / kPanel) && lane < kPanel && c1 + lane <= n - 1) ? ucol(c1 + lane) * ES : kBufNone;
         const T *tl = &htile[set][0][0][0];
 #pragma unroll
         for (int i = 0; i < PD; i++) load_t(tl, i, hw[i]);
@@ -513,7 +524,6 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
 #pragma unroll
         for (int ii = 0; ii < PD; ii++) {
             const int i = ib + ii;
-            const int k = k0 + i;
             const T l = rdlane(lvec, i);
             const bool swk = __builtin_amdgcn_readlane(svec, i) != 0;
             const bool first = lane == 0 && i == kPanel - 1;
@@ -522,8 +532,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
             const T rn = g - l * p;
             const T u = swk ? (first ? g + sigma : g) : p;
             p = swk ? (first ? rs - l * sigma : rs) : (first ? rn + sigma : rn);
-            const BufRsrc ur = buf_rsrc(U + urow(k) + uc1, ubytes);
-            buf_st(u, ur, off);
+            buf_st(u, uall, colb + __builtin_amdgcn_readlane(rowb, i));
             load_t(tl, i + PD, hw[ii]);
         }
     };
@@ -681,6 +690,9 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                 const T lvv = lsh[bp][lane];
                 const int svv = ssh[bp][lane];
                 const int ubytes = (ucol(n - 1) - uc2 + 1) * ES;
+                // row k0 + L's byte offset in lane L, read per step from lane i
+                const int rowb = urow_b(k0 + lane);
+                const char *uc2p = reinterpret_cast<const char *>(U + uc2);
                 for (int i0 = 0; i0 < kPanel; i0 += R) {
 #pragma unroll
                     for (int q = 0; q < R; q++) {
@@ -688,7 +700,7 @@ __global__ void __launch_bounds__(64 + NTB) k_shift_panel(const T *__restrict__ 
                         const int k = k0 + i;
                         const T l = rdlane(lvv, i);
                         const int sk = __builtin_amdgcn_readlane(svv, i);
-                        const BufRsrc ur = buf_rsrc(U + urow(k) + uc2, ubytes);
+                        const BufRsrc ur = buf_rsrc(uc2p + __builtin_amdgcn_readlane(rowb, i), ubytes);
                         // values only in the two branches, then the stores: every
                         // path issues the same memory operations
                         T2 u[NP];
