@@ -954,10 +954,14 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
 
 // The pipelined panel kernel: one panel wave + kPanelBulk bulk threads, NP
 // column groups per bulk thread.  FP32: single columns, 63-step blocks, the
-// panel's rows 21 steps ahead and the bulk's 3; FP64: column pairs, 64-step
-// blocks (even, so no pair straddles the bulk's first column), 16 and 2 steps
-// ahead (measured at n = 4096: pairs take the FP64 shift LUs 3.74 -> 3.20 ms
-// and would cost FP32 10 %).  448 bulk threads are faster than 960 up to
+// panel's rows 21 steps ahead; FP64: column pairs, 64-step blocks (even, so
+// no pair straddles the bulk's first column), the panel's rows 16 steps ahead
+// (measured at n = 4096: pairs take the FP64 shift LUs 3.74 -> 3.20 ms and
+// would cost FP32 10 %).  The bulk's rows of H are prefetched 9 (FP32) or 8
+// (FP64) steps ahead where the registers hold them (448 threads, up to 6 / 3
+// column groups each), else 3 / 2: at n = 4096 with 64 shifts split 4 ways,
+// FP32 1.71 -> 1.49 ms and FP64 2.50 -> 2.06 ms per launch against 3 / 2
+// (21 / 16 steps measured no faster and spill).  448 bulk threads are faster than 960 up to
 // n = 4096 (FP64 3.19 vs 3.74 ms, FP32 2.02 vs 2.25 ms); beyond 448 threads'
 // reach (FP64 4480, FP32 8512) the panel kernel takes 960, up to n = 13440
 // (64 shifts at n = 8192 FP64: 10.6 ms against 18.2 ms in the one-step
@@ -968,9 +972,13 @@ constexpr int kPanelBulk = 448;
 constexpr int kPanelBulkWide = 960;
 template <typename T> constexpr int panel_w() { return sizeof(T) == 8 ? 2 : 1; }
 template <typename T> constexpr int panel_kp() { return sizeof(T) == 8 ? 64 : 63; }
+template <typename T, int NP, int NTB> constexpr int panel_r() {
+    if constexpr (sizeof(T) == 8) return (NTB == kPanelBulk && NP <= 3) ? 8 : 2;
+    else return (NTB == kPanelBulk && NP <= 6) ? 9 : 3;
+}
 template <typename T, int NP, int NTB> constexpr auto panel_fn() {
     constexpr bool F64 = sizeof(T) == 8;
-    return k_shift_panel<T, NP, NTB, F64 ? 16 : 21, F64 ? 2 : 3, panel_w<T>(), F64 ? 64 : 63>;
+    return k_shift_panel<T, NP, NTB, F64 ? 16 : 21, panel_r<T, NP, NTB>(), panel_w<T>(), F64 ? 64 : 63>;
 }
 
 // Workgroups of one panel-kernel shape resident on the whole device at once
