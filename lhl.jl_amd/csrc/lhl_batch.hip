@@ -1444,6 +1444,20 @@ Status blas_gemm_lower(lhl_handle *h, const char *name, int m, int ncol, const T
     return {};
 }
 
+// B <- L^-1 B for the unit lower-triangular L (dense, m x m) by substitution
+// (rocBLAS TRSM: forward substitution over blocks, backward stable, unlike
+// the product with an explicit inverse, whose error grows with cond(L)).
+template <typename T>
+static rocblas_status trsm_unit_lower(rocblas_handle b, int m, int ncol, const T *L, int64_t ldl, T *B, int64_t ldb) {
+    const T one = 1;
+    if constexpr (std::is_same<T, double>::value)
+        return rocblas_dtrsm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit,
+                             m, ncol, &one, L, (int)ldl, B, (int)ldb);
+    else
+        return rocblas_strsm(b, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit,
+                             m, ncol, &one, L, (int)ldl, B, (int)ldb);
+}
+
 // Dense unit lower-triangular m x m matrix in dst: dst(i, c) = src(i, c) below
 // the diagonal, 1 on it, 0 above.  src == dst fixes the diagonal and upper part
 // in place.
@@ -1645,7 +1659,21 @@ Status solve_batch_typed(lhl_handle *h, const SolveArgs &a) {
         else if (own_products(h, a.cact, a.full_products))
             LHL_TRY(mgemm(h, "trsm", 1, n - 1, Q, n - 1, (const T *)h->Lfull.as<T>(), h->ldlf, (const T *)Yw + 1, ldy,
                           Yo + 1, ldy, a.cact, planes));
-        else
+        else if (a.nshift == 1 && !a.col_begin && !a.groups && n >= 2) {
+            // one shift's solve (the drop-in lhl_ldiv! and lhl_refine!
+            // corrections): forward substitution with I + L', as the reference
+            // sweeps (src:2148-2195); batches keep the product with the
+            // explicit inverse, whose extra error their refinement absorbs
+            LHL_TRY(lp1_typed<T>(h));
+            LHL_HIP_TRY(hipMemcpy2DAsync(Yo + 1, sizeof(T) * (size_t)ldy, (const T *)Yw + 1, sizeof(T) * (size_t)ldy,
+                                         sizeof(T) * (size_t)(n - 1), (size_t)Q, hipMemcpyDeviceToDevice, h->stream));
+            rocblas_handle b;
+            LHL_TRY(blas_handle(h, &b));
+            ProfScope ps(h, "trsm");
+            if (trsm_unit_lower<T>(b, n - 1, Q, (const T *)h->Lp1.as<T>(), h->ldlf, Yo + 1, ldy) !=
+                rocblas_status_success)
+                return {LHL_ERR_HIP, "rocBLAS TRSM of the solve failed"};
+        } else
             LHL_TRY(blas_gemm_lower<T>(h, "trsm", n - 1, Q, (const T *)h->Lfull.as<T>(), h->ldlf,
                                        (const T *)Yw + 1, ldy, Yo + 1, ldy));
         if (a.phase == 1) {

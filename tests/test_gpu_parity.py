@@ -171,8 +171,15 @@ def test_shifted_solves(lhl_hip, oracle, n):
         x = lhl_hip.lhl_ldiv_(b.copy(), ws)
         assert relerr(x, np.linalg.solve(W, b)) <= 1e-9
         xo = oracle.ldiv(red, oracle.shift(red.Ht, s, t), b)
-        assert relerr(x, xo) <= 1e-12 * max(n, 1) * max(1.0, np.linalg.cond(W) / 1e3)
-        assert bwd(W, x, b) < 50 * np.finfo(float).eps * max(1, n / 100)
+        # two backward-stable solves of W differ by O(cond(W) eps): 1e-12 n, or
+        # 20 cond(W) eps where W is that ill-conditioned
+        eps = np.finfo(float).eps
+        cw = np.linalg.cond(W)
+        e, bw = relerr(x, xo), bwd(W, x, b)
+        say(f"  shift ({s}, {t}): vs oracle {e:.2e} (cond(W) {cw:.1e}), bwd {bw / eps:.1f} eps")
+        assert e <= max(1e-12 * max(n, 1), 20 * eps * cw)
+        # test/core_tests.jl:54: 50 eps up to n = 500
+        assert bw < 50 * eps * (1 if n <= 500 else n / 100)
 
 
 @pytest.mark.parametrize("n", [3, 40, 129, 300])
@@ -403,6 +410,7 @@ def test_batched_two_chunks(lhl_hip, monkeypatch):
     for s in (0, 1, 64, 65, 128, 129):
         lhl_hip.lhl_shift_(ws, 1, -gam[s])
         x = lhl_hip.lhl_ldiv_(B[:, s].copy(), ws)
+        say(f"  shift {s}: batched vs single {relerr(X[:, s], x):.2e}")
         assert relerr(X[:, s], x) <= 1e-13
 
 
@@ -445,6 +453,7 @@ def test_batched_matches_single(lhl_hip):
         lhl_hip.lhl_shift_(ws, 1, -gam[s])
         for c in range(offs[s], offs[s + 1]):
             x = lhl_hip.lhl_ldiv_(B[:, c].copy(), ws)
+            say(f"  column {c}: batched vs single {relerr(X[:, c], x):.2e}")
             assert relerr(X[:, c], x) <= 1e-13
     # complex shifts s = i*omega on a real reduction, tau = -1
     om = np.logspace(-2, 2, 4)
