@@ -630,16 +630,20 @@ def test_handles_on_host_threads(lhl_hip, dtype):
         n, S = sizes[i], 5
         ws = lhl_hip.lhl(mats[i], dtype=dtype)
         B = np.asfortranarray(np.random.default_rng(i).standard_normal((n, 2 * S)))
-        X, info, _ = lhl_hip.shift_solve_batched(ws, np.ones(S), -np.logspace(-4, -1, S), B,
-                                                 np.arange(0, 2 * S + 1, 2), precision="refine64", tol=1e-12,
-                                                 max_iter=30)
+        X, info, iters = lhl_hip.shift_solve_batched(ws, np.ones(S), -np.logspace(-4, -1, S), B,
+                                                     np.arange(0, 2 * S + 1, 2), precision="refine64", tol=1e-12,
+                                                     max_iter=30)
         assert np.all(info == 0)
-        return ws.ipiv.copy(), X
+        return ws.ipiv.copy(), X, iters
 
     one = [run(i) for i in range(len(sizes))]
     with ThreadPoolExecutor(3) as ex:
         many = list(ex.map(run, range(len(sizes))))
-    for (p1, x1), (p2, x2) in zip(one, many):
+    for n, (p1, x1, i1), (p2, x2, i2) in zip(sizes, one, many):
+        for name, x, it in (("sequential", x1, i1), ("threaded", x2, i2)):
+            bad = np.flatnonzero(~np.all(np.isfinite(x), axis=0))
+            if bad.size:
+                say(f"  n={n} {name}: non-finite columns {bad.tolist()}, refinement rounds per shift {it.tolist()}")
         assert np.array_equal(p1, p2)
         assert relerr(x2, x1) <= 1e-13
 
