@@ -225,7 +225,12 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
     const T *U = Ubase + (int64_t)s * ustride;
     const T *rd = rdbase + (int64_t)s * lstride;
     const int64_t up = ustride / 2, lp = lstride / 2;
-    __shared__ T Us[UP][kT][kT + 1];
+    // the block's strict upper triangle, packed by rows (half the LDS of a
+    // square block: twice the workgroups per CU in FP64); every read is a
+    // broadcast of one element to the whole wave
+    constexpr int NUP = kT * (kT - 1) / 2;
+    __shared__ T Us[UP][NUP];
+    auto upk = [](int i, int j) -> int { return i * kT - (i * (i + 1)) / 2 + (j - i - 1); };   // j > i
     __shared__ T ds[UP][kT];
     // the block's strict upper triangle: lane c loads column c of every row,
     // all loads issued before the first is used (a select on each value right
@@ -238,8 +243,10 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
                 const bool ok = r < nb && c < nb && c > r;
                 const int rr = ok ? r0 + r : 0, cc = ok ? r0 + c : 0;
                 const int64_t o = ut ? utile_row<UV>((n + 63) / 64, rr) + utile_col<UV>(cc) : upack_off(n, rr) - rr + cc;
-                Us[0][r][c] = ok ? U[o] : (T)0;
-                Us[UP - 1][r][c] = ok ? U[o + up] : (T)0;
+                if (c > r) {
+                    Us[0][upk(r, c)] = ok ? U[o] : (T)0;
+                    Us[UP - 1][upk(r, c)] = ok ? U[o + up] : (T)0;
+                }
             }
         } else {
             T v[kT];
@@ -251,7 +258,8 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
                 v[r] = U[o];
             }
 #pragma unroll
-            for (int r = 0; r < kT; r++) Us[0][r][c] = (r < nb && c < nb && c > r) ? v[r] : (T)0;
+            for (int r = 0; r < kT; r++)
+                if (c > r) Us[0][upk(r, c)] = (r < nb && c < nb) ? v[r] : (T)0;
         }
     }
     {
@@ -271,11 +279,11 @@ __global__ void __launch_bounds__(64) k_uinv(int n, const T *__restrict__ Ubase,
 #pragma unroll
         for (int j = i + 1; j < kT; j++) {
             const int q = (j - i - 1) % NPS;
-            pr[q] = fma_t(-Us[0][i][j], xr[j], pr[q]);
+            pr[q] = fma_t(-Us[0][upk(i, j)], xr[j], pr[q]);
             if constexpr (UC) {
-                pr[q] = fma_t(Us[UP - 1][i][j], xi[j], pr[q]);
-                pim[q] = fma_t(-Us[0][i][j], xi[j], pim[q]);
-                pim[q] = fma_t(-Us[UP - 1][i][j], xr[j], pim[q]);
+                pr[q] = fma_t(Us[UP - 1][upk(i, j)], xi[j], pr[q]);
+                pim[q] = fma_t(-Us[0][upk(i, j)], xi[j], pim[q]);
+                pim[q] = fma_t(-Us[UP - 1][upk(i, j)], xr[j], pim[q]);
             }
         }
         const T ar = (pr[0] + pr[1]) + (pr[2] + pr[3]), ai = (pim[0] + pim[1]) + (pim[2] + pim[3]);
