@@ -961,8 +961,9 @@ Status launch_shift(lhl_handle *h, int cplx, int64_t nshift, const T *Ht, int64_
 // (FP64) steps ahead where the registers hold them (448 threads, up to 6 / 3
 // column groups each), else 3 / 2: at n = 4096 with 64 shifts split 4 ways,
 // FP32 1.71 -> 1.49 ms and FP64 2.50 -> 2.06 ms per launch against 3 / 2
-// (21 / 16 steps measured no faster and spill).  448 bulk threads are faster than 960 up to
-// n = 4096 (FP64 3.19 vs 3.74 ms, FP32 2.02 vs 2.25 ms); beyond 448 threads'
+// (21 / 16 steps measured no faster and spill).  448 bulk threads are faster
+// than 960 up to n = 4096 (FP64 3.19 vs 3.74 ms, FP32 2.02 vs 2.25 ms, one
+// workgroup per shift; split 4 ways, 1.49 vs 1.68 ms F32); beyond 448 threads'
 // reach (FP64 4480, FP32 8512) the panel kernel takes 960, up to n = 13440
 // (64 shifts at n = 8192 FP64: 10.6 ms against 18.2 ms in the one-step
 // kernel; 32 at n = 13440: FP32 18.6 vs 55.0 ms, FP64 32.8 vs 163.4 ms).
